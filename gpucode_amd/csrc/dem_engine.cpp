@@ -1,0 +1,654 @@
+// dem_engine.cpp -- the step driver and state import/export of the engine.
+//
+// The reference's time loop (main.cpp:425-623) syncs with the device on
+// every step to read the rebuild flag (main.cpp:470).  dem_step launches a
+// chunk of fused steps back to back instead; a step kernel that finds a
+// rebuild request from an earlier step of the chunk exits at once, and the
+// host rebuilds and re-runs the skipped steps after reading the request
+// once per chunk.
+#include <limits.h>
+#include <string.h>
+
+#include <vector>
+
+#include "dem_host.h"
+#include "dem_launch.h"
+
+static const int kChunk = 64;
+
+static StepArgs step_args(dem_ctx *c, bool fused) {
+    StepArgs A;
+    memset(&A, 0, sizeof(A));
+    if (fused) {
+        A.in = c->buf(c->cur, c->Lidx);
+        A.out = c->buf(1 - c->cur, 1 - c->Lidx);
+    } else {
+        // separate kernels: in place, lqdCnt_Old = lq[1-Lidx], lqdCnt = lq[Lidx]
+        A.in = c->buf(c->cur, 1 - c->Lidx);
+        A.out = c->buf(c->cur, c->Lidx);
+    }
+    A.rad = c->rad[c->pcur];
+    A.pmove = c->pmove;
+    A.nbr = c->nbr[c->hcur];
+    A.nnb = c->nnb[c->hcur];
+    A.h = c->hp[c->hcur].ptr();
+    A.wl = c->wl[c->hcur];
+    A.nwl = c->nwl[c->hcur];
+    A.hw = c->hw[c->hcur].ptr();
+    A.Fx = c->F[0]; A.Fy = c->F[1]; A.Fz = c->F[2];
+    A.Mx = c->F[3]; A.My = c->F[4]; A.Mz = c->F[5]; A.Mr = c->F[6];
+    A.stop_at = c->stop_at;
+    A.cap = c->cap;
+    A.total = c->total;
+    A.step = 0;
+    A.store_forces = fused ? c->store_forces : 1;
+    A.Tw = c->Tw;
+    return A;
+}
+
+static int check_overflow(dem_ctx *c) {
+    int ov = 0;
+    HIPCHK(hipMemcpyAsync(&ov, c->overflow, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (ov > 0)
+        return dem_fail(DEM_ERR_OVERFLOW,
+                        "a neighbour list needed %d slots, more than its size level allows", ov);
+    return DEM_OK;
+}
+
+// ---------------------------------------------------------------------
+// state import / export
+// ---------------------------------------------------------------------
+static void reset_lists(dem_ctx *c) {
+    c->has_lists = false;
+    c->saved = false;
+    c->old_total = 0;
+    c->rebuild_pending = true;
+    c->forces_valid = false;
+}
+
+static int upload_particles(dem_ctx *c, int off, int n, const double *pos, const double *vel,
+                            const double *angv) {
+    if (n <= 0) return DEM_OK;
+    double *tmp = nullptr;
+    HIPCHK(hipMalloc((void **)&tmp, (size_t)n * 3 * sizeof(double)));
+    const double *src[3] = {pos, vel, angv};
+    int rc = DEM_OK;
+    for (int q = 0; q < 3 && rc == DEM_OK; ++q) {
+        double **dst = &c->st[c->cur][3 * q];
+        if (src[q]) {
+            hipError_t e = hipMemcpyAsync(tmp, src[q], (size_t)n * 3 * sizeof(double),
+                                          hipMemcpyHostToDevice, c->stream);
+            if (e != hipSuccess) rc = dem_fail(DEM_ERR_HIP, "upload: %s", hipGetErrorString(e));
+            launch_scatter3(tmp, dst[0], dst[1], dst[2], n, off, c->stream);
+        } else {
+            for (int d = 0; d < 3; ++d) launch_fill_f64(dst[d] + off, 0.0, n, c->stream);
+        }
+    }
+    // ids off..off+n-1 in place, with their radii and levels
+    std::vector<int> ids(n), lv(n);
+    std::vector<double> r(n);
+    for (int k = 0; k < n; ++k) {
+        ids[k] = off + k;
+        lv[k] = c->level_h[off + k];
+        r[k] = c->rad_h[off + k];
+    }
+    if (rc == DEM_OK) {
+        hipError_t e = hipMemcpyAsync(c->id[c->pcur] + off, ids.data(), n * sizeof(int),
+                                      hipMemcpyHostToDevice, c->stream);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(c->level[c->pcur] + off, lv.data(), n * sizeof(int),
+                               hipMemcpyHostToDevice, c->stream);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(c->rad[c->pcur] + off, r.data(), n * sizeof(double),
+                               hipMemcpyHostToDevice, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) rc = dem_fail(DEM_ERR_HIP, "upload: %s", hipGetErrorString(e));
+    }
+    launch_fill_i32(c->lq[0] + off, 0, n, c->stream);
+    launch_fill_i32(c->lq[1] + off, 0, n, c->stream);
+    launch_fill_f64(c->pmove + off, 0.0, n, c->stream);
+    launch_fill_i32(c->nnb[c->hcur] + off, 0, n, c->stream);
+    launch_fill_i32(c->nwl[c->hcur] + off, 0, n, c->stream);
+    hipError_t e = hipStreamSynchronize(c->stream);
+    (void)hipFree(tmp);
+    if (rc == DEM_OK && e != hipSuccess) rc = dem_fail(DEM_ERR_HIP, "upload: %s", hipGetErrorString(e));
+    return rc;
+}
+
+extern "C" int dem_set_particles(dem_ctx *c, int total, const double *pos, const double *vel,
+                                 const double *angv) {
+    if (!c || !pos || total < 0 || total > c->cap) return dem_fail(DEM_ERR_ARG, "bad particle count");
+    reset_lists(c);
+    c->total = total;
+    return upload_particles(c, 0, total, pos, vel, angv);
+}
+
+extern "C" int dem_feed(dem_ctx *c, int n, const double *pos) {
+    if (!c || n < 0 || !pos || c->total + n > c->cap) return dem_fail(DEM_ERR_ARG, "feed exceeds NP");
+    int off = c->total;
+    int rc = upload_particles(c, off, n, pos, nullptr, nullptr);
+    if (rc) return rc;
+    c->total += n;
+    c->rebuild_pending = true;  // FeedFlag (main.cpp:442, 473)
+    return DEM_OK;
+}
+
+extern "C" int dem_total(dem_ctx *c) { return c ? c->total : 0; }
+
+template <class T>
+static int download(dem_ctx *c, T *host, const T *dev, size_t n) {
+    if (!host || n == 0) return DEM_OK;
+    HIPCHK(hipMemcpyAsync(host, dev, n * sizeof(T), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return DEM_OK;
+}
+
+// gathers a sorted-order device array into original order on the host
+static int get3(dem_ctx *c, double *host, double *const *src) {
+    if (!host) return DEM_OK;
+    double *tmp = nullptr;
+    size_t n = (size_t)c->total;
+    HIPCHK(hipMalloc((void **)&tmp, (n ? n : 1) * 3 * sizeof(double)));
+    launch_gather3(c->id[c->pcur], src[0], src[1], src[2], tmp, c->total, c->stream);
+    int rc = download(c, host, tmp, n * 3);
+    (void)hipFree(tmp);
+    return rc;
+}
+
+static int get1(dem_ctx *c, double *host, const double *src) {
+    if (!host) return DEM_OK;
+    double *tmp = nullptr;
+    size_t n = (size_t)c->total;
+    HIPCHK(hipMalloc((void **)&tmp, (n ? n : 1) * sizeof(double)));
+    launch_gather1(c->id[c->pcur], src, tmp, c->total, c->stream);
+    int rc = download(c, host, tmp, n);
+    (void)hipFree(tmp);
+    return rc;
+}
+
+static int get1i(dem_ctx *c, int *host, const int *src) {
+    if (!host) return DEM_OK;
+    int *tmp = nullptr;
+    size_t n = (size_t)c->total;
+    HIPCHK(hipMalloc((void **)&tmp, (n ? n : 1) * sizeof(int)));
+    launch_gather1i(c->id[c->pcur], src, tmp, c->total, c->stream);
+    int rc = download(c, host, tmp, n);
+    (void)hipFree(tmp);
+    return rc;
+}
+
+extern "C" int dem_get_particles(dem_ctx *c, double *pos, double *vel, double *angv) {
+    if (!c) return dem_fail(DEM_ERR_ARG, "null context");
+    int rc;
+    if ((rc = get3(c, pos, &c->st[c->cur][0])) || (rc = get3(c, vel, &c->st[c->cur][3])) ||
+        (rc = get3(c, angv, &c->st[c->cur][6])))
+        return rc;
+    return DEM_OK;
+}
+
+extern "C" int dem_get_forces(dem_ctx *c, double *F, double *M, double *Mr) {
+    if (!c) return dem_fail(DEM_ERR_ARG, "null context");
+    if (!c->forces_valid) return dem_fail(DEM_ERR_STATE, "no forces stored for the current state");
+    int rc;
+    if ((rc = get3(c, F, &c->F[0])) || (rc = get3(c, M, &c->F[3])) || (rc = get1(c, Mr, c->F[6])))
+        return rc;
+    return DEM_OK;
+}
+
+extern "C" int dem_set_store_forces(dem_ctx *c, int on) {
+    if (!c) return dem_fail(DEM_ERR_ARG, "null context");
+    c->store_forces = on ? 1 : 0;
+    return DEM_OK;
+}
+
+extern "C" int dem_get_pmove(dem_ctx *c, double *pmove) {
+    if (!c) return dem_fail(DEM_ERR_ARG, "null context");
+    return get1(c, pmove, c->pmove);
+}
+
+extern "C" int dem_get_lqdcnt(dem_ctx *c, int *lqdCnt) {
+    if (!c) return dem_fail(DEM_ERR_ARG, "null context");
+    return get1i(c, lqdCnt, c->lq[c->Lidx]);
+}
+
+// sorted-order index -> original id, on the host
+static int host_ids(dem_ctx *c, std::vector<int> &ids) {
+    ids.resize(c->total);
+    return download(c, ids.data(), c->id[c->pcur], (size_t)c->total);
+}
+
+extern "C" int dem_get_nebor(dem_ctx *c, int *num, int *list, int maxK) {
+    if (!c || !num || !list || maxK <= 0) return dem_fail(DEM_ERR_ARG, "bad argument");
+    const int n = c->total;
+    std::vector<int> ids, cnt(n), nb((size_t)c->K * (n ? n : 1));
+    int rc;
+    if ((rc = host_ids(c, ids)) || (rc = download(c, cnt.data(), c->nnb[c->hcur], (size_t)n)))
+        return rc;
+    for (int k = 0; k < c->K; ++k)
+        if ((rc = download(c, nb.data() + (size_t)k * n, c->nbr[c->hcur] + (size_t)k * c->cap, (size_t)n)))
+            return rc;
+    int bad = 0;
+    for (int i = 0; i < n; ++i) {
+        int o = ids[i];
+        num[o] = cnt[i];
+        if (cnt[i] > maxK) bad = 1;
+        for (int k = 0; k < maxK; ++k)
+            list[(size_t)o * maxK + k] = k < cnt[i] ? ids[nb[(size_t)k * n + i]] : -1;
+    }
+    return bad ? dem_fail(DEM_ERR_ARG, "maxK smaller than a list") : DEM_OK;
+}
+
+static int get_hist(dem_ctx *c, const DevHist &h, const int *dcnt, int slots, int maxK,
+                    double *dispt, double *nDsp, double *nDsp_mx, double *plstRad,
+                    double *plstDfm, int *stage, int *lqdBrg) {
+    const int n = c->total;
+    std::vector<int> ids, cnt(n), fl(n);
+    std::vector<double> a(n);
+    int rc;
+    if ((rc = host_ids(c, ids)) || (rc = download(c, cnt.data(), dcnt, (size_t)n))) return rc;
+    double *dst[7] = {nullptr, nullptr, nullptr, nDsp, nDsp_mx, plstRad, plstDfm};
+    const double *src[7] = {h.dx, h.dy, h.dz, h.nd, h.ndm, h.prad, h.pdfm};
+    for (int k = 0; k < maxK; ++k) {
+        bool live = k < slots;
+        for (int q = 0; q < 7; ++q) {
+            if (live && (rc = download(c, a.data(), src[q] + (size_t)k * c->cap, (size_t)n))) return rc;
+            for (int i = 0; i < n; ++i) {
+                size_t o = (size_t)ids[i] * maxK + k;
+                double v = (live && k < cnt[i]) ? a[i] : 0.0;
+                if (q < 3) {
+                    if (dispt) dispt[3 * o + q] = v;
+                } else if (dst[q]) {
+                    dst[q][o] = v;
+                }
+            }
+        }
+        if (live && (rc = download(c, fl.data(), h.flags + (size_t)k * c->cap, (size_t)n))) return rc;
+        for (int i = 0; i < n; ++i) {
+            size_t o = (size_t)ids[i] * maxK + k;
+            int f = (live && k < cnt[i]) ? fl[i] : 0;
+            if (stage) stage[o] = f & 0xff;
+            if (lqdBrg) lqdBrg[o] = (f >> 8) & 1;
+        }
+    }
+    return DEM_OK;
+}
+
+extern "C" int dem_get_hist_pp(dem_ctx *c, int maxK, double *dispt, double *nDsp, double *nDsp_mx,
+                               double *plstRad, double *plstDfm, int *stage, int *lqdBrg) {
+    if (!c || maxK <= 0) return dem_fail(DEM_ERR_ARG, "bad argument");
+    return get_hist(c, c->hp[c->hcur], c->nnb[c->hcur], c->K, maxK, dispt, nDsp, nDsp_mx, plstRad,
+                    plstDfm, stage, lqdBrg);
+}
+
+extern "C" int dem_get_walls(dem_ctx *c, int *num, int *list) {
+    if (!c || !num || !list) return dem_fail(DEM_ERR_ARG, "bad argument");
+    const int n = c->total;
+    std::vector<int> ids, cnt(n), w((size_t)DEM_WALLS * (n ? n : 1));
+    int rc;
+    if ((rc = host_ids(c, ids)) || (rc = download(c, cnt.data(), c->nwl[c->hcur], (size_t)n)))
+        return rc;
+    for (int k = 0; k < DEM_WALLS; ++k)
+        if ((rc = download(c, w.data() + (size_t)k * n, c->wl[c->hcur] + (size_t)k * c->cap, (size_t)n)))
+            return rc;
+    for (int i = 0; i < n; ++i) {
+        num[ids[i]] = cnt[i];
+        for (int k = 0; k < DEM_WALLS; ++k)
+            list[(size_t)ids[i] * DEM_WALLS + k] = k < cnt[i] ? w[(size_t)k * n + i] : -1;
+    }
+    return DEM_OK;
+}
+
+extern "C" int dem_get_hist_pw(dem_ctx *c, double *dispt, double *nDsp, double *nDsp_mx,
+                               double *plstRad, double *plstDfm, int *stage, int *lqdBrg) {
+    if (!c) return dem_fail(DEM_ERR_ARG, "null context");
+    return get_hist(c, c->hw[c->hcur], c->nwl[c->hcur], DEM_WALLS, DEM_WALLS, dispt, nDsp, nDsp_mx,
+                    plstRad, plstDfm, stage, lqdBrg);
+}
+
+// restores lists and history in original order (identity permutation)
+static int put_hist(dem_ctx *c, const DevHist &h, int maxK, int n, const double *dispt,
+                    const double *nDsp, const double *nDsp_mx, const double *plstRad,
+                    const double *plstDfm, const int *stage, const int *lqdBrg) {
+    std::vector<double> a(n);
+    std::vector<int> fl(n);
+    const double *src[4] = {nDsp, nDsp_mx, plstRad, plstDfm};
+    double *dst[7] = {h.dx, h.dy, h.dz, h.nd, h.ndm, h.prad, h.pdfm};
+    for (int k = 0; k < maxK; ++k) {
+        for (int q = 0; q < 7; ++q) {
+            for (int i = 0; i < n; ++i) {
+                size_t o = (size_t)i * maxK + k;
+                a[i] = q < 3 ? (dispt ? dispt[3 * o + q] : 0.0) : (src[q - 3] ? src[q - 3][o] : 0.0);
+            }
+            HIPCHK(hipMemcpyAsync(dst[q] + (size_t)k * c->cap, a.data(), n * sizeof(double),
+                                  hipMemcpyHostToDevice, c->stream));
+            HIPCHK(hipStreamSynchronize(c->stream));
+        }
+        for (int i = 0; i < n; ++i) {
+            size_t o = (size_t)i * maxK + k;
+            fl[i] = (stage ? (stage[o] & 0xff) : 0) | ((lqdBrg && lqdBrg[o]) ? 256 : 0);
+        }
+        HIPCHK(hipMemcpyAsync(h.flags + (size_t)k * c->cap, fl.data(), n * sizeof(int),
+                              hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+    }
+    return DEM_OK;
+}
+
+extern "C" int dem_set_contacts(dem_ctx *c, int maxK, const int *num, const int *list,
+                                const double *dispt, const double *nDsp, const double *nDsp_mx,
+                                const double *plstRad, const double *plstDfm, const int *stage,
+                                const int *lqdBrg, const int *wnum, const int *wlist,
+                                const double *wdispt, const double *wnDsp, const double *wnDsp_mx,
+                                const double *wplstRad, const double *wplstDfm, const int *wstage,
+                                const int *wlqdBrg) {
+    if (!c || !num || !list || !wnum || !wlist || maxK <= 0 || maxK > c->K)
+        return dem_fail(DEM_ERR_ARG, "bad argument");
+    const int n = c->total;
+    // the restored lists must be in the current (identity) order: ids 0..n-1
+    std::vector<int> ids;
+    int rc = host_ids(c, ids);
+    if (rc) return rc;
+    for (int i = 0; i < n; ++i)
+        if (ids[i] != i)
+            return dem_fail(DEM_ERR_STATE, "dem_set_contacts needs state fresh from dem_set_particles");
+    std::vector<int> col(n);
+    for (int k = 0; k < maxK; ++k) {
+        for (int i = 0; i < n; ++i) {
+            int j = list[(size_t)i * maxK + k];
+            col[i] = (k < num[i] && j >= 0 && j < n) ? j : 0;
+        }
+        HIPCHK(hipMemcpyAsync(c->nbr[c->hcur] + (size_t)k * c->cap, col.data(), n * sizeof(int),
+                              hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+    }
+    std::vector<int> cnt(num, num + n);
+    for (int i = 0; i < n; ++i)
+        if (cnt[i] < 0 || cnt[i] > maxK) return dem_fail(DEM_ERR_ARG, "list count out of range");
+    HIPCHK(hipMemcpyAsync(c->nnb[c->hcur], cnt.data(), n * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    std::vector<int> wc(wnum, wnum + n);
+    for (int k = 0; k < DEM_WALLS; ++k) {
+        for (int i = 0; i < n; ++i) col[i] = k < wnum[i] ? wlist[(size_t)i * DEM_WALLS + k] : 0;
+        HIPCHK(hipMemcpyAsync(c->wl[c->hcur] + (size_t)k * c->cap, col.data(), n * sizeof(int),
+                              hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+    }
+    HIPCHK(hipMemcpyAsync(c->nwl[c->hcur], wc.data(), n * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if ((rc = put_hist(c, c->hp[c->hcur], maxK, n, dispt, nDsp, nDsp_mx, plstRad, plstDfm, stage, lqdBrg)))
+        return rc;
+    if ((rc = put_hist(c, c->hw[c->hcur], DEM_WALLS, n, wdispt, wnDsp, wnDsp_mx, wplstRad, wplstDfm,
+                       wstage, wlqdBrg)))
+        return rc;
+    c->has_lists = true;
+    c->saved = false;
+    c->old_total = n;
+    c->rebuild_pending = true;  // RESTART (main.cpp:473)
+    return DEM_OK;
+}
+
+extern "C" int dem_get_hash(dem_ctx *c, int *hash, int *index) {
+    if (!c) return dem_fail(DEM_ERR_ARG, "null context");
+    // after a build, keys are the sorted hashes and the sorted order is the
+    // particle order, so the sorted original index is the id array
+    int rc;
+    if ((rc = download(c, (unsigned *)hash, c->keys, (size_t)c->total))) return rc;
+    return download(c, index, c->id[c->pcur], (size_t)c->total);
+}
+
+extern "C" int dem_get_cells(dem_ctx *c, int *cellStart, int *cellEnd) {
+    if (!c) return dem_fail(DEM_ERR_ARG, "null context");
+    int rc;
+    if ((rc = download(c, cellStart, c->cstart, (size_t)c->S.sysNCells))) return rc;
+    return download(c, cellEnd, c->cend, (size_t)c->S.sysNCells);
+}
+
+// ---------------------------------------------------------------------
+// rebuild pieces (main.cpp:473-527)
+// ---------------------------------------------------------------------
+extern "C" int dem_cnt_hstry_save(dem_ctx *c) {
+    if (!c) return dem_fail(DEM_ERR_ARG, "null context");
+    if (c->has_lists) {
+        c->hcur = 1 - c->hcur;
+        c->saved = true;
+    }
+    return DEM_OK;
+}
+
+extern "C" int dem_bld_nebor_list(dem_ctx *c) {
+    if (!c) return dem_fail(DEM_ERR_ARG, "null context");
+    const int n = c->total;
+    hipStream_t s = c->stream;
+    c->old_total = c->saved ? c->old_total : 0;
+    StateBuf a = c->buf(c->cur, c->Lidx), b = c->buf(1 - c->cur, 1 - c->Lidx);
+    launch_calc_hash(a, c->level[c->pcur], c->id[c->pcur], c->P, c->keys, c->vals, n, s);
+    launch_radix_sort(c->keys, c->vals, c->keys2, c->vals2, n, c->key_bits, c->sort_tmp, s);
+    launch_permute(c->vals, a, b, c->rad[c->pcur], c->rad[1 - c->pcur], c->level[c->pcur],
+                   c->level[1 - c->pcur], c->id[c->pcur], c->id[1 - c->pcur], c->pmove, n, s);
+    c->cur = 1 - c->cur;
+    c->Lidx = 1 - c->Lidx;
+    c->pcur = 1 - c->pcur;
+    launch_cell_range(c->keys, c->cstart, c->cend, c->S.sysNCells, n, s);
+    launch_fill_i32(c->overflow, 0, 1, s);
+    launch_neighbor_find(c->buf(c->cur, c->Lidx), c->rad[c->pcur], c->level[c->pcur], c->P,
+                         c->cstart, c->cend, c->nbr[c->hcur], c->nnb[c->hcur], c->overflow, c->cap,
+                         n, s);
+    HIPCHK(hipGetLastError());
+    return DEM_OK;
+}
+
+extern "C" int dem_bld_cir_wall(dem_ctx *c) {
+    if (!c) return dem_fail(DEM_ERR_ARG, "null context");
+    launch_wall_list(c->buf(c->cur, c->Lidx), c->rad[c->pcur], c->P, c->Tw, c->wl[c->hcur],
+                     c->nwl[c->hcur], c->cap, c->total, c->stream);
+    HIPCHK(hipGetLastError());
+    return DEM_OK;
+}
+
+extern "C" int dem_hstry_copy(dem_ctx *c) {
+    if (!c) return dem_fail(DEM_ERR_ARG, "null context");
+    const int o = 1 - c->hcur;
+    launch_hist_remap(c->vals, c->id[c->pcur], c->id[1 - c->pcur], c->nbr[c->hcur], c->nnb[c->hcur],
+                      c->nbr[o], c->nnb[o], c->hp[c->hcur].ptr(), c->hp[o].ptr(), c->wl[c->hcur],
+                      c->nwl[c->hcur], c->wl[o], c->nwl[o], c->hw[c->hcur].ptr(), c->hw[o].ptr(),
+                      c->cap, c->total, c->saved ? c->old_total : 0, c->stream);
+    HIPCHK(hipGetLastError());
+    c->saved = false;
+    c->has_lists = true;
+    c->old_total = c->total;
+    return DEM_OK;
+}
+
+int dem_do_rebuild(dem_ctx *c) {
+    int rc;
+    if ((rc = dem_cnt_hstry_save(c)) || (rc = dem_bld_nebor_list(c)) || (rc = dem_bld_cir_wall(c)) ||
+        (rc = dem_hstry_copy(c)))
+        return rc;
+    c->rebuild_pending = false;
+    c->n_rebuild++;
+    c->forces_valid = false;
+    return DEM_OK;
+}
+
+extern "C" int dem_rebuild(dem_ctx *c) {
+    if (!c) return dem_fail(DEM_ERR_ARG, "null context");
+    int rc = dem_do_rebuild(c);
+    if (rc) return rc;
+    return check_overflow(c);
+}
+
+// ---------------------------------------------------------------------
+// the reference's separate kernels
+// ---------------------------------------------------------------------
+extern "C" int dem_var_init(dem_ctx *c) {
+    if (!c) return dem_fail(DEM_ERR_ARG, "null context");
+    c->Lidx = 1 - c->Lidx;  // lqdCnt_Old <- lqdCnt
+    launch_fill_i32(c->lq[c->Lidx], 0, c->cap, c->stream);
+    for (int q = 0; q < 7; ++q) launch_fill_f64(c->F[q], 0.0, c->cap, c->stream);
+    HIPCHK(hipGetLastError());
+    return DEM_OK;
+}
+
+extern "C" int dem_pp_inter_calc(dem_ctx *c) {
+    if (!c) return dem_fail(DEM_ERR_ARG, "null context");
+    StepArgs A = step_args(c, false);
+    launch_pp_force(c->P, A, c->S.forceCap == 1, c->stream);
+    HIPCHK(hipGetLastError());
+    return DEM_OK;
+}
+
+extern "C" int dem_cir_pw_calc(dem_ctx *c) {
+    if (!c) return dem_fail(DEM_ERR_ARG, "null context");
+    c->Tw = c->Tw + c->S.topv * c->S.dt;  // kernel.cu:966-969
+    StepArgs A = step_args(c, false);
+    launch_pw_force(c->P, A, c->stream);
+    HIPCHK(hipGetLastError());
+    return DEM_OK;
+}
+
+extern "C" int dem_update_system(dem_ctx *c, int *reNebld) {
+    if (!c) return dem_fail(DEM_ERR_ARG, "null context");
+    StepArgs A = step_args(c, false);
+    launch_fill_i32(c->upd_count, 0, 1, c->stream);
+    launch_update(c->P, A, c->upd_count, c->stream);
+    HIPCHK(hipGetLastError());
+    int cnt = 0;
+    HIPCHK(hipMemcpyAsync(&cnt, c->upd_count, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (reNebld) *reNebld = cnt;
+    c->forces_valid = true;
+    return DEM_OK;
+}
+
+extern "C" int dem_step_unfused(dem_ctx *c) {
+    if (!c) return dem_fail(DEM_ERR_ARG, "null context");
+    int rc;
+    if (c->rebuild_pending && (rc = dem_rebuild(c))) return rc;
+    int flag = 0;
+    if ((rc = dem_var_init(c)) || (rc = dem_pp_inter_calc(c)) || (rc = dem_cir_pw_calc(c)) ||
+        (rc = dem_update_system(c, &flag)))
+        return rc;
+    c->rebuild_pending = flag > 0;
+    c->ctime = c->ctime + c->S.dt;
+    c->iter = c->iter + 1;
+    return DEM_OK;
+}
+
+// ---------------------------------------------------------------------
+// the fused path
+// ---------------------------------------------------------------------
+static int run_chunk(dem_ctx *c, int nsteps, int *executed) {
+    const bool wet = c->S.forceCap == 1;
+    int big = INT_MAX;
+    HIPCHK(hipMemcpyAsync(c->stop_at, &big, sizeof(int), hipMemcpyHostToDevice, c->stream));
+    StepArgs A = step_args(c, true);
+    const double tw0 = c->Tw;
+    double tw = tw0;
+    HIPCHK(hipEventRecord(c->ev0, c->stream));
+    for (int s = 0; s < nsteps; ++s) {
+        const int b = (c->cur + s) & 1, l = (c->Lidx + s) & 1;
+        A.in = c->buf(b, l);
+        A.out = c->buf(1 - b, 1 - l);
+        A.step = s;
+        tw = tw + c->S.topv * c->S.dt;
+        A.Tw = tw;
+        launch_step_fused(c->P, A, wet, c->stream);
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(c->ev1, c->stream));
+    int stop = INT_MAX;
+    HIPCHK(hipMemcpyAsync(&stop, c->stop_at, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    int done = stop < nsteps ? stop : nsteps;
+    c->fused_ms += ms;
+    c->fused_steps += done;
+    c->fused_launches += nsteps;
+    c->cur = (c->cur + done) & 1;
+    c->Lidx = (c->Lidx + done) & 1;
+    c->Tw = tw0;
+    for (int s = 0; s < done; ++s) c->Tw = c->Tw + c->S.topv * c->S.dt;
+    for (int s = 0; s < done; ++s) c->ctime = c->ctime + c->S.dt;
+    c->iter += done;
+    if (stop <= nsteps) c->rebuild_pending = true;
+    *executed = done;
+    return DEM_OK;
+}
+
+extern "C" int dem_step(dem_ctx *c, int nsteps) {
+    if (!c || nsteps < 0) return dem_fail(DEM_ERR_ARG, "bad argument");
+    int done = 0, rc;
+    while (done < nsteps) {
+        if (c->rebuild_pending) {
+            HIPCHK(hipEventRecord(c->ev0, c->stream));
+            if ((rc = dem_do_rebuild(c))) return rc;
+            HIPCHK(hipEventRecord(c->ev1, c->stream));
+            if ((rc = check_overflow(c))) return rc;
+            float ms = 0.f;
+            HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+            c->rebuild_ms += ms;
+        }
+        int chunk = nsteps - done < kChunk ? nsteps - done : kChunk;
+        int ex = 0;
+        if ((rc = run_chunk(c, chunk, &ex))) return rc;
+        done += ex;
+        c->forces_valid = c->store_forces != 0 && ex > 0;
+    }
+    return DEM_OK;
+}
+
+extern "C" int dem_time_steps(dem_ctx *c, int nsteps, double *ms_per_step, int *rebuild_needed) {
+    if (!c || nsteps <= 0) return dem_fail(DEM_ERR_ARG, "bad argument");
+    int rc;
+    if (c->rebuild_pending && (rc = dem_rebuild(c))) return rc;
+    double ms0 = c->fused_ms;
+    long long s0 = c->fused_steps, l0 = c->fused_launches;
+    int ex = 0;
+    if ((rc = run_chunk(c, nsteps, &ex))) return rc;
+    if (ms_per_step)
+        *ms_per_step = (c->fused_ms - ms0) / (double)(c->fused_launches - l0 ? c->fused_launches - l0 : 1);
+    if (rebuild_needed) *rebuild_needed = (c->fused_steps - s0) < nsteps || c->rebuild_pending;
+    return DEM_OK;
+}
+
+extern "C" int dem_get_timing(dem_ctx *c, double *fused_ms, long long *fused_steps,
+                              long long *fused_launches, double *rebuild_ms, int *n_rebuild) {
+    if (!c) return dem_fail(DEM_ERR_ARG, "null context");
+    if (fused_ms) *fused_ms = c->fused_ms;
+    if (fused_steps) *fused_steps = c->fused_steps;
+    if (fused_launches) *fused_launches = c->fused_launches;
+    if (rebuild_ms) *rebuild_ms = c->rebuild_ms;
+    if (n_rebuild) *n_rebuild = c->n_rebuild;
+    return DEM_OK;
+}
+
+extern "C" int dem_sync(dem_ctx *c) {
+    if (!c) return dem_fail(DEM_ERR_ARG, "null context");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return DEM_OK;
+}
+
+extern "C" int dem_iter(dem_ctx *c) { return c ? c->iter : 0; }
+extern "C" double dem_ctime(dem_ctx *c) { return c ? c->ctime : 0.0; }
+extern "C" int dem_n_rebuild(dem_ctx *c) { return c ? c->n_rebuild : 0; }
+
+extern "C" int dem_contact_stats(dem_ctx *c, dem_stats *out) {
+    if (!c || !out) return dem_fail(DEM_ERR_ARG, "bad argument");
+    int rc;
+    if (c->rebuild_pending && (rc = dem_rebuild(c))) return rc;
+    unsigned long long h[8] = {0};
+    HIPCHK(hipMemcpyAsync(c->stats, h, sizeof h, hipMemcpyHostToDevice, c->stream));
+    StepArgs A = step_args(c, true);
+    launch_stats(c->P, A, c->stats, c->stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(h, c->stats, sizeof h, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    out->n_particles = c->total;
+    out->n_nebor = (long long)h[0];
+    out->n_contact = (long long)h[1];
+    out->n_wall_slots = (long long)h[2];
+    out->n_wall_contact = (long long)h[3];
+    out->n_bridge = (long long)h[4];
+    return DEM_OK;
+}

@@ -1,0 +1,96 @@
+// dem_host.h -- host-side context of the DEM engine.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "../../include/dem_capi.h"
+#include "dem_internal.h"
+
+int dem_fail(int code, const char *fmt, ...);
+void dem_calpara_from_material(dem_setup *S);
+extern "C" int dem_read_deck(const char *path, dem_input *in);
+
+#define HIPCHK(call)                                                                    \
+    do {                                                                                \
+        hipError_t e_ = (call);                                                         \
+        if (e_ != hipSuccess)                                                           \
+            return dem_fail(DEM_ERR_HIP, "%s:%d %s: %s", __FILE__, __LINE__, #call,      \
+                            hipGetErrorString(e_));                                     \
+    } while (0)
+
+struct DevHist {
+    double *dx = nullptr, *dy = nullptr, *dz = nullptr, *nd = nullptr, *ndm = nullptr,
+           *prad = nullptr, *pdfm = nullptr;
+    int *flags = nullptr;
+    HistPtr ptr() const { return HistPtr{dx, dy, dz, nd, ndm, prad, pdfm, flags}; }
+};
+
+struct dem_ctx {
+    dem_input in;
+    dem_setup S;
+    int precision = DEM_PREC_FP64;
+    int cap = 0, total = 0, K = 0;
+    hipStream_t stream = nullptr;
+    DevInst P;
+    double Tw = 0.0;
+    std::vector<double> rad_h;   // reduced radius by original id
+    std::vector<int> level_h;    // size level by original id
+
+    // particle state, cell-sorted, two buffers
+    double *st[2][9] = {};       // x y z vx vy vz wx wy wz
+    int cur = 0;
+    int *lq[2] = {};             // bridge counts: lq[Lidx] = lqdCnt, other = lqdCnt_Old
+    int Lidx = 0;
+    double *rad[2] = {};
+    int *level[2] = {}, *id[2] = {};
+    int pcur = 0;
+    double *pmove = nullptr;
+
+    // lists and history, current = [hcur], old = [1-hcur]
+    int *nbr[2] = {}, *nnb[2] = {};
+    int *wl[2] = {}, *nwl[2] = {};
+    DevHist hp[2], hw[2];
+    int hcur = 0;
+    bool has_lists = false, saved = false;
+    int old_total = 0;           // particles covered by the old lists
+
+    // search cells and sort scratch
+    int *cstart = nullptr, *cend = nullptr;
+    unsigned *keys = nullptr, *vals = nullptr, *keys2 = nullptr, *vals2 = nullptr, *sort_tmp = nullptr;
+    int key_bits = 1;
+
+    // force arrays (separate-kernel path and optional output of the fused path)
+    double *F[7] = {};           // Fx Fy Fz Mx My Mz Mr
+    int store_forces = 0;
+
+    // device flags and counters
+    int *stop_at = nullptr, *overflow = nullptr, *upd_count = nullptr;
+    unsigned long long *stats = nullptr;
+
+    // host bookkeeping (main.cpp:425-623)
+    int iter = 0, n_rebuild = 0;
+    double ctime = 0.0;
+    bool rebuild_pending = true;
+    bool forces_valid = false;
+
+    // timing of the fused path
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double fused_ms = 0.0, rebuild_ms = 0.0;
+    long long fused_steps = 0, fused_launches = 0;
+    long long bytes = 0;
+
+    StateBuf buf(int b, int lqd_idx) const {
+        StateBuf s;
+        s.x = st[b][0]; s.y = st[b][1]; s.z = st[b][2];
+        s.vx = st[b][3]; s.vy = st[b][4]; s.vz = st[b][5];
+        s.wx = st[b][6]; s.wy = st[b][7]; s.wz = st[b][8];
+        s.lqd = lq[lqd_idx];
+        return s;
+    }
+};
+
+// DevInst from the derived setup (one instance)
+void dem_fill_inst(const dem_setup &S, DevInst &P);
+// Internal pieces of the rebuild and step, shared by the C ABI functions.
+int dem_do_rebuild(dem_ctx *c);

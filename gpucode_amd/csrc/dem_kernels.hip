@@ -1,0 +1,968 @@
+// dem_kernels.hip -- hand-written gfx950 kernels of the DEM time step.
+//
+// One thread owns one particle i (cell-sorted order) and gathers every
+// directed pair (i, j) itself, so forces accumulate in registers with no
+// atomics; each directed pair has its own history slot, as in the reference
+// (kernel.cu:413-552).  Work is HBM-bound integer/FP64 streaming; nothing
+// here is a dense contraction, so there is no MFMA.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+
+#include "dem_internal.h"
+#include "dem_launch.h"
+
+#define DEV __device__ __forceinline__
+
+// ---------------------------------------------------------------------
+// small helpers
+// ---------------------------------------------------------------------
+static inline unsigned blocks_for(long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+// Blocks are dispatched round-robin over the 8 XCDs; remap so that each
+// XCD walks one contiguous range of particles (neighbouring particles share
+// that XCD's L2).
+DEV unsigned xcd_block(unsigned nb) {
+    unsigned b = blockIdx.x;
+    unsigned per = nb / 8, rem = nb % 8, x = b % 8, k = b / 8;
+    if (nb < 64) return b;
+    // XCD x owns [start_x, start_x + per + (x < rem))
+    unsigned start = x * per + (x < rem ? x : rem);
+    return start + k;
+}
+
+__global__ void k_fill_i32(int *p, int v, long n) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+__global__ void k_fill_f64(double *p, double v, long n) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+void launch_fill_i32(int *p, int v, long n, hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL(k_fill_i32, dim3(blocks_for(n, 256)), dim3(256), 0, s, p, v, n);
+}
+void launch_fill_f64(double *p, double v, long n, hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL(k_fill_f64, dim3(blocks_for(n, 256)), dim3(256), 0, s, p, v, n);
+}
+
+// ---------------------------------------------------------------------
+// exclusive scan of uint32 (radix offsets)
+// ---------------------------------------------------------------------
+#define SCAN_BS 256
+#define SCAN_ITEMS 4
+#define SCAN_TILE (SCAN_BS * SCAN_ITEMS)
+
+DEV unsigned wave_incl_scan(unsigned v) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        unsigned t = __shfl_up(v, d, 64);
+        if (lane >= d) v += t;
+    }
+    return v;
+}
+
+// exclusive scan over a 256-thread block; returns the exclusive prefix and
+// the block total
+DEV unsigned block_excl_scan(unsigned v, unsigned *total, unsigned *lds /*[4]*/) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    unsigned inc = wave_incl_scan(v);
+    if (lane == 63) lds[w] = inc;
+    __syncthreads();
+    unsigned base = 0, tot = 0;
+#pragma unroll
+    for (int k = 0; k < SCAN_BS / 64; ++k) {
+        unsigned t = lds[k];
+        if (k < w) base += t;
+        tot += t;
+    }
+    __syncthreads();
+    *total = tot;
+    return base + inc - v;
+}
+
+__global__ void __launch_bounds__(SCAN_BS) k_scan_local(const unsigned *in, unsigned *out, long n,
+                                                        unsigned *bsum) {
+    __shared__ unsigned lds[4];
+    long base = (long)blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
+    unsigned v[SCAN_ITEMS], s = 0;
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; ++k) {
+        v[k] = (base + k < n) ? in[base + k] : 0u;
+        s += v[k];
+    }
+    unsigned tot;
+    unsigned p = block_excl_scan(s, &tot, lds);
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; ++k) {
+        if (base + k < n) out[base + k] = p;
+        p += v[k];
+    }
+    if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
+}
+
+// single block: exclusive scan of nb block sums in place
+__global__ void __launch_bounds__(SCAN_BS) k_scan_top(unsigned *bsum, long nb) {
+    __shared__ unsigned lds[4];
+    unsigned carry = 0;
+    for (long c = 0; c < nb; c += SCAN_TILE) {
+        long base = c + threadIdx.x * SCAN_ITEMS;
+        unsigned v[SCAN_ITEMS], s = 0;
+#pragma unroll
+        for (int k = 0; k < SCAN_ITEMS; ++k) {
+            v[k] = (base + k < nb) ? bsum[base + k] : 0u;
+            s += v[k];
+        }
+        unsigned tot;
+        unsigned p = block_excl_scan(s, &tot, lds) + carry;
+#pragma unroll
+        for (int k = 0; k < SCAN_ITEMS; ++k) {
+            if (base + k < nb) bsum[base + k] = p;
+            p += v[k];
+        }
+        carry += tot;
+    }
+}
+
+__global__ void k_scan_add(unsigned *out, long n, const unsigned *bsum) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] += bsum[i / SCAN_TILE];
+}
+
+long scan_tmp_elems(long n) { return (n + SCAN_TILE - 1) / SCAN_TILE; }
+
+void launch_excl_scan(const unsigned *in, unsigned *out, long n, unsigned *tmp, hipStream_t s) {
+    if (n <= 0) return;
+    long nb = scan_tmp_elems(n);
+    hipLaunchKernelGGL(k_scan_local, dim3((unsigned)nb), dim3(SCAN_BS), 0, s, in, out, n, tmp);
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(SCAN_BS), 0, s, tmp, nb);
+    hipLaunchKernelGGL(k_scan_add, dim3(blocks_for(n, 256)), dim3(256), 0, s, out, n, tmp);
+}
+
+// ---------------------------------------------------------------------
+// stable LSD radix sort of (key, value) uint32 pairs, 8-bit digits
+// ---------------------------------------------------------------------
+#define RS_BS 256
+#define RS_ROUNDS 8
+#define RS_TILE (RS_BS * RS_ROUNDS)
+
+__global__ void __launch_bounds__(RS_BS) k_rs_hist(const unsigned *keys, long n, int shift,
+                                                   unsigned *hist, long nblk) {
+    __shared__ unsigned cnt[256];
+    cnt[threadIdx.x] = 0;
+    __syncthreads();
+    long base = (long)blockIdx.x * RS_TILE;
+#pragma unroll
+    for (int r = 0; r < RS_ROUNDS; ++r) {
+        long e = base + (long)r * RS_BS + threadIdx.x;
+        if (e < n) atomicAdd(&cnt[(keys[e] >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    hist[(long)threadIdx.x * nblk + blockIdx.x] = cnt[threadIdx.x];
+}
+
+// lanes of this wavefront holding the same 9-bit value d
+DEV unsigned long long wave_match9(unsigned d) {
+    unsigned long long m = ~0ull;
+#pragma unroll
+    for (int b = 0; b < 9; ++b) {
+        int bit = (d >> b) & 1u;
+        unsigned long long bb = __ballot(bit);
+        m &= bit ? bb : ~bb;
+    }
+    return m;
+}
+
+__global__ void __launch_bounds__(RS_BS) k_rs_scatter(const unsigned *kin, const unsigned *vin,
+                                                      unsigned *kout, unsigned *vout, long n,
+                                                      int shift, const unsigned *offs, long nblk) {
+    __shared__ unsigned base[256];
+    __shared__ unsigned wcnt[RS_BS / 64][256];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    base[tid] = offs[(long)tid * nblk + blockIdx.x];
+    const unsigned long long lt = (1ull << lane) - 1ull;
+    long tile = (long)blockIdx.x * RS_TILE;
+    for (int r = 0; r < RS_ROUNDS; ++r) {
+#pragma unroll
+        for (int q = 0; q < RS_BS / 64; ++q) wcnt[q][tid] = 0;
+        __syncthreads();
+        long e = tile + (long)r * RS_BS + tid;
+        bool valid = e < n;
+        unsigned key = 0, val = 0, d = 256u;
+        if (valid) {
+            key = kin[e];
+            val = vin[e];
+            d = (key >> shift) & 255u;
+        }
+        unsigned long long peers = wave_match9(d);
+        unsigned rank = (unsigned)__popcll(peers & lt);
+        bool leader = (peers & lt) == 0ull;
+        if (valid && leader) wcnt[w][d] = (unsigned)__popcll(peers);
+        __syncthreads();
+        if (valid) {
+            unsigned pos = base[d] + rank;
+            for (int q = 0; q < w; ++q) pos += wcnt[q][d];
+            kout[pos] = key;
+            vout[pos] = val;
+        }
+        __syncthreads();
+        unsigned add = 0;
+#pragma unroll
+        for (int q = 0; q < RS_BS / 64; ++q) add += wcnt[q][tid];
+        base[tid] += add;
+        __syncthreads();
+    }
+}
+
+long radix_tmp_elems(long n) {
+    long nblk = (n + RS_TILE - 1) / RS_TILE;
+    long h = 256 * nblk;
+    return 2 * h + scan_tmp_elems(h) + 16;
+}
+
+// Sorts (k, v) of length n by the low `bits` bits of k.  k2/v2 are
+// ping-pong buffers; the result ends in (k, v).
+void launch_radix_sort(unsigned *k, unsigned *v, unsigned *k2, unsigned *v2, long n, int bits,
+                       unsigned *tmp, hipStream_t s) {
+    if (n <= 1) return;
+    long nblk = (n + RS_TILE - 1) / RS_TILE;
+    long h = 256 * nblk;
+    unsigned *hist = tmp, *offs = tmp + h, *stmp = tmp + 2 * h;
+    int passes = (bits + 7) / 8;
+    if (passes < 1) passes = 1;
+    unsigned *ka = k, *va = v, *kb = k2, *vb = v2;
+    for (int p = 0; p < passes; ++p) {
+        int shift = 8 * p;
+        hipLaunchKernelGGL(k_rs_hist, dim3((unsigned)nblk), dim3(RS_BS), 0, s, ka, n, shift, hist,
+                           nblk);
+        launch_excl_scan(hist, offs, h, stmp, s);
+        hipLaunchKernelGGL(k_rs_scatter, dim3((unsigned)nblk), dim3(RS_BS), 0, s, ka, va, kb, vb,
+                           n, shift, offs, nblk);
+        unsigned *t = ka; ka = kb; kb = t;
+        t = va; va = vb; vb = t;
+    }
+    if (ka != k) {
+        (void)hipMemcpyAsync(k, ka, n * sizeof(unsigned), hipMemcpyDeviceToDevice, s);
+        (void)hipMemcpyAsync(v, va, n * sizeof(unsigned), hipMemcpyDeviceToDevice, s);
+    }
+}
+
+// ---------------------------------------------------------------------
+// cell hash (kernel.cu:617-673)
+// ---------------------------------------------------------------------
+DEV int cell_coord(double p, double o, double cs, int g) {
+    int c = (int)floor((p - o) / cs);
+    if (c > g - 1) c = g - 1;
+    if (c < 0) c = 0;
+    return c;
+}
+
+// keys[id] = hash, vals[id] = current index: sorting the id-ordered pairs
+// stably by hash yields (hash, original index) order, the order
+// thrust::sort_by_key produces from calcHashG's output.
+__global__ void k_calc_hash(const double *x, const double *y, const double *z, const int *level,
+                            const int *id, DevInst P, unsigned *keys, unsigned *vals, int total) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    int L = level[i];
+    double cs = P.cellSize[L];
+    int cx = cell_coord(x[i], P.origin[0], cs, P.gx[L]);
+    int cy = cell_coord(y[i], P.origin[1], cs, P.gy[L]);
+    int cz = cell_coord(z[i], P.origin[2], cs, P.gz[L]);
+    int h = (cz * P.gy[L] * P.gx[L]) + (cy * P.gx[L]) + cx + P.cellOff[L];
+    keys[id[i]] = (unsigned)h;
+    vals[id[i]] = (unsigned)i;
+}
+
+void launch_calc_hash(const StateBuf &s, const int *level, const int *id, const DevInst &P,
+                      unsigned *keys, unsigned *vals, int total, hipStream_t st) {
+    if (total <= 0) return;
+    hipLaunchKernelGGL(k_calc_hash, dim3(blocks_for(total, 256)), dim3(256), 0, st, s.x, s.y, s.z,
+                       level, id, P, keys, vals, total);
+}
+
+// ---------------------------------------------------------------------
+// permutation into cell order
+// ---------------------------------------------------------------------
+__global__ void k_permute(const unsigned *perm, StateBuf a, StateBuf b, const double *rad_a,
+                          double *rad_b, const int *lvl_a, int *lvl_b, const int *id_a, int *id_b,
+                          double *pmove, int total) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    int o = (int)perm[i];
+    b.x[i] = a.x[o]; b.y[i] = a.y[o]; b.z[i] = a.z[o];
+    b.vx[i] = a.vx[o]; b.vy[i] = a.vy[o]; b.vz[i] = a.vz[o];
+    b.wx[i] = a.wx[o]; b.wy[i] = a.wy[o]; b.wz[i] = a.wz[o];
+    b.lqd[i] = a.lqd[o];
+    rad_b[i] = rad_a[o];
+    lvl_b[i] = lvl_a[o];
+    id_b[i] = id_a[o];
+    pmove[i] = 0.0;  // main.cpp:492
+}
+
+void launch_permute(const unsigned *perm, const StateBuf &a, const StateBuf &b,
+                    const double *rad_a, double *rad_b, const int *lvl_a, int *lvl_b,
+                    const int *id_a, int *id_b, double *pmove, int total, hipStream_t s) {
+    if (total <= 0) return;
+    hipLaunchKernelGGL(k_permute, dim3(blocks_for(total, 256)), dim3(256), 0, s, perm, a, b, rad_a,
+                       rad_b, lvl_a, lvl_b, id_a, id_b, pmove, total);
+}
+
+// ---------------------------------------------------------------------
+// cell ranges (kernel.cu:682-724)
+// ---------------------------------------------------------------------
+__global__ void k_cell_range(const unsigned *keys, int *cstart, int *cend, int total) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= total) return;
+    unsigned h = keys[k];
+    if (k == 0 || h != keys[k - 1]) {
+        cstart[h] = k;
+        if (k > 0) cend[keys[k - 1]] = k;
+    }
+    if (k == total - 1) cend[h] = k + 1;
+}
+
+void launch_cell_range(const unsigned *keys, int *cstart, int *cend, int ncells, int total,
+                       hipStream_t s) {
+    launch_fill_i32(cstart, DEM_EMPTY, ncells, s);
+    launch_fill_i32(cend, DEM_EMPTY, ncells, s);
+    if (total <= 0) return;
+    hipLaunchKernelGGL(k_cell_range, dim3(blocks_for(total, 256)), dim3(256), 0, s, keys, cstart,
+                       cend, total);
+}
+
+// ---------------------------------------------------------------------
+// neighbour search (kernel.cu:729-855), in the reference's scan order:
+// level, z row, y row, then the trimmed contiguous sorted range of the row.
+// The file is compiled with -ffp-contract=off, so the distance test rounds
+// like the host restatement.
+// ---------------------------------------------------------------------
+__global__ void __launch_bounds__(128) k_neighbor_find(const double *x, const double *y,
+                                                       const double *z, const double *rad,
+                                                       const int *level, DevInst P,
+                                                       const int *cstart, const int *cend,
+                                                       int *nbr, int *nnb, int *overflow, int cap,
+                                                       int total) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    const double ox = x[i], oy = y[i], oz = z[i], ri = rad[i];
+    const int iL = level[i];
+    const int K = P.levelK[iL];
+    int num = 0;
+    for (int L = 0; L < P.nLevels; ++L) {
+        const double cs = P.cellSize[L];
+        const int gx = P.gx[L], gy = P.gy[L], gz = P.gz[L], coff = P.cellOff[L];
+        const int cx = cell_coord(ox, P.origin[0], cs, gx);
+        const int cy = cell_coord(oy, P.origin[1], cs, gy);
+        const int cz = cell_coord(oz, P.origin[2], cs, gz);
+        const int off = (int)ceil(P.levelSize[iL] / P.levelSize[L]);
+        for (int dz = -off; dz <= off; ++dz) {
+            const int zz = cz + dz;
+            if (zz < 0 || zz > gz - 1) continue;
+            for (int dy = -off; dy <= off; ++dy) {
+                const int yy = cy + dy;
+                if (yy < 0 || yy > gy - 1) continue;
+                int sx = cx - off, ex = cx + off;
+                if (sx < 0) sx = 0;
+                if (ex > gx - 1) ex = gx - 1;
+                const int rowb = coff + zz * gy * gx + yy * gx;
+                while (sx != ex) {
+                    if (cstart[rowb + sx] == DEM_EMPTY) ++sx;
+                    else break;
+                }
+                const int s0 = cstart[rowb + sx];
+                while (sx != ex) {
+                    if (cend[rowb + ex] == DEM_EMPTY) --ex;
+                    else break;
+                }
+                const int e0 = cend[rowb + ex];
+                if (s0 == DEM_EMPTY || e0 == DEM_EMPTY) continue;
+                for (int j = s0; j < e0; ++j) {
+                    if (j == i) continue;
+                    double dx = ox - x[j], dy2 = oy - y[j], dz2 = oz - z[j];
+                    double len = sqrt(dx * dx + dy2 * dy2 + dz2 * dz2);
+                    if (len < rad[j] + ri + P.searchGap) {
+                        if (num < K) nbr[(long)num * cap + i] = j;
+                        ++num;
+                    }
+                }
+            }
+        }
+    }
+    if (num > K) {
+        atomicMax(overflow, num);
+        num = K;
+    }
+    nnb[i] = num;
+}
+
+void launch_neighbor_find(const StateBuf &s, const double *rad, const int *level,
+                          const DevInst &P, const int *cstart, const int *cend, int *nbr, int *nnb,
+                          int *overflow, int cap, int total, hipStream_t st) {
+    if (total <= 0) return;
+    hipLaunchKernelGGL(k_neighbor_find, dim3(blocks_for(total, 128)), dim3(128), 0, st, s.x, s.y,
+                       s.z, rad, level, P, cstart, cend, nbr, nnb, overflow, cap, total);
+}
+
+// ---------------------------------------------------------------------
+// cylinder wall candidates (kernel.cu:1034-1073)
+// ---------------------------------------------------------------------
+__global__ void k_wall_list(const double *x, const double *y, const double *z, const double *rad,
+                            DevInst P, double Tw, int *wl, int *nwl, int cap, int total) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    const double px = x[i], py = y[i], pz = z[i], r = rad[i];
+    int n = 0;
+    if (pz >= (Tw - (2.0 * r + P.searchGap))) wl[(long)(n++) * cap + i] = 0;
+    if (pz <= ((2.0 * r + P.searchGap) + P.Bw)) wl[(long)(n++) * cap + i] = 1;
+    double ax = P.cirX - px, ay = P.cirY - py;
+    double dc = sqrt(ax * ax + ay * ay);
+    if (dc >= P.R - (2.0 * r + P.searchGap)) wl[(long)(n++) * cap + i] = 2;
+    nwl[i] = n;
+}
+
+void launch_wall_list(const StateBuf &s, const double *rad, const DevInst &P, double Tw, int *wl,
+                      int *nwl, int cap, int total, hipStream_t st) {
+    if (total <= 0) return;
+    hipLaunchKernelGGL(k_wall_list, dim3(blocks_for(total, 256)), dim3(256), 0, st, s.x, s.y, s.z,
+                       rad, P, Tw, wl, nwl, cap, total);
+}
+
+// ---------------------------------------------------------------------
+// history remap across a rebuild (kernel.cu:869-950 with
+// ParticleSystem.cu:336-375): each new slot takes the history of the same
+// partner in the old list, or zero.  Partners are compared by original id
+// because the particle order changes at every rebuild.
+// ---------------------------------------------------------------------
+DEV void hist_copy(HistPtr d, long ds, HistPtr s, long ss) {
+    d.dx[ds] = s.dx[ss]; d.dy[ds] = s.dy[ss]; d.dz[ds] = s.dz[ss];
+    d.nd[ds] = s.nd[ss]; d.ndm[ds] = s.ndm[ss];
+    d.prad[ds] = s.prad[ss]; d.pdfm[ds] = s.pdfm[ss];
+    d.flags[ds] = s.flags[ss];
+}
+DEV void hist_zero(HistPtr d, long ds) {
+    d.dx[ds] = 0.0; d.dy[ds] = 0.0; d.dz[ds] = 0.0;
+    d.nd[ds] = 0.0; d.ndm[ds] = 0.0; d.prad[ds] = 0.0; d.pdfm[ds] = 0.0;
+    d.flags[ds] = 0;
+}
+
+__global__ void k_hist_remap(const unsigned *perm, const int *id_new, const int *id_old,
+                             const int *nbr, const int *nnb, const int *nbr_old,
+                             const int *nnb_old, HistPtr h, HistPtr h_old, int cap, int total,
+                             int old_total) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    int p = (int)perm[i];
+    int nold = p < old_total ? nnb_old[p] : 0;
+    int n = nnb[i];
+    for (int a = 0; a < n; ++a) {
+        long sa = (long)a * cap + i;
+        int jid = id_new[nbr[sa]];
+        int hit = -1;
+        for (int b = 0; b < nold; ++b)
+            if (id_old[nbr_old[(long)b * cap + p]] == jid) hit = b;
+        if (hit >= 0)
+            hist_copy(h, sa, h_old, (long)hit * cap + p);
+        else
+            hist_zero(h, sa);
+    }
+}
+
+__global__ void k_hist_remap_pw(const unsigned *perm, const int *wl, const int *nwl,
+                                const int *wl_old, const int *nwl_old, HistPtr h, HistPtr h_old,
+                                int cap, int total, int old_total) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    int p = (int)perm[i];
+    int nold = p < old_total ? nwl_old[p] : 0;
+    for (int a = 0; a < nwl[i]; ++a) {
+        long sa = (long)a * cap + i;
+        int w = wl[sa], hit = -1;
+        for (int b = 0; b < nold; ++b)
+            if (wl_old[(long)b * cap + p] == w) hit = b;
+        if (hit >= 0)
+            hist_copy(h, sa, h_old, (long)hit * cap + p);
+        else
+            hist_zero(h, sa);
+    }
+}
+
+void launch_hist_remap(const unsigned *perm, const int *id_new, const int *id_old, const int *nbr,
+                       const int *nnb, const int *nbr_old, const int *nnb_old, HistPtr h,
+                       HistPtr h_old, const int *wl, const int *nwl, const int *wl_old,
+                       const int *nwl_old, HistPtr hw, HistPtr hw_old, int cap, int total,
+                       int old_total, hipStream_t s) {
+    if (total <= 0) return;
+    hipLaunchKernelGGL(k_hist_remap, dim3(blocks_for(total, 128)), dim3(128), 0, s, perm, id_new,
+                       id_old, nbr, nnb, nbr_old, nnb_old, h, h_old, cap, total, old_total);
+    hipLaunchKernelGGL(k_hist_remap_pw, dim3(blocks_for(total, 256)), dim3(256), 0, s, perm, wl,
+                       nwl, wl_old, nwl_old, hw, hw_old, cap, total, old_total);
+}
+
+// ---------------------------------------------------------------------
+// contact laws, FP64 (kernel.cu:89-359), same evaluation order as the
+// reference expressions (no FP contraction, see above).
+// ---------------------------------------------------------------------
+
+struct V3 { double x, y, z; };
+DEV V3 v3(double x, double y, double z) { V3 r = {x, y, z}; return r; }
+DEV V3 add(V3 a, V3 b) { return v3(a.x + b.x, a.y + b.y, a.z + b.z); }
+DEV V3 sub(V3 a, V3 b) { return v3(a.x - b.x, a.y - b.y, a.z - b.z); }
+DEV V3 mulv(V3 a, double s) { return v3(a.x * s, a.y * s, a.z * s); }
+DEV V3 smulv(double s, V3 a) { return v3(s * a.x, s * a.y, s * a.z); }
+DEV V3 divv(V3 a, double s) { return v3(a.x / s, a.y / s, a.z / s); }
+DEV double dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+DEV double len(V3 a) { return sqrt(dot(a, a)); }
+DEV V3 cross(V3 a, V3 b) {
+    return v3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+DEV double pow15(double x) { return x * sqrt(x); }
+
+struct Hist1 {
+    double dx, dy, dz, nd, ndm, prad, pdfm;
+    int flags;
+};
+DEV Hist1 hist_load(const HistPtr &h, long s) {
+    Hist1 r;
+    r.dx = h.dx[s]; r.dy = h.dy[s]; r.dz = h.dz[s];
+    r.nd = h.nd[s]; r.ndm = h.ndm[s]; r.prad = h.prad[s]; r.pdfm = h.pdfm[s];
+    r.flags = h.flags[s];
+    return r;
+}
+DEV void hist_store(const HistPtr &h, long s, const Hist1 &r) {
+    h.dx[s] = r.dx; h.dy[s] = r.dy; h.dz[s] = r.dz;
+    h.nd[s] = r.nd; h.ndm[s] = r.ndm; h.prad[s] = r.prad; h.pdfm[s] = r.pdfm;
+    h.flags[s] = r.flags;
+}
+
+// normal force with the load/unload/reload verdict (kernel.cu:104-144 p-p,
+// 250-290 p-w; YIELD_LE selects the p-p yield test "<=")
+template <bool YIELD_LE>
+DEV double normal_law(const DevLaw &c, Hist1 &h, double d, double R) {
+    int stage;
+    if (d < h.pdfm) stage = 1;
+    else if (d >= h.pdfm && d < h.nd) stage = 3;
+    else if (d >= h.nd && d < h.ndm) stage = 4;
+    else stage = 2;
+    double fN = 0.0;
+    if (stage == 2) {
+        bool elastic = YIELD_LE ? (d <= c.ydsp_t * R) : (d < (c.ydsp_t * R));
+        if (elastic) {
+            fN = c.c43E * sqrt(R * d) * d;
+            h.prad = R;
+            h.pdfm = 0.0;
+        } else {
+            fN = (c.yforce_t * (R * R)) + (c.ystiff_t * R) * (d - (c.ydsp_t * R));
+            h.prad = R * (c.c43E * sqrt(R * d) * d) / fN;
+            h.pdfm = d - pow((fN / (c.c43E * sqrt(R))), 2.0 / 3.0);
+        }
+        h.ndm = d;
+    } else if (stage != 1) {
+        fN = c.c43E * sqrt(h.prad) * pow15(d - h.pdfm);
+    }
+    h.nd = d;
+    h.flags = (h.flags & ~0xff) | stage;
+    return fN;
+}
+
+// tangential spring (kernel.cu:172-217 p-p with 1e-7 tolerances,
+// 306-350 p-w with 1e-10)
+template <bool PP>
+DEV V3 tangential_law(const DevLaw &c, Hist1 &h, double fN, double d, double R, V3 n, V3 vt,
+                      double dt) {
+    const double tol = PP ? 1.0e-7 : 1.0e-10;
+    V3 old = v3(h.dx, h.dy, h.dz);
+    V3 t = cross(smulv(-1.0, n), cross(n, old));
+    double lt = len(t);
+    if (lt > tol)
+        t = add(divv(mulv(t, len(old)), lt), mulv(vt, dt));
+    else
+        t = add(t, mulv(vt, dt));
+    double dsmx = c.dsmxA * fN / (c.dsmxB * sqrt(R * d));
+    double ratio = 1.0;
+    double l2 = len(t);
+    if (l2 < dsmx) {
+        if (dsmx > tol) ratio = 1.0 - pow15(1.0 - l2 / dsmx);
+    } else {
+        ratio = 1.0;
+        if (l2 > tol)
+            t = divv(smulv(dsmx, t), l2);
+        else
+            t = smulv(dsmx, t);
+    }
+    h.dx = t.x; h.dy = t.y; h.dz = t.z;
+    V3 ft = v3(0.0, 0.0, 0.0);
+    double l3 = len(t);
+    if (l3 < dsmx) {
+        if (l3 > 1.0e-10) ft = divv(smulv(-1.0 * c.sfrc * fN * ratio, t), l3);
+    } else {
+        double lv = len(vt);
+        if (lv > 1.0e-10) ft = divv(smulv(-1.0 * c.sfrc * fN * ratio, vt), lv);
+    }
+    return ft;
+}
+
+// one directed contact; cpv = contact-point relative velocity
+template <bool PP>
+DEV void contact(const DevLaw &c, double rolling, double dt, Hist1 &h, double d, V3 n, double R,
+                 double ri, V3 cpv, V3 &F, V3 &M, double &Mr) {
+    double fN = normal_law<PP>(c, h, d, R);
+    double cpvn = dot(cpv, n);
+    double fD = 0.0;
+    if (c.damp_on) fD = c.damp * sqrt(R * d) * cpvn;
+    V3 vt = sub(cpv, smulv(cpvn, n));
+    V3 ft = tangential_law<PP>(c, h, fN, d, R, n, vt, dt);
+    V3 dF = add(smulv(fN + fD, n), ft);
+    double arm = ri - d / 2.0;
+    V3 dM = cross(smulv(-arm, n), ft);
+    F = add(F, dF);
+    M = add(M, dM);
+    Mr = Mr + rolling * fN * ri;
+}
+
+// Willett bridge force (kernel.cu:65-81) at the instance's contact angle
+DEV double fcap(const DevInst &P, double S, double V, double effR) {
+    S = fmax(P.gapmn, S);
+    S = S / sqrt(V);
+    double L = log(V), L2 = L * L, L3 = L2 * L;
+    const double *a = P.wc;
+    double f1 = a[0] + a[1] * L + a[2] * L2 + a[3] * L3;
+    double f2 = a[4] + a[5] * L + a[6] * L2;
+    double f3 = a[7] + a[8] * L + a[9] * L2 + a[10] * L3;
+    double f4 = a[11] + a[12] * L + a[13] * L2;
+    double lS = log(S);
+    return P.twoPI * effR * P.stension * exp(f1 - f2 * exp(f3 * lS + f4 * (lS * lS)));
+}
+
+DEV double rupture(const DevInst &P, double V) {
+    return P.ruptureMode == 1 ? (1.0 + 0.5 * P.theta) * cbrt(V) : 0.99 * pow(V, 0.33);
+}
+
+// gravity + all p-p slots of particle i (kernel.cu:424-551)
+template <bool WET>
+DEV void pp_particle(const DevInst &P, const StepArgs &A, int i, V3 &F, V3 &M, double &Mr,
+                     int &nbridge) {
+    const StateBuf &s = A.in;
+    const V3 ip = v3(s.x[i], s.y[i], s.z[i]);
+    const V3 iv = v3(s.vx[i], s.vy[i], s.vz[i]);
+    const V3 iw = v3(s.wx[i], s.wy[i], s.wz[i]);
+    const double ri = A.rad[i];
+    double iVol = 0.0;
+    if (WET) {
+        int inl = s.lqd[i];
+        if (inl < 12) inl = 12;
+        double di = ri * 2;
+        iVol = P.vol * (di * di * di) / inl;
+    }
+    double di = 2.0 * ri;
+    F.z = F.z - di * di * di;
+    const int n = A.nnb[i];
+    for (int k = 0; k < n; ++k) {
+        const long sk = (long)k * A.cap + i;
+        const int j = A.nbr[sk];
+        const V3 jp = v3(s.x[j], s.y[j], s.z[j]);
+        const double rj = A.rad[j];
+        const V3 dv = sub(ip, jp);
+        const double l = len(dv);
+        const V3 nv = divv(dv, l);
+        const double gap = l - ri - rj;
+        if (gap < 0.0) {
+            const V3 jv = v3(s.vx[j], s.vy[j], s.vz[j]);
+            const V3 jw = v3(s.wx[j], s.wy[j], s.wz[j]);
+            Hist1 h = hist_load(A.h, sk);
+            double R = ri * rj / (ri + rj);
+            V3 cpv = sub(sub(iv, jv), cross(add(mulv(iw, ri), mulv(jw, rj)), nv));
+            contact<true>(P.pp, P.rolling, P.dt, h, -gap, nv, R, ri, cpv, F, M, Mr);
+            hist_store(A.h, sk, h);
+        }
+        if (WET && P.forceCap == 1) {
+            int jnl = s.lqd[j];
+            if (jnl < 12) jnl = 12;
+            double dj = rj * 2;
+            double jVol = P.vol * (dj * dj * dj) / jnl;
+            double V = iVol + jVol;
+            double rup = rupture(P, V);
+            int fl = A.h.flags[sk];
+            int brg = (fl >> 8) & 1;
+            if (gap < P.layer2 * (ri + rj)) brg = 1;
+            else if (gap > rup) brg = 0;
+            int nfl = (fl & 0xff) | (brg << 8);
+            if (nfl != fl) A.h.flags[sk] = nfl;
+            double Sp = gap < 0 ? 0.0 : gap;
+            if (brg == 1) {
+                double effR = 2 * ri * rj / (ri + rj);
+                V = V * 8.0;
+                V3 fc = smulv(-1.0 * fcap(P, Sp, V, effR), nv);
+                F = add(F, fc);
+                nbridge = nbridge + 1;
+            }
+        }
+    }
+}
+
+// walls of particle i (kernel.cu:958-1023, 370-403)
+DEV void pw_particle(const DevInst &P, const StepArgs &A, int i, V3 &F, V3 &M, double &Mr) {
+    const StateBuf &s = A.in;
+    const int n = A.nwl[i];
+    if (n == 0) return;
+    const V3 ip = v3(s.x[i], s.y[i], s.z[i]);
+    const V3 iv = v3(s.vx[i], s.vy[i], s.vz[i]);
+    const V3 iw = v3(s.wx[i], s.wy[i], s.wz[i]);
+    const double ri = A.rad[i];
+    V3 jp = v3(0.0, 0.0, 0.0), jv = v3(0.0, 0.0, 0.0);
+    for (int k = 0; k < n; ++k) {
+        const long sk = (long)k * A.cap + i;
+        const int w = A.wl[sk];
+        if (w == 0) {
+            jp = v3(ip.x, ip.y, A.Tw);
+            jv = v3(0.0, 0.0, P.topv);
+        } else if (w == 1) {
+            jp = v3(ip.x, ip.y, P.Bw);
+        } else {
+            double ax = P.cirX - ip.x, ay = P.cirY - ip.y;
+            double dc = sqrt(ax * ax + ay * ay);
+            double cs = (ip.x - P.cirX) / dc, sn = (ip.y - P.cirY) / dc;
+            jp.x = P.R * cs + P.cirX;
+            jp.y = P.R * sn + P.cirY;
+            jp.z = ip.z;
+        }
+        const V3 dv = sub(ip, jp);
+        const double l = len(dv);
+        const V3 nv = divv(dv, l);
+        const double gap = l - ri;
+        if (gap < 0.0) {
+            Hist1 h = hist_load(A.hw, sk);
+            V3 cpv = sub(sub(iv, jv), cross(mulv(iw, ri), nv));
+            contact<false>(P.pw, P.rolling, P.dt, h, -gap, nv, ri, ri, cpv, F, M, Mr);
+            hist_store(A.hw, sk, h);
+        }
+    }
+}
+
+// semi-implicit Euler (kernel.cu:563-606); returns true when the particle's
+// path length passed the rebuild threshold
+DEV bool integrate(const DevInst &P, const StepArgs &A, int i, V3 F, V3 M, double Mr) {
+    const StateBuf &a = A.in, &b = A.out;
+    const double dt = P.dt;
+    const double di = 2.0 * A.rad[i];
+    const double mass = di * di * di;
+    double vx = a.vx[i] + F.x * dt / mass;
+    double vy = a.vy[i] + F.y * dt / mass;
+    double vz = a.vz[i] + F.z * dt / mass;
+    b.vx[i] = vx; b.vy[i] = vy; b.vz[i] = vz;
+    b.x[i] = a.x[i] + vx * dt;
+    b.y[i] = a.y[i] + vy * dt;
+    b.z[i] = a.z[i] + vz * dt;
+    double pm = A.pmove[i] + sqrt(vx * vx + vy * vy + vz * vz) * dt;
+    A.pmove[i] = pm;
+    const double d2 = di * di;
+    const double inert = 0.1 * (d2 * d2 * di);
+    double wx = a.wx[i] + M.x * dt / inert;
+    double wy = a.wy[i] + M.y * dt / inert;
+    double wz = a.wz[i] + M.z * dt / inert;
+    double mr = Mr * dt / inert;
+    double al = sqrt(wx * wx + wy * wy + wz * wz);
+    if (al > 1.0e-7) {
+        double f = (1.0 - fmin(1.0, mr / al));
+        wx = wx * f; wy = wy * f; wz = wz * f;
+    }
+    b.wx[i] = wx; b.wy[i] = wy; b.wz[i] = wz;
+    if (A.store_forces) A.Mr[i] = mr;
+    return pm > P.rebuildThr;
+}
+
+// A wave that saw any particle over the threshold lowers stop_at to the
+// next step: that step and all later ones in the launch chunk skip, and the
+// host rebuilds before re-running them.
+DEV void request_rebuild(const StepArgs &A, bool mine) {
+    unsigned long long b = __ballot(mine);
+    if (b != 0ull && (threadIdx.x & 63) == __ffsll((long long)b) - 1)
+        atomicMin(A.stop_at, A.step + 1);
+}
+
+template <bool WET>
+__global__ void __launch_bounds__(256) k_step_fused(DevInst P, StepArgs A) {
+    // a rebuild is due before this step
+    if (__hip_atomic_load(A.stop_at, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <= A.step) return;
+    const unsigned nb = gridDim.x;
+    const int i = (int)(xcd_block(nb) * blockDim.x + threadIdx.x);
+    bool over = false;
+    if (i < A.total) {
+        V3 F = v3(0.0, 0.0, 0.0), M = v3(0.0, 0.0, 0.0);
+        double Mr = 0.0;
+        int nbridge = 0;
+        pp_particle<WET>(P, A, i, F, M, Mr, nbridge);
+        if (WET) A.out.lqd[i] = nbridge;
+        pw_particle(P, A, i, F, M, Mr);
+        if (A.store_forces) {
+            A.Fx[i] = F.x; A.Fy[i] = F.y; A.Fz[i] = F.z;
+            A.Mx[i] = M.x; A.My[i] = M.y; A.Mz[i] = M.z;
+        }
+        over = integrate(P, A, i, F, M, Mr);
+    }
+    request_rebuild(A, over);
+}
+
+void launch_step_fused(const DevInst &P, const StepArgs &A, bool wet, hipStream_t s) {
+    if (A.total <= 0) return;
+    unsigned nb = blocks_for(A.total, 256);
+    if (wet)
+        hipLaunchKernelGGL(k_step_fused<true>, dim3(nb), dim3(256), 0, s, P, A);
+    else
+        hipLaunchKernelGGL(k_step_fused<false>, dim3(nb), dim3(256), 0, s, P, A);
+}
+
+// --- the reference's separate kernels (PpInterCalc, CirPWCalc,
+// UpdateSystem), accumulating into the force arrays -----------------------
+template <bool WET>
+__global__ void __launch_bounds__(256) k_pp_force(DevInst P, StepArgs A) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= A.total) return;
+    V3 F = v3(A.Fx[i], A.Fy[i], A.Fz[i]), M = v3(A.Mx[i], A.My[i], A.Mz[i]);
+    double Mr = A.Mr[i];
+    int nbridge = 0;
+    pp_particle<WET>(P, A, i, F, M, Mr, nbridge);
+    if (WET) A.out.lqd[i] = A.out.lqd[i] + nbridge;
+    A.Fx[i] = F.x; A.Fy[i] = F.y; A.Fz[i] = F.z;
+    A.Mx[i] = M.x; A.My[i] = M.y; A.Mz[i] = M.z;
+    A.Mr[i] = Mr;
+}
+
+__global__ void __launch_bounds__(256) k_pw_force(DevInst P, StepArgs A) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= A.total) return;
+    V3 F = v3(A.Fx[i], A.Fy[i], A.Fz[i]), M = v3(A.Mx[i], A.My[i], A.Mz[i]);
+    double Mr = A.Mr[i];
+    pw_particle(P, A, i, F, M, Mr);
+    A.Fx[i] = F.x; A.Fy[i] = F.y; A.Fz[i] = F.z;
+    A.Mx[i] = M.x; A.My[i] = M.y; A.Mz[i] = M.z;
+    A.Mr[i] = Mr;
+}
+
+__global__ void __launch_bounds__(256) k_update(DevInst P, StepArgs A, int *count) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    bool over = false;
+    if (i < A.total) {
+        V3 F = v3(A.Fx[i], A.Fy[i], A.Fz[i]), M = v3(A.Mx[i], A.My[i], A.Mz[i]);
+        over = integrate(P, A, i, F, M, A.Mr[i]);
+    }
+    unsigned long long b = __ballot(over);
+    if (b != 0ull && (threadIdx.x & 63) == __ffsll((long long)b) - 1)
+        atomicAdd(count, (int)__popcll(b));
+}
+
+void launch_pp_force(const DevInst &P, const StepArgs &A, bool wet, hipStream_t s) {
+    if (A.total <= 0) return;
+    if (wet)
+        hipLaunchKernelGGL(k_pp_force<true>, dim3(blocks_for(A.total, 256)), dim3(256), 0, s, P, A);
+    else
+        hipLaunchKernelGGL(k_pp_force<false>, dim3(blocks_for(A.total, 256)), dim3(256), 0, s, P,
+                           A);
+}
+void launch_pw_force(const DevInst &P, const StepArgs &A, hipStream_t s) {
+    if (A.total <= 0) return;
+    hipLaunchKernelGGL(k_pw_force, dim3(blocks_for(A.total, 256)), dim3(256), 0, s, P, A);
+}
+void launch_update(const DevInst &P, const StepArgs &A, int *count, hipStream_t s) {
+    if (A.total <= 0) return;
+    hipLaunchKernelGGL(k_update, dim3(blocks_for(A.total, 256)), dim3(256), 0, s, P, A, count);
+}
+
+// ---------------------------------------------------------------------
+// import / export (original particle order)
+// ---------------------------------------------------------------------
+__global__ void k_gather3(const int *id, const double *a, const double *b, const double *c,
+                          double *out, int total) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    long o = (long)id[i] * 3;
+    out[o] = a[i]; out[o + 1] = b[i]; out[o + 2] = c[i];
+}
+__global__ void k_gather1(const int *id, const double *a, double *out, int total) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    out[id[i]] = a[i];
+}
+__global__ void k_gather1i(const int *id, const int *a, int *out, int total) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    out[id[i]] = a[i];
+}
+__global__ void k_scatter3(const double *in, double *a, double *b, double *c, int n, int off) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    a[off + i] = in[3 * (long)i]; b[off + i] = in[3 * (long)i + 1]; c[off + i] = in[3 * (long)i + 2];
+}
+void launch_gather3(const int *id, const double *a, const double *b, const double *c, double *out,
+                    int total, hipStream_t s) {
+    if (total > 0)
+        hipLaunchKernelGGL(k_gather3, dim3(blocks_for(total, 256)), dim3(256), 0, s, id, a, b, c,
+                           out, total);
+}
+void launch_gather1(const int *id, const double *a, double *out, int total, hipStream_t s) {
+    if (total > 0)
+        hipLaunchKernelGGL(k_gather1, dim3(blocks_for(total, 256)), dim3(256), 0, s, id, a, out,
+                           total);
+}
+void launch_gather1i(const int *id, const int *a, int *out, int total, hipStream_t s) {
+    if (total > 0)
+        hipLaunchKernelGGL(k_gather1i, dim3(blocks_for(total, 256)), dim3(256), 0, s, id, a, out,
+                           total);
+}
+void launch_scatter3(const double *in, double *a, double *b, double *c, int n, int off,
+                     hipStream_t s) {
+    if (n > 0)
+        hipLaunchKernelGGL(k_scatter3, dim3(blocks_for(n, 256)), dim3(256), 0, s, in, a, b, c, n,
+                           off);
+}
+
+// ---------------------------------------------------------------------
+// contact statistics (for the roofline bytes)
+// ---------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_stats(DevInst P, StepArgs A, unsigned long long *out) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned long long nn = 0, nc = 0, nw = 0, nwc = 0, nbr = 0;
+    if (i < A.total) {
+        const StateBuf &s = A.in;
+        nn = A.nnb[i];
+        for (int k = 0; k < (int)nn; ++k) {
+            long sk = (long)k * A.cap + i;
+            int j = A.nbr[sk];
+            double dx = s.x[i] - s.x[j], dy = s.y[i] - s.y[j], dz = s.z[i] - s.z[j];
+            double gap = sqrt(dx * dx + dy * dy + dz * dz) - A.rad[i] - A.rad[j];
+            if (gap < 0.0) nc++;
+            if ((A.h.flags[sk] >> 8) & 1) nbr++;
+        }
+        nw = A.nwl[i];
+        for (int k = 0; k < (int)nw; ++k) {
+            int w = A.wl[(long)k * A.cap + i];
+            double gap;
+            if (w == 0) gap = (A.Tw - s.z[i]) - A.rad[i];
+            else if (w == 1) gap = (s.z[i] - P.Bw) - A.rad[i];
+            else {
+                double ax = P.cirX - s.x[i], ay = P.cirY - s.y[i];
+                gap = (P.R - sqrt(ax * ax + ay * ay)) - A.rad[i];
+            }
+            if (gap < 0.0) nwc++;
+        }
+    }
+    __shared__ unsigned long long red[5][4];
+    unsigned long long v[5] = {nn, nc, nw, nwc, nbr};
+    for (int q = 0; q < 5; ++q) {
+        unsigned long long t = v[q];
+        for (int d = 32; d >= 1; d >>= 1) t += __shfl_xor(t, d, 64);
+        if ((threadIdx.x & 63) == 0) red[q][threadIdx.x >> 6] = t;
+    }
+    __syncthreads();
+    if (threadIdx.x < 5) {
+        unsigned long long t = 0;
+        for (int w = 0; w < 4; ++w) t += red[threadIdx.x][w];
+        atomicAdd(&out[threadIdx.x], t);
+    }
+}
+
+void launch_stats(const DevInst &P, const StepArgs &A, unsigned long long *out, hipStream_t s) {
+    if (A.total <= 0) return;
+    hipLaunchKernelGGL(k_stats, dim3(blocks_for(A.total, 256)), dim3(256), 0, s, P, A, out);
+}

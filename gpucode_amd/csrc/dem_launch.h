@@ -1,0 +1,45 @@
+// dem_launch.h -- host-side launchers of the kernels in dem_kernels.hip.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "dem_internal.h"
+
+void launch_fill_i32(int *p, int v, long n, hipStream_t s);
+void launch_fill_f64(double *p, double v, long n, hipStream_t s);
+
+long scan_tmp_elems(long n);
+void launch_excl_scan(const unsigned *in, unsigned *out, long n, unsigned *tmp, hipStream_t s);
+long radix_tmp_elems(long n);
+void launch_radix_sort(unsigned *k, unsigned *v, unsigned *k2, unsigned *v2, long n, int bits,
+                       unsigned *tmp, hipStream_t s);
+
+void launch_calc_hash(const StateBuf &s, const int *level, const int *id, const DevInst &P,
+                      unsigned *keys, unsigned *vals, int total, hipStream_t st);
+void launch_permute(const unsigned *perm, const StateBuf &a, const StateBuf &b,
+                    const double *rad_a, double *rad_b, const int *lvl_a, int *lvl_b,
+                    const int *id_a, int *id_b, double *pmove, int total, hipStream_t s);
+void launch_cell_range(const unsigned *keys, int *cstart, int *cend, int ncells, int total,
+                       hipStream_t s);
+void launch_neighbor_find(const StateBuf &s, const double *rad, const int *level,
+                          const DevInst &P, const int *cstart, const int *cend, int *nbr, int *nnb,
+                          int *overflow, int cap, int total, hipStream_t st);
+void launch_wall_list(const StateBuf &s, const double *rad, const DevInst &P, double Tw, int *wl,
+                      int *nwl, int cap, int total, hipStream_t st);
+void launch_hist_remap(const unsigned *perm, const int *id_new, const int *id_old, const int *nbr,
+                       const int *nnb, const int *nbr_old, const int *nnb_old, HistPtr h,
+                       HistPtr h_old, const int *wl, const int *nwl, const int *wl_old,
+                       const int *nwl_old, HistPtr hw, HistPtr hw_old, int cap, int total,
+                       int old_total, hipStream_t s);
+
+void launch_step_fused(const DevInst &P, const StepArgs &A, bool wet, hipStream_t s);
+void launch_pp_force(const DevInst &P, const StepArgs &A, bool wet, hipStream_t s);
+void launch_pw_force(const DevInst &P, const StepArgs &A, hipStream_t s);
+void launch_update(const DevInst &P, const StepArgs &A, int *count, hipStream_t s);
+
+void launch_gather3(const int *id, const double *a, const double *b, const double *c, double *out,
+                    int total, hipStream_t s);
+void launch_gather1(const int *id, const double *a, double *out, int total, hipStream_t s);
+void launch_gather1i(const int *id, const int *a, int *out, int total, hipStream_t s);
+void launch_scatter3(const double *in, double *a, double *b, double *c, int n, int off,
+                     hipStream_t s);
+void launch_stats(const DevInst &P, const StepArgs &A, unsigned long long *out, hipStream_t s);

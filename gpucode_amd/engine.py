@@ -1,0 +1,243 @@
+"""Python host wrapper of the DEM engine's C ABI.
+
+Mirrors the reference's host-side flow (main.cpp:10-404 setup, 425-623 time
+loop) on top of include/dem_capi.h.  Every method calls into
+libgpucode_amd.so; nothing here computes physics.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _capi as A
+from ._capi import check, dptr, iptr
+
+
+def derive(inp, dia=None):
+    """Host-side derived setup: returns (DemSetup, rad, pLevel, neborIdx)."""
+    n = inp.NP
+    S = A.DemSetup()
+    rad = np.zeros(n)
+    lvl = np.zeros(n, np.int32)
+    idx = np.zeros(n, np.int32)
+    d = None if dia is None else np.ascontiguousarray(dia, np.float64)
+    check(A.lib().dem_derive(C.byref(inp), dptr(d), C.byref(S), dptr(rad), iptr(lvl), iptr(idx)))
+    return S, rad, lvl, idx
+
+
+def read_deck(path):
+    inp = A.DemInput()
+    check(A.lib().dem_read_deck(str(path).encode(), C.byref(inp)))
+    return inp
+
+
+def particle_feed(setup, rad, pos, n_placed, seed):
+    """ParticleFeed: fills pos[n_placed:] in place; returns (count, seed)."""
+    s = C.c_ulonglong(seed)
+    r = A.lib().dem_particle_feed(C.byref(setup), dptr(rad), dptr(pos), int(n_placed),
+                                  int(len(rad)), C.byref(s))
+    if r < 0:
+        check(A.DEM_ERR_ARG if r == -1 else r)
+    return r, s.value
+
+
+def write_tec(path, outs, time_s, pos, rad, vel, angv, force):
+    n = len(pos)
+    arrs = [np.ascontiguousarray(a, np.float64) for a in (pos, rad, vel, angv, force)]
+    check(A.lib().dem_write_tec(str(path).encode(), int(outs), float(time_s), n,
+                                *[dptr(a) for a in arrs]))
+
+
+def load_para(path):
+    v = [C.c_int(), C.c_int(), C.c_int()]
+    check(A.lib().dem_load_para(str(path).encode(), *[C.byref(x) for x in v]))
+    return tuple(x.value for x in v)
+
+
+class Engine:
+    """One packing on the GPU (a dem_ctx)."""
+
+    def __init__(self, inp, dia=None, precision=A.PREC_FP64):
+        self._ctx = C.c_void_p()
+        d = None if dia is None else np.ascontiguousarray(dia, np.float64)
+        check(A.lib().dem_create(C.byref(inp), dptr(d), int(precision), C.byref(self._ctx)))
+        self.inp = inp
+        self.NP = inp.NP
+        self.setup = A.DemSetup()
+        check(A.lib().dem_get_setup(self._ctx, C.byref(self.setup)))
+        self.K = max(self.setup.levelNNebors[l] for l in range(self.setup.nLevels))
+
+    def close(self):
+        if self._ctx:
+            A.lib().dem_destroy(self._ctx)
+            self._ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # --- state ------------------------------------------------------------
+    @property
+    def total(self):
+        return A.lib().dem_total(self._ctx)
+
+    def set_particles(self, pos, vel=None, angv=None):
+        pos = np.ascontiguousarray(pos, np.float64)
+        v = None if vel is None else np.ascontiguousarray(vel, np.float64)
+        w = None if angv is None else np.ascontiguousarray(angv, np.float64)
+        check(A.lib().dem_set_particles(self._ctx, len(pos), dptr(pos), dptr(v), dptr(w)))
+
+    def feed(self, pos):
+        pos = np.ascontiguousarray(pos, np.float64)
+        check(A.lib().dem_feed(self._ctx, len(pos), dptr(pos)))
+
+    def particles(self):
+        n = self.total
+        pos, vel, w = (np.zeros((n, 3)) for _ in range(3))
+        check(A.lib().dem_get_particles(self._ctx, dptr(pos), dptr(vel), dptr(w)))
+        return pos, vel, w
+
+    def forces(self):
+        n = self.total
+        F, M = np.zeros((n, 3)), np.zeros((n, 3))
+        Mr = np.zeros(n)
+        check(A.lib().dem_get_forces(self._ctx, dptr(F), dptr(M), dptr(Mr)))
+        return F, M, Mr
+
+    def store_forces(self, on=True):
+        check(A.lib().dem_set_store_forces(self._ctx, int(bool(on))))
+
+    def pmove(self):
+        a = np.zeros(self.total)
+        check(A.lib().dem_get_pmove(self._ctx, dptr(a)))
+        return a
+
+    def lqdcnt(self):
+        a = np.zeros(self.total, np.int32)
+        check(A.lib().dem_get_lqdcnt(self._ctx, iptr(a)))
+        return a
+
+    def nebor(self, maxK=None):
+        maxK = maxK or self.K
+        n = self.total
+        num = np.zeros(n, np.int32)
+        lst = np.zeros((n, maxK), np.int32)
+        check(A.lib().dem_get_nebor(self._ctx, iptr(num), iptr(lst), maxK))
+        return num, lst
+
+    def hist_pp(self, maxK=None):
+        maxK = maxK or self.K
+        n = self.total
+        d = np.zeros((n, maxK, 3))
+        f = [np.zeros((n, maxK)) for _ in range(4)]
+        s, b = np.zeros((n, maxK), np.int32), np.zeros((n, maxK), np.int32)
+        check(A.lib().dem_get_hist_pp(self._ctx, maxK, dptr(d), *[dptr(x) for x in f],
+                                      iptr(s), iptr(b)))
+        return dict(dispt=d, nDsp=f[0], nDsp_mx=f[1], plstRad=f[2], plstDfm=f[3], stage=s,
+                    lqdBrg=b)
+
+    def walls(self):
+        n = self.total
+        num = np.zeros(n, np.int32)
+        lst = np.zeros((n, A.WALLS), np.int32)
+        check(A.lib().dem_get_walls(self._ctx, iptr(num), iptr(lst)))
+        return num, lst
+
+    def hist_pw(self):
+        n = self.total
+        d = np.zeros((n, A.WALLS, 3))
+        f = [np.zeros((n, A.WALLS)) for _ in range(4)]
+        s, b = np.zeros((n, A.WALLS), np.int32), np.zeros((n, A.WALLS), np.int32)
+        check(A.lib().dem_get_hist_pw(self._ctx, dptr(d), *[dptr(x) for x in f], iptr(s),
+                                      iptr(b)))
+        return dict(dispt=d, nDsp=f[0], nDsp_mx=f[1], plstRad=f[2], plstDfm=f[3], stage=s,
+                    lqdBrg=b)
+
+    def hash_index(self):
+        n = self.total
+        h, i = np.zeros(n, np.int32), np.zeros(n, np.int32)
+        check(A.lib().dem_get_hash(self._ctx, iptr(h), iptr(i)))
+        return h, i
+
+    def cells(self):
+        n = self.setup.sysNCells
+        s, e = np.zeros(n, np.int32), np.zeros(n, np.int32)
+        check(A.lib().dem_get_cells(self._ctx, iptr(s), iptr(e)))
+        return s, e
+
+    # --- stepping -----------------------------------------------------------
+    def rebuild(self):
+        check(A.lib().dem_rebuild(self._ctx))
+
+    def step(self, n=1):
+        check(A.lib().dem_step(self._ctx, int(n)))
+
+    def step_unfused(self, n=1):
+        for _ in range(n):
+            check(A.lib().dem_step_unfused(self._ctx))
+
+    def var_init(self):
+        check(A.lib().dem_var_init(self._ctx))
+
+    def pp_inter_calc(self):
+        check(A.lib().dem_pp_inter_calc(self._ctx))
+
+    def cir_pw_calc(self):
+        check(A.lib().dem_cir_pw_calc(self._ctx))
+
+    def update_system(self):
+        c = C.c_int()
+        check(A.lib().dem_update_system(self._ctx, C.byref(c)))
+        return c.value
+
+    def time_steps(self, n):
+        ms, rb = C.c_double(), C.c_int()
+        check(A.lib().dem_time_steps(self._ctx, int(n), C.byref(ms), C.byref(rb)))
+        return ms.value, bool(rb.value)
+
+    def sync(self):
+        check(A.lib().dem_sync(self._ctx))
+
+    @property
+    def iter(self):
+        return A.lib().dem_iter(self._ctx)
+
+    @property
+    def ctime(self):
+        return A.lib().dem_ctime(self._ctx)
+
+    @property
+    def n_rebuild(self):
+        return A.lib().dem_n_rebuild(self._ctx)
+
+    def stats(self):
+        s = A.DemStats()
+        check(A.lib().dem_contact_stats(self._ctx, C.byref(s)))
+        return {f: getattr(s, f) for f, _ in A.DemStats._fields_}
+
+    def timing(self):
+        ms, st, la, rms, nr = (C.c_double(), C.c_longlong(), C.c_longlong(), C.c_double(),
+                               C.c_int())
+        check(A.lib().dem_get_timing(self._ctx, C.byref(ms), C.byref(st), C.byref(la),
+                                     C.byref(rms), C.byref(nr)))
+        return dict(fused_ms=ms.value, fused_steps=st.value, fused_launches=la.value,
+                    rebuild_ms=rms.value, n_rebuild=nr.value)
+
+    def device_bytes(self):
+        return A.lib().dem_device_bytes(self._ctx)
+
+    # --- restart --------------------------------------------------------------
+    def dump(self, path, nBatch=1):
+        check(A.lib().dem_dump_write(self._ctx, str(path).encode(), int(nBatch)))
+
+    def load(self, path):
+        nb = C.c_int()
+        check(A.lib().dem_dump_read(self._ctx, str(path).encode(), C.byref(nb)))
+        return nb.value

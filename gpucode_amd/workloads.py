@@ -1,0 +1,168 @@
+"""Seeded synthetic workloads of the DEM engine.
+
+Each builder returns a dict with the input deck (DemInput), the diameters
+[m] (or None for monodisperse), initial positions in reduced units and the
+step count of the workload:
+
+  dry_column   2,000 monodisperse spheres d = 1 mm, rho 2500, E 70 GPa,
+               nu 0.25, mu 0.3, e 0.7, yield at 1 % d, cylinder R 10 mm with
+               a floor, random non-overlapping start in a 10 x 60 mm column,
+               dt 1e-7 s, 20,000 steps, cell 1.1 d
+  wet_poly     lognormal diameters (median 0.5 mm, sigma 0.25, clipped
+               0.25-1.0 mm), capillary bridges at 1 % liquid, gamma 0.072,
+               theta 20 deg, rupture (1 + theta/2) V^(1/3), cylinder R 25 mm
+  packed_bed   a dense bed of monodisperse spheres (2,000,000 of d = 0.3 mm
+               in R 30 mm, skin 0.1 d, mu 0.5 for the throughput case)
+
+Positions come from the engine's own ParticleFeed (seeded) or from
+pocket_bed(), a layer-by-layer deposition onto triangular pockets.
+"""
+import math
+
+import numpy as np
+
+from . import _capi as A
+from .calibrate import restitution_to_dmpn, yield_for_overlap
+from .engine import derive, particle_feed
+
+
+def make_input(NP, d_max, R, H, mu, *, E=70e9, nu=0.25, rho=2500.0, e=0.7, yield_frac=0.01,
+               rfrc=0.01, cutGap=0.1, expand=2.0, dt=None, dtFactor=1.0, g=9.81, nLevels=1,
+               forceCap=0, liquid=0.0, tension=0.072, gapmn=1e-6, theta=0.0, rupture_mode=0,
+               mu_wall=None, feed=None, impact_speed=0.5):
+    inp = A.DemInput()
+    inp.NP = NP
+    inp.nLevels = nLevels
+    inp.loadStage = 1
+    inp.forceCap = forceCap
+    inp.tPacking = 1.0
+    inp.realSize = d_max
+    inp.cutGap = cutGap
+    inp.expandSize = expand
+    inp.cirX, inp.cirY, inp.Bw, inp.R, inp.H = 0.0, 0.0, 0.0, R, H
+    lo, hi = feed if feed else (0.5 * d_max, H - 0.5 * d_max)
+    inp.feedLmin, inp.feedLmax = lo, hi
+    dmpn = restitution_to_dmpn(e, d_max, rho, E, nu, impact_speed)
+    yldp = yield_for_overlap(E, nu, yield_frac)
+    mw = mu if mu_wall is None else mu_wall
+    for m, row in enumerate(((rho, E, nu, mu, rfrc, dmpn, yldp), (rho, E, nu, mw, rfrc, dmpn, yldp))):
+        for k, v in enumerate(row):
+            inp.mat[m][k] = v
+    inp.stension = tension
+    inp.vol = liquid
+    inp.gapmn = gapmn
+    inp.dtFactor = dtFactor
+    inp.nOutPacking = 1
+    inp.gravity = g
+    inp.dtOverride = dt or 0.0
+    inp.theta = theta
+    inp.ruptureMode = rupture_mode
+    return inp
+
+
+def feed_positions(inp, dia, n, seed):
+    S, rad, _, _ = derive(inp, dia)
+    pos = np.zeros((inp.NP, 3))
+    count, _ = particle_feed(S, rad, pos, 0, seed)
+    if count < n:
+        raise RuntimeError(f"feed placed only {count} of {n} particles")
+    return pos[:n].copy()
+
+
+def dry_column(n=2000, seed=0xC1, steps=20000):
+    """2,000 dry monodisperse spheres in a 10 x 60 mm column."""
+    d = 1.0e-3
+    inp = make_input(n, d, 10e-3, 60e-3, 0.3, cutGap=0.1 / 1.4, dt=1e-7)
+    return dict(name="dry_column", inp=inp, dia=None, pos=feed_positions(inp, None, n, seed),
+                steps=steps)
+
+
+def lognormal_diameters(n, seed, median=0.5e-3, sigma=0.25, lo=0.25e-3, hi=1.0e-3):
+    rng = np.random.default_rng(seed)
+    return np.clip(median * np.exp(sigma * rng.standard_normal(n)), lo, hi)
+
+
+def wet_poly(n=200_000, seed=0xC2, R=25e-3, H=60e-3, steps=50_000, theta_deg=20.0,
+             reference_law=False, nLevels=3, dt=None):
+    """Wet polydisperse packing; reference_law selects theta = 0 with the
+    reference's rupture rule (kernel.cu:522)."""
+    dia = lognormal_diameters(n, seed)
+    th = 0.0 if reference_law else math.radians(theta_deg)
+    inp = make_input(n, float(dia.max()), R, H, 0.3, cutGap=0.1, nLevels=nLevels, forceCap=1,
+                     liquid=0.01, theta=th, rupture_mode=0 if reference_law else 1, dt=dt)
+    return dict(name="wet_poly", inp=inp, dia=dia, pos=feed_positions(inp, dia, n, seed + 1),
+                steps=steps)
+
+
+def pocket_bed(n, R, d=1.0, a=1.10, jitter=0.03, seed=0, z0=0.0):
+    """Dense bed of n spheres (diameter d, reduced units) in a cylinder of
+    radius R centred at the origin, floor at z0.  Layers are triangular
+    lattices of spacing a stacked into each other's pockets (ABC order);
+    every site is jittered in the plane and then lowered onto the highest of
+    the spheres below it, so each sphere rests on at least one support."""
+    from scipy.spatial import cKDTree
+
+    rng = np.random.default_rng(seed)
+    r = d / 2
+    s3 = math.sqrt(3.0)
+    nmax = int(2 * R / a) + 4
+    ii, jj = np.meshgrid(np.arange(-nmax, nmax + 1), np.arange(-nmax, nmax + 1), indexing="ij")
+    ii, jj = ii.ravel().astype(float), jj.ravel().astype(float)
+    out = []
+    total = 0
+    prev_xy = prev_z = tree = None
+    L = 0
+    h = math.sqrt(max(d * d - a * a / 3.0, 0.25 * d * d))
+    while total < n:
+        ox, oy = (L % 3) * a / 2, (L % 3) * a / (2 * s3)
+        x = ii * a + jj * a / 2 + ox
+        y = jj * a * s3 / 2 + oy
+        x = x + jitter * a * rng.uniform(-1, 1, x.shape)
+        y = y + jitter * a * rng.uniform(-1, 1, y.shape)
+        keep = x * x + y * y <= (R - r - 1e-9) ** 2
+        x, y = x[keep], y[keep]
+        if L == 0:
+            z = np.full(x.shape, z0 + r)
+        else:
+            xy = np.stack([x, y], 1)
+            dist, idx = tree.query(xy, k=8, distance_upper_bound=d)
+            z = np.full(x.shape, -np.inf)
+            for c in range(dist.shape[1]):
+                ok = np.isfinite(dist[:, c])
+                cand = np.full(x.shape, -np.inf)
+                j = idx[ok, c]
+                rho = dist[ok, c]
+                cand[ok] = prev_z[j] + np.sqrt(np.maximum(d * d - rho * rho, 0.0))
+                z = np.maximum(z, cand)
+            nominal = z0 + r + L * h
+            z = np.where(np.isfinite(z), z, nominal)
+        layer = np.stack([x, y, z], 1)
+        if total + len(layer) > n:
+            pick = np.sort(rng.choice(len(layer), n - total, replace=False))
+            layer = layer[pick]
+        out.append(layer)
+        total += len(layer)
+        prev_xy = np.stack([x, y], 1)
+        prev_z = z
+        tree = cKDTree(prev_xy)
+        L += 1
+    return np.concatenate(out)[:n]
+
+
+def packed_bed(n=2_000_000, d=0.3e-3, R=30e-3, H=30e-3, mu=0.5, skin=0.1, seed=0xC3,
+               steps=10_000, dt=None, jitter=0.03):
+    """Dense monodisperse bed (the 2M-sphere throughput case by default)."""
+    inp = make_input(n, d, R, H, mu, cutGap=skin, dt=dt)
+    pos = pocket_bed(n, R / d, 1.0, jitter=jitter, seed=seed)
+    if pos[:, 2].max() + 0.5 > H / d:
+        raise ValueError("bed taller than the cylinder")
+    return dict(name="packed_bed", inp=inp, dia=None, pos=pos, steps=steps)
+
+
+def small_bed(n=3000, R_d=8.0, mu=0.5, seed=7, dt=None, jitter=0.03):
+    """A small dense bed for contact-heavy parity tests (d = 1 mm)."""
+    d = 1.0e-3
+    H = d * (n / (math.pi * R_d * R_d * 0.5) + 6)
+    inp = make_input(n, d, R_d * d, H, mu, cutGap=0.1, dt=dt)
+    pos = pocket_bed(n, R_d, 1.0, jitter=jitter, seed=seed)
+    return dict(name="small_bed", inp=inp, dia=None, pos=pos, steps=2000)

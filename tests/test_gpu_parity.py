@@ -1,0 +1,295 @@
+"""GPU parity: the HIP engine (through the C ABI) against the FP64 oracle on
+identical seeded inputs.
+
+Bars (also stated per test):
+  * integer work (cell hash, sorted index, cell ranges, neighbour and wall
+    lists, history slots) -- bit-exact, including slot order;
+  * FP64 path, one step -- per-particle force error / (sum_j |f_ij| + m g)
+    <= 1e-12 (the GPU uses x*sqrt(x) and sqrt where the reference calls pow,
+    a few ulp apart);
+  * FP64 path, N steps -- positions within 1e-9 d, velocities within 1e-7 of
+    the largest speed (differences grow only from those ulps).
+"""
+import math
+
+import numpy as np
+import pytest
+
+import gpucode_amd as G
+from gpucode_amd import workloads as W
+from oracle import pyoracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def pair(w, precision=G.PREC_FP64, vel=None, angv=None):
+    e = G.Engine(w["inp"], w["dia"], precision)
+    o = O.Oracle(w["inp"], w["dia"])
+    e.set_particles(w["pos"], vel, angv)
+    o.set_particles(w["pos"], vel, angv)
+    return e, o
+
+
+def force_scale(o, e_forces=None):
+    """sum_j |f_ij| + m g per particle from the oracle's contact magnitudes."""
+    F, M, Mr = o.forces()
+    S = o.S
+    pos = o.particles()[0]
+    num, lst = o.nebor()
+    h = o.hist_pp()
+    n = len(pos)
+    scale = np.zeros(n)
+    rad = None
+    # bound |f_ij| by the normal force magnitude implied by the overlap
+    p = pos
+    for i in range(n):
+        for k in range(num[i]):
+            j = lst[i, k]
+            d = np.linalg.norm(p[i] - p[j])
+            ov = 1.0 - d if rad is None else rad[i] + rad[j] - d
+            if ov > 0:
+                scale[i] += 4 / 3 * S.pp.emod * math.sqrt(0.25) * ov ** 1.5 * (1 + S.pp.sfrc)
+    return scale + 1.0
+
+
+def test_hash_sort_cells_bit_exact():
+    w = W.small_bed(n=3000)
+    e, o = pair(w)
+    e.rebuild()
+    o.rebuild()
+    h1, i1 = e.hash_index()
+    h2, i2 = o.hash_index()
+    assert np.array_equal(h1, h2) and np.array_equal(i1, i2)
+    s1, t1 = e.cells()
+    s2, t2 = o.cells()
+    assert np.array_equal(s1, s2) and np.array_equal(t1, t2)
+
+
+def test_hash_sort_polydisperse_levels():
+    w = W.wet_poly(n=4000, R=10e-3, H=60e-3, nLevels=3)
+    e, o = pair(w)
+    e.rebuild()
+    o.rebuild()
+    assert np.array_equal(e.hash_index()[0], o.hash_index()[0])
+    assert np.array_equal(e.hash_index()[1], o.hash_index()[1])
+    for a, b in zip(e.cells(), o.cells()):
+        assert np.array_equal(a, b)
+    n1, l1 = e.nebor(o.K)
+    n2, l2 = o.nebor()
+    assert np.array_equal(n1, n2) and np.array_equal(l1, l2)
+
+
+def test_neighbour_and_wall_lists_bit_exact():
+    w = W.small_bed(n=3000)
+    e, o = pair(w)
+    e.rebuild()
+    o.rebuild()
+    n1, l1 = e.nebor()
+    n2, l2 = o.nebor()
+    assert np.array_equal(n1, n2)
+    assert np.array_equal(l1, l2)  # same slot order, as original ids
+    w1, wl1 = e.walls()
+    w2, wl2 = o.walls()
+    assert np.array_equal(w1, w2) and np.array_equal(wl1, wl2)
+
+
+def test_radix_sort_edge_sizes():
+    # tile boundaries of the sort (2048 keys per block) and a single particle
+    for n in (1, 2, 2047, 2048, 2049, 5000):
+        w = W.small_bed(n=max(n, 2))
+        pos = w["pos"][:n]
+        e = G.Engine(w["inp"])
+        o = O.Oracle(w["inp"])
+        e.set_particles(pos)
+        o.set_particles(pos)
+        e.rebuild()
+        o.rebuild()
+        assert np.array_equal(e.hash_index()[1], o.hash_index()[1]), n
+        assert np.array_equal(e.nebor()[1], o.nebor()[1]), n
+        e.close()
+
+
+def one_step_forces(w, vel=None, angv=None, precision=G.PREC_FP64):
+    e, o = pair(w, precision, vel, angv)
+    e.store_forces(True)
+    e.step(1)
+    o.step(1)
+    return e, o
+
+
+def normalized_force_error(e, o):
+    F1, M1, Mr1 = e.forces()
+    F2, M2, Mr2 = o.forces()
+    sc = force_scale(o)
+    errF = np.linalg.norm(F1 - F2, axis=1) / sc
+    return errF, (F1, M1, Mr1), (F2, M2, Mr2), sc
+
+
+def test_one_step_forces_fp64_dense_bed():
+    w = W.small_bed(n=3000)
+    rng = np.random.default_rng(5)
+    vel = rng.normal(size=(3000, 3)) * 1e-3
+    angv = rng.normal(size=(3000, 3)) * 1e-2
+    e, o = one_step_forces(w, vel, angv)
+    err, g, r, sc = normalized_force_error(e, o)
+    assert err.max() <= 1e-12, err.max()
+    # moments and rolling terms, relative to the same scale times the radius
+    assert (np.linalg.norm(g[1] - r[1], axis=1) / sc).max() <= 1e-12
+    assert (np.abs(g[2] - r[2]) / sc).max() <= 1e-12
+    # the history written by that step
+    h1, h2 = e.hist_pp(), o.hist_pp()
+    assert np.array_equal(h1["stage"], h2["stage"])
+    for f in ("nDsp", "nDsp_mx", "plstRad", "plstDfm"):
+        assert np.allclose(h1[f], h2[f], rtol=1e-13, atol=1e-18), f
+    assert np.allclose(h1["dispt"], h2["dispt"], rtol=1e-12, atol=1e-20)
+
+
+def compare_state(e, o, pos_tol=1e-9, vel_rel=1e-7):
+    p1, v1, w1 = e.particles()
+    p2, v2, w2 = o.particles()
+    vs = max(np.abs(v2).max(), 1e-12)
+    ws = max(np.abs(w2).max(), 1e-12)
+    assert np.abs(p1 - p2).max() <= pos_tol, np.abs(p1 - p2).max()
+    assert np.abs(v1 - v2).max() <= vel_rel * vs, np.abs(v1 - v2).max() / vs
+    assert np.abs(w1 - w2).max() <= 1e-6 * ws, np.abs(w1 - w2).max() / ws
+
+
+def test_trajectory_fp64_dense_bed_300_steps():
+    w = W.small_bed(n=3000)
+    e, o = pair(w)
+    e.step(300)
+    o.step(300)
+    compare_state(e, o)
+    assert e.iter == o.iter == 300
+    assert e.n_rebuild == o.n_rebuild
+    n1, l1 = e.nebor()
+    n2, l2 = o.nebor()
+    assert np.array_equal(n1, n2) and np.array_equal(l1, l2)
+    s = e.stats()
+    assert s["n_contact"] > 3000  # a contact-heavy case
+
+
+def test_fused_equals_separate_kernels():
+    w = W.small_bed(n=2000)
+    rng = np.random.default_rng(2)
+    vel = rng.normal(size=(2000, 3)) * 1e-2
+    a = G.Engine(w["inp"])
+    b = G.Engine(w["inp"])
+    for x in (a, b):
+        x.set_particles(w["pos"], vel)
+    a.step(100)
+    b.step_unfused(100)
+    for u, v in zip(a.particles(), b.particles()):
+        assert np.array_equal(u, v)
+    assert a.n_rebuild == b.n_rebuild
+
+
+def test_dry_column_20000_steps():
+    w = W.dry_column()
+    e, o = pair(w)
+    O.threads(16)
+    e.step(w["steps"])
+    o.step(w["steps"])
+    compare_state(e, o, pos_tol=1e-8, vel_rel=1e-6)
+    assert e.n_rebuild == o.n_rebuild
+
+
+def test_rebuilds_triggered_on_device_match_oracle():
+    # a poured column with fast particles: many rebuilds inside step chunks
+    w = W.dry_column()
+    rng = np.random.default_rng(11)
+    vel = rng.normal(size=(2000, 3)) * 0.5
+    e, o = pair(w, vel=vel)
+    e.step(3000)
+    o.step(3000)
+    assert o.n_rebuild > 20
+    assert e.n_rebuild == o.n_rebuild
+    compare_state(e, o, pos_tol=1e-8, vel_rel=1e-6)
+
+
+def test_wet_polydisperse_parity():
+    w = W.wet_poly(n=3000, R=8e-3, H=60e-3, nLevels=3, reference_law=True)
+    # a compact wet start: shrink the column so bridges form
+    w["pos"][:, 2] = w["pos"][:, 2] * 0.25 + 0.5
+    e, o = pair(w)
+    e.step(200)
+    o.step(200)
+    compare_state(e, o, pos_tol=1e-9, vel_rel=1e-6)
+    assert np.array_equal(e.lqdcnt(), o.lqdcnt())
+    h1, h2 = e.hist_pp(o.K), o.hist_pp()
+    assert np.array_equal(h1["lqdBrg"], h2["lqdBrg"])
+    assert h2["lqdBrg"].sum() > 100
+
+
+def test_wet_theta_law_runs_and_matches():
+    w = W.wet_poly(n=2000, R=8e-3, H=60e-3, nLevels=2, theta_deg=20.0)
+    w["pos"][:, 2] = w["pos"][:, 2] * 0.25 + 0.5
+    e, o = pair(w)
+    e.step(100)
+    o.step(100)
+    compare_state(e, o, pos_tol=1e-9, vel_rel=1e-6)
+
+
+def test_feed_then_rebuild_matches_oracle():
+    w = W.dry_column()
+    half = 1000
+    e = G.Engine(w["inp"])
+    o = O.Oracle(w["inp"])
+    e.set_particles(w["pos"][:half])
+    o.set_particles(w["pos"][:half])
+    e.step(50)
+    o.step(50)
+    e.feed(w["pos"][half:])
+    o.feed(w["pos"][half:])
+    e.step(50)
+    o.step(50)
+    assert e.total == o.total == 2000
+    compare_state(e, o)
+    assert np.array_equal(e.nebor()[1], o.nebor()[1])
+
+
+def test_overflow_is_reported_not_written():
+    w = W.small_bed(n=200)
+    pos = w["pos"].copy()
+    pos[:60] = pos[0] + np.random.default_rng(0).normal(size=(60, 3)) * 0.01
+    e = G.Engine(w["inp"])
+    e.set_particles(pos)
+    with pytest.raises(G.DemError) as err:
+        e.step(1)
+    assert err.value.code == -3
+
+
+def test_dump_load_round_trip(tmp_path):
+    w = W.small_bed(n=2000)
+    a = G.Engine(w["inp"])
+    a.set_particles(w["pos"])
+    a.step(200)
+    path = tmp_path / "heyi_dump.dat"
+    a.dump(path, nBatch=4)
+    assert G.load_para(path) == (2000, 4, 2000)
+    b = G.Engine(w["inp"])
+    assert b.load(path) == 4
+    assert b.iter == a.iter
+    # a restart rebuilds the lists (main.cpp:473) and remaps the restored
+    # history; rebuild the original too so both sum in the same slot order
+    a.rebuild()
+    a.step(100)
+    b.step(100)
+    for u, v in zip(a.particles(), b.particles()):
+        assert np.array_equal(u, v)
+
+
+def test_contact_stats_match_oracle_count():
+    w = W.small_bed(n=3000)
+    e, o = pair(w)
+    e.step(50)
+    o.step(50)
+    s = e.stats()
+    p = o.particles()[0]
+    num, lst = o.nebor()
+    nc = 0
+    for i in range(len(p)):
+        d = np.linalg.norm(p[i] - p[lst[i, :num[i]]], axis=1)
+        nc += int((d - 1.0 < 0).sum())
+    assert s["n_nebor"] == num.sum()
+    assert s["n_contact"] == nc
