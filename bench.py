@@ -115,6 +115,10 @@ def main():
     if tr:
         traffic = tr.get("bytes_per_launch")
 
+    copy_gbs = G.engine.stream_copy(1 << 30, 20)
+    log(f"plain copy {copy_gbs:.0f} GB/s; fused kernel {kern_ms:.4f} ms, {achieved:.0f} GB/s "
+        f"algorithmic ({B:.1f} B/particle-step)")
+
     base = None
     if not args.no_cpu_baseline:
         thr = min(16, os.cpu_count() or 1)
@@ -147,6 +151,7 @@ def main():
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
             "kernel": "k_step_fused", "kernel_ms": kern_ms, "bytes_per_particle": B,
+            "attainable_copy_gbs": copy_gbs,
         },
         "cpu_baseline": base,
     }

@@ -126,6 +126,7 @@ _SIGS = {
     "dem_contact_stats": (C.c_int, [P, C.POINTER(DemStats)]),
     "dem_get_timing": (C.c_int, [P, D, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong), D, I]),
     "dem_device_bytes": (C.c_longlong, [P]),
+    "dem_stream_copy": (C.c_int, [C.c_longlong, C.c_int, D]),
     "dem_read_deck": (C.c_int, [C.c_char_p, C.POINTER(DemInput)]),
     "dem_particle_feed": (C.c_int, [C.POINTER(DemSetup), D, D, C.c_int, C.c_int,
                                     C.POINTER(C.c_ulonglong)]),

@@ -492,7 +492,7 @@ extern "C" int dem_var_init(dem_ctx *c) {
 extern "C" int dem_pp_inter_calc(dem_ctx *c) {
     if (!c) return dem_fail(DEM_ERR_ARG, "null context");
     StepArgs A = step_args(c, false);
-    launch_pp_force(c->P, A, c->S.forceCap == 1, c->stream);
+    launch_pp_force(c->P, A, c->S.forceCap == 1, c->precision == DEM_PREC_MIXED, c->stream);
     HIPCHK(hipGetLastError());
     return DEM_OK;
 }
@@ -501,7 +501,7 @@ extern "C" int dem_cir_pw_calc(dem_ctx *c) {
     if (!c) return dem_fail(DEM_ERR_ARG, "null context");
     c->Tw = c->Tw + c->S.topv * c->S.dt;  // kernel.cu:966-969
     StepArgs A = step_args(c, false);
-    launch_pw_force(c->P, A, c->stream);
+    launch_pw_force(c->P, A, c->precision == DEM_PREC_MIXED, c->stream);
     HIPCHK(hipGetLastError());
     return DEM_OK;
 }
@@ -552,7 +552,7 @@ static int run_chunk(dem_ctx *c, int nsteps, int *executed) {
         A.step = s;
         tw = tw + c->S.topv * c->S.dt;
         A.Tw = tw;
-        launch_step_fused(c->P, A, wet, c->stream);
+        launch_step_fused(c->P, A, wet, c->precision == DEM_PREC_MIXED, c->stream);
     }
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(c->ev1, c->stream));
@@ -632,6 +632,42 @@ extern "C" int dem_sync(dem_ctx *c) {
 extern "C" int dem_iter(dem_ctx *c) { return c ? c->iter : 0; }
 extern "C" double dem_ctime(dem_ctx *c) { return c ? c->ctime : 0.0; }
 extern "C" int dem_n_rebuild(dem_ctx *c) { return c ? c->n_rebuild : 0; }
+
+extern "C" int dem_stream_copy(long long nbytes, int iters, double *gbps) {
+    if (nbytes < 16 || iters < 1 || !gbps) return dem_fail(DEM_ERR_ARG, "bad argument");
+    void *a = nullptr, *b = nullptr;
+    hipStream_t s = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = DEM_OK;
+    do {
+        hipError_t e;
+        if ((e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking)) != hipSuccess ||
+            (e = hipMalloc(&a, (size_t)nbytes)) != hipSuccess ||
+            (e = hipMalloc(&b, (size_t)nbytes)) != hipSuccess ||
+            (e = hipEventCreate(&e0)) != hipSuccess || (e = hipEventCreate(&e1)) != hipSuccess ||
+            (e = hipMemsetAsync(a, 0, (size_t)nbytes, s)) != hipSuccess) {
+            rc = dem_fail(DEM_ERR_HIP, "stream copy setup: %s", hipGetErrorString(e));
+            break;
+        }
+        launch_stream_copy(a, b, nbytes, s);  // warm
+        (void)hipEventRecord(e0, s);
+        for (int k = 0; k < iters; ++k) launch_stream_copy(a, b, nbytes, s);
+        (void)hipEventRecord(e1, s);
+        if ((e = hipStreamSynchronize(s)) != hipSuccess) {
+            rc = dem_fail(DEM_ERR_HIP, "stream copy: %s", hipGetErrorString(e));
+            break;
+        }
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        *gbps = 2.0 * (double)nbytes * iters / (ms * 1e-3) / 1e9;
+    } while (0);
+    if (a) (void)hipFree(a);
+    if (b) (void)hipFree(b);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (s) (void)hipStreamDestroy(s);
+    return rc;
+}
 
 extern "C" int dem_contact_stats(dem_ctx *c, dem_stats *out) {
     if (!c || !out) return dem_fail(DEM_ERR_ARG, "bad argument");

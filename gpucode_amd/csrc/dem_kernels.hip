@@ -519,6 +519,70 @@ DEV V3 cross(V3 a, V3 b) {
 }
 DEV double pow15(double x) { return x * sqrt(x); }
 
+// exp, log and x^y for the FP64 path, built from correctly rounded
+// +, -, *, /, sqrt and exact scaling only, so the host restatement rounds
+// identically (the contact law is discontinuous at the stick/slip and
+// yield boundaries, where a last-bit difference between two libm's would
+// change a branch).
+//   exp: x = k ln2 + r with k = rint(x / ln2) and a two-part ln2 (the high
+//        part has 32 bits, so k * hi is exact); e^r by its Taylor series to
+//        r^14 (|r| <= ln2/2, truncation < 1e-17); times 2^k.
+//   log: x = m 2^e with m in [sqrt(1/2), sqrt(2)); s = (m - 1)/(m + 1);
+//        log m = 2 s (1 + z/3 + z^2/5 + ... + z^12/25), z = s^2 <= 0.0295.
+DEV double dm_exp(double x) {
+    if (x != x) return x;
+    if (x > 709.0) return HUGE_VAL;
+    if (x < -708.0) return 0.0;
+    const double k = rint(x * 1.4426950408889634);
+    const double r = (x - k * 0x1.62e42feep-1) - k * 0x1.a39ef35793c76p-33;
+    double p = 1.0 / 87178291200.0;
+    p = 1.0 / 6227020800.0 + r * p;
+    p = 1.0 / 479001600.0 + r * p;
+    p = 1.0 / 39916800.0 + r * p;
+    p = 1.0 / 3628800.0 + r * p;
+    p = 1.0 / 362880.0 + r * p;
+    p = 1.0 / 40320.0 + r * p;
+    p = 1.0 / 5040.0 + r * p;
+    p = 1.0 / 720.0 + r * p;
+    p = 1.0 / 120.0 + r * p;
+    p = 1.0 / 24.0 + r * p;
+    p = 1.0 / 6.0 + r * p;
+    p = 0.5 + r * p;
+    p = 1.0 + r * p;
+    p = 1.0 + r * p;
+    return ldexp(p, (int)k);
+}
+
+DEV double dm_log(double x) {
+    if (!(x > 0.0)) return x == 0.0 ? -HUGE_VAL : (x - x) / 0.0;
+    int e;
+    double m = frexp(x, &e);
+    if (m < 0.70710678118654752) {
+        m = m * 2.0;
+        e = e - 1;
+    }
+    const double f = m - 1.0;
+    const double s = f / (2.0 + f);
+    const double z = s * s;
+    double q = 1.0 / 25.0;
+    q = 1.0 / 23.0 + z * q;
+    q = 1.0 / 21.0 + z * q;
+    q = 1.0 / 19.0 + z * q;
+    q = 1.0 / 17.0 + z * q;
+    q = 1.0 / 15.0 + z * q;
+    q = 1.0 / 13.0 + z * q;
+    q = 1.0 / 11.0 + z * q;
+    q = 1.0 / 9.0 + z * q;
+    q = 1.0 / 7.0 + z * q;
+    q = 1.0 / 5.0 + z * q;
+    q = 1.0 / 3.0 + z * q;
+    const double lm = 2.0 * s + 2.0 * s * (z * q);
+    const double de = (double)e;
+    return de * 0x1.62e42feep-1 + (de * 0x1.a39ef35793c76p-33 + lm);
+}
+
+DEV double dm_pow(double x, double y) { return dm_exp(y * dm_log(x)); }
+
 struct Hist1 {
     double dx, dy, dz, nd, ndm, prad, pdfm;
     int flags;
@@ -555,7 +619,7 @@ DEV double normal_law(const DevLaw &c, Hist1 &h, double d, double R) {
         } else {
             fN = (c.yforce_t * (R * R)) + (c.ystiff_t * R) * (d - (c.ydsp_t * R));
             h.prad = R * (c.c43E * sqrt(R * d) * d) / fN;
-            h.pdfm = d - pow((fN / (c.c43E * sqrt(R))), 2.0 / 3.0);
+            h.pdfm = d - dm_pow((fN / (c.c43E * sqrt(R))), 2.0 / 3.0);
         }
         h.ndm = d;
     } else if (stage != 1) {
@@ -621,26 +685,122 @@ DEV void contact(const DevLaw &c, double rolling, double dt, Hist1 &h, double d,
     Mr = Mr + rolling * fN * ri;
 }
 
+// The same contact in mixed precision.  FP64: the overlap d and every
+// difference of history values (the load/unload verdict, d - plstDfm, the
+// new plstDfm), the normal n, and the tangential spring update (rotation
+// into the tangent plane, rescale, + v_t dt), so nothing that accumulates
+// over steps is rounded to FP32.  FP32: the force magnitudes and their
+// square roots and powers (Hertz, plastic radius, damping, xi_max, Mindlin
+// ratio) and the force/moment vectors of this pair.
+template <bool PP>
+DEV void contact_mixed(const DevLaw &c, double rolling, double dt, Hist1 &h, double d, V3 n,
+                       double R, double ri, V3 cpv, V3 &F, V3 &M, double &Mr) {
+    const float tol = PP ? 1.0e-7f : 1.0e-10f;
+    int stage;
+    if (d < h.pdfm) stage = 1;
+    else if (d >= h.pdfm && d < h.nd) stage = 3;
+    else if (d >= h.nd && d < h.ndm) stage = 4;
+    else stage = 2;
+    const float c43E = (float)c.c43E;
+    const float sRd = sqrtf((float)(R * d));
+    float fN = 0.0f;
+    if (stage == 2) {
+        bool elastic = PP ? (d <= c.ydsp_t * R) : (d < (c.ydsp_t * R));
+        if (elastic) {
+            fN = c43E * sRd * (float)d;
+            h.prad = R;
+            h.pdfm = 0.0;
+        } else {
+            double fp = (c.yforce_t * (R * R)) + (c.ystiff_t * R) * (d - (c.ydsp_t * R));
+            fN = (float)fp;
+            float fH = c43E * sRd * (float)d;
+            h.prad = R * (double)(fH / fN);
+            float q = fN / (c43E * sqrtf((float)R));
+            h.pdfm = d - (double)cbrtf(q * q);
+        }
+        h.ndm = d;
+    } else if (stage != 1) {
+        float x = (float)(d - h.pdfm);
+        fN = c43E * sqrtf((float)h.prad) * (x * sqrtf(x));
+    }
+    h.nd = d;
+    h.flags = (h.flags & ~0xff) | stage;
+
+    const double cpvn = dot(cpv, n);
+    const float fD = c.damp_on ? (float)c.damp * sRd * (float)cpvn : 0.0f;
+    const V3 vt = sub(cpv, smulv(cpvn, n));
+
+    // spring: project onto the tangent plane, keep its length, add v_t dt
+    V3 old = v3(h.dx, h.dy, h.dz);
+    V3 t = sub(old, smulv(dot(n, old), n));
+    const double lt2 = dot(t, t);
+    if (lt2 > (double)tol * (double)tol)
+        t = add(mulv(t, sqrt(dot(old, old) / lt2)), mulv(vt, dt));
+    else
+        t = add(t, mulv(vt, dt));
+    const float dsmx = (float)c.dsmxA * fN / ((float)c.dsmxB * sRd);
+    const float lt = sqrtf((float)dot(t, t));
+    float ratio = 1.0f;
+    if (lt < dsmx) {
+        if (dsmx > tol) {
+            float u = 1.0f - lt / dsmx;
+            ratio = 1.0f - u * sqrtf(u);
+        }
+    } else if (lt > tol) {
+        t = mulv(t, (double)(dsmx / lt));
+    } else {
+        t = mulv(t, (double)dsmx);
+    }
+    h.dx = t.x; h.dy = t.y; h.dz = t.z;
+    float fx = 0.f, fy = 0.f, fz = 0.f;
+    const float sf = -(float)c.sfrc * fN * ratio;
+    const float l3 = sqrtf((float)dot(t, t));
+    if (l3 < dsmx) {
+        if (l3 > 1.0e-10f) {
+            float k = sf / l3;
+            fx = k * (float)t.x; fy = k * (float)t.y; fz = k * (float)t.z;
+        }
+    } else {
+        const float lv = sqrtf((float)dot(vt, vt));
+        if (lv > 1.0e-10f) {
+            float k = sf / lv;
+            fx = k * (float)vt.x; fy = k * (float)vt.y; fz = k * (float)vt.z;
+        }
+    }
+    const float fn = fN + fD;
+    const float nx = (float)n.x, ny = (float)n.y, nz = (float)n.z;
+    F.x += (double)(fn * nx + fx);
+    F.y += (double)(fn * ny + fy);
+    F.z += (double)(fn * nz + fz);
+    const float arm = -(float)(ri - d / 2.0);
+    const float ax = arm * nx, ay = arm * ny, az = arm * nz;
+    M.x += (double)(ay * fz - az * fy);
+    M.y += (double)(az * fx - ax * fz);
+    M.z += (double)(ax * fy - ay * fx);
+    Mr += (double)((float)rolling * fN * (float)ri);
+}
+
 // Willett bridge force (kernel.cu:65-81) at the instance's contact angle
 DEV double fcap(const DevInst &P, double S, double V, double effR) {
     S = fmax(P.gapmn, S);
     S = S / sqrt(V);
-    double L = log(V), L2 = L * L, L3 = L2 * L;
+    double L = dm_log(V), L2 = L * L, L3 = L2 * L;
     const double *a = P.wc;
     double f1 = a[0] + a[1] * L + a[2] * L2 + a[3] * L3;
     double f2 = a[4] + a[5] * L + a[6] * L2;
     double f3 = a[7] + a[8] * L + a[9] * L2 + a[10] * L3;
     double f4 = a[11] + a[12] * L + a[13] * L2;
-    double lS = log(S);
-    return P.twoPI * effR * P.stension * exp(f1 - f2 * exp(f3 * lS + f4 * (lS * lS)));
+    double lS = dm_log(S);
+    return P.twoPI * effR * P.stension * dm_exp(f1 - f2 * dm_exp(f3 * lS + f4 * (lS * lS)));
 }
 
 DEV double rupture(const DevInst &P, double V) {
-    return P.ruptureMode == 1 ? (1.0 + 0.5 * P.theta) * cbrt(V) : 0.99 * pow(V, 0.33);
+    return P.ruptureMode == 1 ? (1.0 + 0.5 * P.theta) * dm_pow(V, 1.0 / 3.0)
+                              : 0.99 * dm_pow(V, 0.33);
 }
 
 // gravity + all p-p slots of particle i (kernel.cu:424-551)
-template <bool WET>
+template <bool WET, bool MIXED>
 DEV void pp_particle(const DevInst &P, const StepArgs &A, int i, V3 &F, V3 &M, double &Mr,
                      int &nbridge) {
     const StateBuf &s = A.in;
@@ -664,8 +824,19 @@ DEV void pp_particle(const DevInst &P, const StepArgs &A, int i, V3 &F, V3 &M, d
         const V3 jp = v3(s.x[j], s.y[j], s.z[j]);
         const double rj = A.rad[j];
         const V3 dv = sub(ip, jp);
+        if (MIXED && !WET) {
+            // dry: pairs apart need no square root
+            const double rs = ri + rj;
+            if (!(dot(dv, dv) < rs * rs)) continue;
+        }
         const double l = len(dv);
-        const V3 nv = divv(dv, l);
+        V3 nv;
+        if (MIXED) {
+            const double il = 1.0 / l;
+            nv = mulv(dv, il);
+        } else {
+            nv = divv(dv, l);
+        }
         const double gap = l - ri - rj;
         if (gap < 0.0) {
             const V3 jv = v3(s.vx[j], s.vy[j], s.vz[j]);
@@ -673,7 +844,10 @@ DEV void pp_particle(const DevInst &P, const StepArgs &A, int i, V3 &F, V3 &M, d
             Hist1 h = hist_load(A.h, sk);
             double R = ri * rj / (ri + rj);
             V3 cpv = sub(sub(iv, jv), cross(add(mulv(iw, ri), mulv(jw, rj)), nv));
-            contact<true>(P.pp, P.rolling, P.dt, h, -gap, nv, R, ri, cpv, F, M, Mr);
+            if (MIXED)
+                contact_mixed<true>(P.pp, P.rolling, P.dt, h, -gap, nv, R, ri, cpv, F, M, Mr);
+            else
+                contact<true>(P.pp, P.rolling, P.dt, h, -gap, nv, R, ri, cpv, F, M, Mr);
             hist_store(A.h, sk, h);
         }
         if (WET && P.forceCap == 1) {
@@ -702,6 +876,7 @@ DEV void pp_particle(const DevInst &P, const StepArgs &A, int i, V3 &F, V3 &M, d
 }
 
 // walls of particle i (kernel.cu:958-1023, 370-403)
+template <bool MIXED>
 DEV void pw_particle(const DevInst &P, const StepArgs &A, int i, V3 &F, V3 &M, double &Mr) {
     const StateBuf &s = A.in;
     const int n = A.nwl[i];
@@ -734,7 +909,10 @@ DEV void pw_particle(const DevInst &P, const StepArgs &A, int i, V3 &F, V3 &M, d
         if (gap < 0.0) {
             Hist1 h = hist_load(A.hw, sk);
             V3 cpv = sub(sub(iv, jv), cross(mulv(iw, ri), nv));
-            contact<false>(P.pw, P.rolling, P.dt, h, -gap, nv, ri, ri, cpv, F, M, Mr);
+            if (MIXED)
+                contact_mixed<false>(P.pw, P.rolling, P.dt, h, -gap, nv, ri, ri, cpv, F, M, Mr);
+            else
+                contact<false>(P.pw, P.rolling, P.dt, h, -gap, nv, ri, ri, cpv, F, M, Mr);
             hist_store(A.hw, sk, h);
         }
     }
@@ -781,7 +959,7 @@ DEV void request_rebuild(const StepArgs &A, bool mine) {
         atomicMin(A.stop_at, A.step + 1);
 }
 
-template <bool WET>
+template <bool WET, bool MIXED>
 __global__ void __launch_bounds__(256) k_step_fused(DevInst P, StepArgs A) {
     // a rebuild is due before this step
     if (__hip_atomic_load(A.stop_at, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <= A.step) return;
@@ -792,9 +970,9 @@ __global__ void __launch_bounds__(256) k_step_fused(DevInst P, StepArgs A) {
         V3 F = v3(0.0, 0.0, 0.0), M = v3(0.0, 0.0, 0.0);
         double Mr = 0.0;
         int nbridge = 0;
-        pp_particle<WET>(P, A, i, F, M, Mr, nbridge);
+        pp_particle<WET, MIXED>(P, A, i, F, M, Mr, nbridge);
         if (WET) A.out.lqd[i] = nbridge;
-        pw_particle(P, A, i, F, M, Mr);
+        pw_particle<MIXED>(P, A, i, F, M, Mr);
         if (A.store_forces) {
             A.Fx[i] = F.x; A.Fy[i] = F.y; A.Fz[i] = F.z;
             A.Mx[i] = M.x; A.My[i] = M.y; A.Mz[i] = M.z;
@@ -804,37 +982,42 @@ __global__ void __launch_bounds__(256) k_step_fused(DevInst P, StepArgs A) {
     request_rebuild(A, over);
 }
 
-void launch_step_fused(const DevInst &P, const StepArgs &A, bool wet, hipStream_t s) {
+void launch_step_fused(const DevInst &P, const StepArgs &A, bool wet, bool mixed, hipStream_t s) {
     if (A.total <= 0) return;
     unsigned nb = blocks_for(A.total, 256);
-    if (wet)
-        hipLaunchKernelGGL(k_step_fused<true>, dim3(nb), dim3(256), 0, s, P, A);
+    if (wet && mixed)
+        hipLaunchKernelGGL((k_step_fused<true, true>), dim3(nb), dim3(256), 0, s, P, A);
+    else if (wet)
+        hipLaunchKernelGGL((k_step_fused<true, false>), dim3(nb), dim3(256), 0, s, P, A);
+    else if (mixed)
+        hipLaunchKernelGGL((k_step_fused<false, true>), dim3(nb), dim3(256), 0, s, P, A);
     else
-        hipLaunchKernelGGL(k_step_fused<false>, dim3(nb), dim3(256), 0, s, P, A);
+        hipLaunchKernelGGL((k_step_fused<false, false>), dim3(nb), dim3(256), 0, s, P, A);
 }
 
 // --- the reference's separate kernels (PpInterCalc, CirPWCalc,
 // UpdateSystem), accumulating into the force arrays -----------------------
-template <bool WET>
+template <bool WET, bool MIXED>
 __global__ void __launch_bounds__(256) k_pp_force(DevInst P, StepArgs A) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= A.total) return;
     V3 F = v3(A.Fx[i], A.Fy[i], A.Fz[i]), M = v3(A.Mx[i], A.My[i], A.Mz[i]);
     double Mr = A.Mr[i];
     int nbridge = 0;
-    pp_particle<WET>(P, A, i, F, M, Mr, nbridge);
+    pp_particle<WET, MIXED>(P, A, i, F, M, Mr, nbridge);
     if (WET) A.out.lqd[i] = A.out.lqd[i] + nbridge;
     A.Fx[i] = F.x; A.Fy[i] = F.y; A.Fz[i] = F.z;
     A.Mx[i] = M.x; A.My[i] = M.y; A.Mz[i] = M.z;
     A.Mr[i] = Mr;
 }
 
+template <bool MIXED>
 __global__ void __launch_bounds__(256) k_pw_force(DevInst P, StepArgs A) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= A.total) return;
     V3 F = v3(A.Fx[i], A.Fy[i], A.Fz[i]), M = v3(A.Mx[i], A.My[i], A.Mz[i]);
     double Mr = A.Mr[i];
-    pw_particle(P, A, i, F, M, Mr);
+    pw_particle<MIXED>(P, A, i, F, M, Mr);
     A.Fx[i] = F.x; A.Fy[i] = F.y; A.Fz[i] = F.z;
     A.Mx[i] = M.x; A.My[i] = M.y; A.Mz[i] = M.z;
     A.Mr[i] = Mr;
@@ -852,17 +1035,19 @@ __global__ void __launch_bounds__(256) k_update(DevInst P, StepArgs A, int *coun
         atomicAdd(count, (int)__popcll(b));
 }
 
-void launch_pp_force(const DevInst &P, const StepArgs &A, bool wet, hipStream_t s) {
+void launch_pp_force(const DevInst &P, const StepArgs &A, bool wet, bool mixed, hipStream_t s) {
     if (A.total <= 0) return;
-    if (wet)
-        hipLaunchKernelGGL(k_pp_force<true>, dim3(blocks_for(A.total, 256)), dim3(256), 0, s, P, A);
-    else
-        hipLaunchKernelGGL(k_pp_force<false>, dim3(blocks_for(A.total, 256)), dim3(256), 0, s, P,
-                           A);
+    dim3 g(blocks_for(A.total, 256)), b(256);
+    if (wet && mixed) hipLaunchKernelGGL((k_pp_force<true, true>), g, b, 0, s, P, A);
+    else if (wet) hipLaunchKernelGGL((k_pp_force<true, false>), g, b, 0, s, P, A);
+    else if (mixed) hipLaunchKernelGGL((k_pp_force<false, true>), g, b, 0, s, P, A);
+    else hipLaunchKernelGGL((k_pp_force<false, false>), g, b, 0, s, P, A);
 }
-void launch_pw_force(const DevInst &P, const StepArgs &A, hipStream_t s) {
+void launch_pw_force(const DevInst &P, const StepArgs &A, bool mixed, hipStream_t s) {
     if (A.total <= 0) return;
-    hipLaunchKernelGGL(k_pw_force, dim3(blocks_for(A.total, 256)), dim3(256), 0, s, P, A);
+    dim3 g(blocks_for(A.total, 256)), b(256);
+    if (mixed) hipLaunchKernelGGL((k_pw_force<true>), g, b, 0, s, P, A);
+    else hipLaunchKernelGGL((k_pw_force<false>), g, b, 0, s, P, A);
 }
 void launch_update(const DevInst &P, const StepArgs &A, int *count, hipStream_t s) {
     if (A.total <= 0) return;
@@ -915,6 +1100,24 @@ void launch_scatter3(const double *in, double *a, double *b, double *c, int n, i
     if (n > 0)
         hipLaunchKernelGGL(k_scatter3, dim3(blocks_for(n, 256)), dim3(256), 0, s, in, a, b, c, n,
                            off);
+}
+
+// ---------------------------------------------------------------------
+// plain device copy: the attainable HBM bandwidth, and the known-size
+// kernel that calibrates the FETCH_SIZE / WRITE_SIZE counters
+// ---------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_stream_copy(const double2 *__restrict__ a,
+                                                     double2 *__restrict__ b, long n) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long stride = (long)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) b[i] = a[i];
+}
+
+void launch_stream_copy(const void *a, void *b, long nbytes, hipStream_t s) {
+    long n = nbytes / 16;
+    if (n > 0)
+        hipLaunchKernelGGL(k_stream_copy, dim3(256 * 8 * 4), dim3(256), 0, s, (const double2 *)a,
+                           (double2 *)b, n);
 }
 
 // ---------------------------------------------------------------------

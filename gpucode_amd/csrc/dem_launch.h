@@ -31,9 +31,9 @@ void launch_hist_remap(const unsigned *perm, const int *id_new, const int *id_ol
                        const int *nwl_old, HistPtr hw, HistPtr hw_old, int cap, int total,
                        int old_total, hipStream_t s);
 
-void launch_step_fused(const DevInst &P, const StepArgs &A, bool wet, hipStream_t s);
-void launch_pp_force(const DevInst &P, const StepArgs &A, bool wet, hipStream_t s);
-void launch_pw_force(const DevInst &P, const StepArgs &A, hipStream_t s);
+void launch_step_fused(const DevInst &P, const StepArgs &A, bool wet, bool mixed, hipStream_t s);
+void launch_pp_force(const DevInst &P, const StepArgs &A, bool wet, bool mixed, hipStream_t s);
+void launch_pw_force(const DevInst &P, const StepArgs &A, bool mixed, hipStream_t s);
 void launch_update(const DevInst &P, const StepArgs &A, int *count, hipStream_t s);
 
 void launch_gather3(const int *id, const double *a, const double *b, const double *c, double *out,
@@ -43,3 +43,4 @@ void launch_gather1i(const int *id, const int *a, int *out, int total, hipStream
 void launch_scatter3(const double *in, double *a, double *b, double *c, int n, int off,
                      hipStream_t s);
 void launch_stats(const DevInst &P, const StepArgs &A, unsigned long long *out, hipStream_t s);
+void launch_stream_copy(const void *a, void *b, long nbytes, hipStream_t s);

@@ -47,6 +47,13 @@ def write_tec(path, outs, time_s, pos, rad, vel, angv, force):
                                 *[dptr(a) for a in arrs]))
 
 
+def stream_copy(nbytes=1 << 30, iters=20):
+    """Attainable HBM bandwidth [GB/s] of a plain device copy (read + write)."""
+    g = C.c_double()
+    check(A.lib().dem_stream_copy(int(nbytes), int(iters), C.byref(g)))
+    return g.value
+
+
 def load_para(path):
     v = [C.c_int(), C.c_int(), C.c_int()]
     check(A.lib().dem_load_para(str(path).encode(), *[C.byref(x) for x in v]))

@@ -159,6 +159,26 @@ def packed_bed(n=2_000_000, d=0.3e-3, R=30e-3, H=30e-3, mu=0.5, skin=0.1, seed=0
     return dict(name="packed_bed", inp=inp, dia=None, pos=pos, steps=steps)
 
 
+def wet_bed(n=3000, R_d=8.0, poly=False, theta_deg=0.0, reference_law=True, nLevels=None,
+            seed=17, liquid=0.01):
+    """A dense wet bed for capillary parity tests (d_max = 1 mm).  poly draws
+    diameters uniform in [0.7, 1.0] mm on the monodisperse pocket sites, so
+    no pair overlaps and the size levels and cross-level search are used."""
+    d = 1.0e-3
+    rng = np.random.default_rng(seed)
+    dia = None
+    if poly:
+        dia = d * rng.uniform(0.7, 1.0, n)
+        dia[0] = d
+    H = d * (n / (math.pi * R_d * R_d * 0.5) + 6)
+    th = 0.0 if reference_law else math.radians(theta_deg)
+    inp = make_input(n, d, R_d * d, H, 0.3, cutGap=0.1, forceCap=1, liquid=liquid, theta=th,
+                     rupture_mode=0 if reference_law else 1,
+                     nLevels=nLevels or (3 if poly else 1))
+    pos = pocket_bed(n, R_d, 1.0, seed=seed)
+    return dict(name="wet_bed", inp=inp, dia=dia, pos=pos, steps=200)
+
+
 def small_bed(n=3000, R_d=8.0, mu=0.5, seed=7, dt=None, jitter=0.03):
     """A small dense bed for contact-heavy parity tests (d = 1 mm)."""
     d = 1.0e-3

@@ -230,6 +230,11 @@ int dem_get_timing(dem_ctx *ctx, double *fused_ms, long long *fused_steps,
                    long long *fused_launches, double *rebuild_ms, int *n_rebuild);
 /* Bytes of device memory held by the context. */
 long long dem_device_bytes(dem_ctx *ctx);
+/* Measurement utility with no reference counterpart: a plain device copy
+ * of nbytes, timed over iters launches with HIP events; *gbps = read +
+ * write bytes per second / 1e9 (the attainable HBM bandwidth, and the
+ * known-size kernel that calibrates rocprofv3's FETCH_SIZE/WRITE_SIZE). */
+int dem_stream_copy(long long nbytes, int iters, double *gbps);
 
 /* ---- host interfaces (reference names kept) -------------------------- */
 /* heyi.in keyword deck (ReadData.cpp:7-188, main.cpp:65-68); optional

@@ -65,6 +65,81 @@ static inline v3 ld3(const double *p) { return mk(p[0], p[1], p[2]); }
 static inline void st3(double *p, v3 a) { p[0] = a.x; p[1] = a.y; p[2] = a.z; }
 
 /* ------------------------------------------------------------------ */
+/* Rounding-reproducible transcendentals.                              */
+/* The reference calls libm pow/exp/log; libm results are within an ulp */
+/* of the true value but differ between libraries in the last bit, and */
+/* the contact law is discontinuous at the stick/slip and yield        */
+/* boundaries, so an ulp can change a branch.  The oracle therefore    */
+/* evaluates pow(x, 0.5) as sqrt(x), pow(x, 1.5) as x*sqrt(x), integer */
+/* powers as products, and exp/log/other powers with the functions     */
+/* below, built only from correctly rounded +, -, *, /, sqrt and exact */
+/* scaling, so any IEEE machine reproduces them bit for bit.           */
+/*   exp: Cody-Waite reduction x = k ln2 + r, |r| <= ln2/2, Taylor      */
+/*        series to r^14 (truncation < 1e-17), scale by 2^k.            */
+/*   log: x = m 2^e, m in [sqrt(1/2), sqrt(2)), s = (m-1)/(m+1),        */
+/*        log m = 2 atanh s = 2 s sum z^n/(2n+1), z = s^2 <= 0.0295.    */
+/* Both are within a few ulp of libm (tests/test_oracle.py).           */
+/* ------------------------------------------------------------------ */
+static const double DM_LN2_HI = 0x1.62e42feep-1;          /* ln 2 to 32 bits: k*hi exact */
+static const double DM_LN2_LO = 0x1.a39ef35793c76p-33;    /* ln 2 - hi */
+static const double DM_INV_LN2 = 1.4426950408889634;
+
+double orc_exp(double x) {
+    if (x != x) return x;
+    if (x > 709.0) return HUGE_VAL;
+    if (x < -708.0) return 0.0;
+    double k = rint(x * DM_INV_LN2);
+    double r = (x - k * DM_LN2_HI) - k * DM_LN2_LO;
+    double p = 1.0 / 87178291200.0;           /* 1/14! */
+    p = 1.0 / 6227020800.0 + r * p;           /* 1/13! */
+    p = 1.0 / 479001600.0 + r * p;
+    p = 1.0 / 39916800.0 + r * p;
+    p = 1.0 / 3628800.0 + r * p;
+    p = 1.0 / 362880.0 + r * p;
+    p = 1.0 / 40320.0 + r * p;
+    p = 1.0 / 5040.0 + r * p;
+    p = 1.0 / 720.0 + r * p;
+    p = 1.0 / 120.0 + r * p;
+    p = 1.0 / 24.0 + r * p;
+    p = 1.0 / 6.0 + r * p;
+    p = 0.5 + r * p;
+    p = 1.0 + r * p;
+    p = 1.0 + r * p;
+    return ldexp(p, (int)k);
+}
+
+double orc_log(double x) {
+    if (!(x > 0.0)) return x == 0.0 ? -HUGE_VAL : (x - x) / 0.0;
+    int e;
+    double m = frexp(x, &e);
+    if (m < 0.70710678118654752) {
+        m = m * 2.0;
+        e = e - 1;
+    }
+    double f = m - 1.0;
+    double s = f / (2.0 + f);
+    double z = s * s;
+    double q = 1.0 / 25.0;
+    q = 1.0 / 23.0 + z * q;
+    q = 1.0 / 21.0 + z * q;
+    q = 1.0 / 19.0 + z * q;
+    q = 1.0 / 17.0 + z * q;
+    q = 1.0 / 15.0 + z * q;
+    q = 1.0 / 13.0 + z * q;
+    q = 1.0 / 11.0 + z * q;
+    q = 1.0 / 9.0 + z * q;
+    q = 1.0 / 7.0 + z * q;
+    q = 1.0 / 5.0 + z * q;
+    q = 1.0 / 3.0 + z * q;
+    double lm = 2.0 * s + 2.0 * s * (z * q);
+    double de = (double)e;
+    return de * DM_LN2_HI + (de * DM_LN2_LO + lm);
+}
+
+/* x^y for x > 0 */
+double orc_pow(double x, double y) { return orc_exp(y * orc_log(x)); }
+
+/* ------------------------------------------------------------------ */
 /* state                                                               */
 /* ------------------------------------------------------------------ */
 typedef struct hist_arr {
@@ -326,17 +401,18 @@ static double normal_law(const orc_calpara *c, orc_hist1 *h, double nrmDsp,
     } else if (h->stage == 2) {
         int elastic = yield_le ? (nrmDsp <= c->ydsp_t * ijRad) : (nrmDsp < (c->ydsp_t * ijRad));
         if (elastic) {
-            fN = 4.0 / 3.0 * c->emod * pow(ijRad * nrmDsp, 0.5) * nrmDsp;
+            fN = 4.0 / 3.0 * c->emod * sqrt(ijRad * nrmDsp) * nrmDsp;
             h->plstRad = ijRad;
             h->plstDfm = 0.0;
         } else {
-            fN = (c->yforce_t * pow(ijRad, 2.0)) + (c->ystiff_t * ijRad) * (nrmDsp - (c->ydsp_t * ijRad));
-            h->plstRad = ijRad * (4.0 / 3.0 * c->emod * pow(ijRad * nrmDsp, 0.5) * nrmDsp) / fN;
-            h->plstDfm = nrmDsp - pow((fN / (4.0 / 3.0 * c->emod * pow(ijRad, 0.5))), 2.0 / 3.0);
+            fN = (c->yforce_t * (ijRad * ijRad)) + (c->ystiff_t * ijRad) * (nrmDsp - (c->ydsp_t * ijRad));
+            h->plstRad = ijRad * (4.0 / 3.0 * c->emod * sqrt(ijRad * nrmDsp) * nrmDsp) / fN;
+            h->plstDfm = nrmDsp - orc_pow((fN / (4.0 / 3.0 * c->emod * sqrt(ijRad))), 2.0 / 3.0);
         }
         h->nDsp_mx = nrmDsp;
     } else {
-        fN = 4.0 / 3.0 * c->emod * pow(h->plstRad, 0.5) * pow((nrmDsp - h->plstDfm), 1.5);
+        double u = nrmDsp - h->plstDfm;
+        fN = 4.0 / 3.0 * c->emod * sqrt(h->plstRad) * (u * sqrt(u));
     }
     h->nDsp = nrmDsp;
     return fN;
@@ -357,10 +433,13 @@ static v3 tangential_law(const orc_calpara *c, orc_hist1 *h, double fN, double n
         d = vadd(d, vmul(vt, dt));
 
     double ijDsmx = 3.0 * c->sfrc * fN /
-                    (16.0 * (1 - c->pois) * c->emod / (4.0 - 2.0 * c->pois) * pow(ijRad * nrmDsp, 0.5));
+                    (16.0 * (1 - c->pois) * c->emod / (4.0 - 2.0 * c->pois) * sqrt(ijRad * nrmDsp));
     double ratio = 1.0;
     if (vlen(d) < ijDsmx) {
-        if (ijDsmx > tol_ratio) ratio = 1.0 - pow((1.0 - vlen(d) / ijDsmx), 1.5);
+        if (ijDsmx > tol_ratio) {
+            double u = 1.0 - vlen(d) / ijDsmx;
+            ratio = 1.0 - u * sqrt(u);
+        }
     } else {
         ratio = 1.0;
         if (vlen(d) > tol_slip)
@@ -439,8 +518,8 @@ void orc_contact_pw(const orc_setup *S, orc_hist1 *h, double nrmDsp,
 double orc_fcap(const orc_setup *S, double Splus, double ijVol, double effectR) {
     const double th = S->theta;
     Splus = fmax(S->gapmn, Splus);
-    Splus = Splus / pow(ijVol, 0.5);
-    double LV = log(ijVol);
+    Splus = Splus / sqrt(ijVol);
+    double LV = orc_log(ijVol);
     double a10 = -0.44507 + th * (0.050832 + th * (-1.1466));
     double a11 = -0.1119 + th * (-0.000411 + th * (-0.1490));
     double a12 = -0.012101 + th * (-0.0036456 + th * (-0.01255));
@@ -455,19 +534,20 @@ double orc_fcap(const orc_setup *S, double Splus, double ijVol, double effectR) 
     double a40 = -0.010703 + th * (0.073776 + th * (-0.34742));
     double a41 = 0.03345 + th * (0.04543 + th * (-0.09056));
     double a42 = 0.0018574 + th * (0.004456 + th * (-0.006257));
-    double f1 = a10 + a11 * LV + a12 * pow(LV, 2.0) + a13 * pow(LV, 3.0);
-    double f2 = a20 + a21 * LV + a22 * pow(LV, 2.0);
-    double f3 = a30 + a31 * LV + a32 * pow(LV, 2.0) + a33 * pow(LV, 3.0);
-    double f4 = a40 + a41 * LV + a42 * pow(LV, 2.0);
-    double lS = log(Splus);
-    return 2 * ORC_PI * effectR * S->stension * exp(f1 - f2 * exp(f3 * lS + f4 * pow(lS, 2.0)));
+    double L2 = LV * LV, L3 = L2 * LV;
+    double f1 = a10 + a11 * LV + a12 * L2 + a13 * L3;
+    double f2 = a20 + a21 * LV + a22 * L2;
+    double f3 = a30 + a31 * LV + a32 * L2 + a33 * L3;
+    double f4 = a40 + a41 * LV + a42 * L2;
+    double lS = orc_log(Splus);
+    return 2 * ORC_PI * effectR * S->stension * orc_exp(f1 - f2 * orc_exp(f3 * lS + f4 * (lS * lS)));
 }
 
 /* bridge rupture distance: kernel.cu:522, or Lian et al. (1993)
  * (1 + theta/2) V^(1/3) when ruptureMode = 1 */
 double orc_rupture(const orc_setup *S, double ijVol) {
-    if (S->ruptureMode == 1) return (1.0 + 0.5 * S->theta) * cbrt(ijVol);
-    return 0.99 * pow(ijVol, 0.33);
+    if (S->ruptureMode == 1) return (1.0 + 0.5 * S->theta) * orc_pow(ijVol, 1.0 / 3.0);
+    return 0.99 * orc_pow(ijVol, 0.33);
 }
 
 /* ------------------------------------------------------------------ */
@@ -762,7 +842,8 @@ void orc_wall_list(orc_state *st) {
         int n = 0, *row = &st->wallList[i * ORC_WALLS];
         if (p[2] >= (S->Tw - (2.0 * r + S->searchGap))) row[n++] = 0;
         if (p[2] <= ((2.0 * r + S->searchGap) + S->Bw)) row[n++] = 1;
-        double dc = pow(pow((S->cirX - p[0]), 2.0) + pow((S->cirY - p[1]), 2.0), 0.5);
+        double ax = S->cirX - p[0], ay = S->cirY - p[1];
+        double dc = sqrt(ax * ax + ay * ay);
         if (dc >= S->R - (2.0 * r + S->searchGap)) row[n++] = 2;
         st->numWall[i] = n;
     }
@@ -830,11 +911,13 @@ void orc_pp_force(orc_state *st) {
         double ir = st->rad[i];
         int inl = st->lqdCntOld[i];
         if (inl < 12) inl = 12;
-        double iVol = S->vol * pow((st->rad[i] * 2), 3.0) / inl;
+        double dv = st->rad[i] * 2;
+        double iVol = S->vol * (dv * dv * dv) / inl;
         v3 F = ld3(&st->force[3 * (size_t)i]);
         v3 M = ld3(&st->mom[3 * (size_t)i]);
         double Mr = st->rmom[i];
-        F.z = F.z - pow(2.0 * st->rad[i], 3.0); /* gravity, kernel.cu:439 */
+        double di = 2.0 * st->rad[i];
+        F.z = F.z - di * di * di; /* gravity, kernel.cu:439 */
         for (int k = 0; k < st->numNebor[i]; ++k) {
             size_t ine = slot_of(st, i, k);
             int j = st->nebor[ine];
@@ -857,7 +940,8 @@ void orc_pp_force(orc_state *st) {
             if (S->forceCap == 1) {
                 int jnl = st->lqdCntOld[j];
                 if (jnl < 12) jnl = 12;
-                double jVol = S->vol * pow((jr * 2), 3.0) / jnl;
+                double dj = jr * 2;
+                double jVol = S->vol * (dj * dj * dj) / jnl;
                 double ijVol = iVol + jVol;
                 double rupture = orc_rupture(S, ijVol);
                 if (gap < 2 * S->layer * (ir + jr))
@@ -899,7 +983,8 @@ void orc_pw_force(orc_state *st) {
             } else if (w == 1) {
                 jp = mk(ip.x, ip.y, S->Bw);
             } else if (w == 2) {
-                double dc = pow(pow((S->cirX - ip.x), 2.0) + pow((S->cirY - ip.y), 2.0), 0.5);
+                double ax = S->cirX - ip.x, ay = S->cirY - ip.y;
+                double dc = sqrt(ax * ax + ay * ay);
                 double cs = (ip.x - S->cirX) / dc, sn = (ip.y - S->cirY) / dc;
                 jp.x = S->R * cs + S->cirX;
                 jp.y = S->R * sn + S->cirY;
@@ -935,13 +1020,14 @@ int orc_update(orc_state *st) {
         double *v = &st->vel[3 * (size_t)i], *x = &st->pos[3 * (size_t)i];
         double *w = &st->angv[3 * (size_t)i];
         const double *F = &st->force[3 * (size_t)i], *M = &st->mom[3 * (size_t)i];
-        double mass = pow(2.0 * st->rad[i], 3.0);
+        double di = 2.0 * st->rad[i], d2 = di * di;
+        double mass = di * di * di;
         v[0] = v[0] + F[0] * dt / mass;
         v[1] = v[1] + F[1] * dt / mass;
         v[2] = v[2] + F[2] * dt / mass;
         st3(x, vadd(ld3(x), vmul(ld3(v), dt)));
         st->pmove[i] = st->pmove[i] + vlen(ld3(v)) * dt;
-        double inert = 0.1 * pow(2.0 * st->rad[i], 5.0);
+        double inert = 0.1 * (d2 * d2 * di);
         w[0] = w[0] + M[0] * dt / inert;
         w[1] = w[1] + M[1] * dt / inert;
         w[2] = w[2] + M[2] * dt / inert;

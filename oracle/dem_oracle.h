@@ -127,6 +127,11 @@ void orc_contact_pw(const orc_setup *S, orc_hist1 *h, double nrmDsp,
                     const double iAngv[3], const double jVel[3], int stage,
                     double F[3], double M[3], double *Mr);
 double orc_fcap(const orc_setup *S, double Splus, double ijVol, double effectR);
+/* rounding-reproducible exp / log / x^y (x > 0) used for the reference's
+ * libm calls; see dem_oracle.c */
+double orc_exp(double x);
+double orc_log(double x);
+double orc_pow(double x, double y);
 double orc_rupture(const orc_setup *S, double ijVol);
 
 /* --- whole state -------------------------------------------------------- */

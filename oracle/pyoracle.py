@@ -75,6 +75,9 @@ _SIGS = {
                               C.c_double, D, D, D, C.c_int, D, D, D]),
     "orc_fcap": (C.c_double, [C.POINTER(OrcSetup), C.c_double, C.c_double, C.c_double]),
     "orc_rupture": (C.c_double, [C.POINTER(OrcSetup), C.c_double]),
+    "orc_exp": (C.c_double, [C.c_double]),
+    "orc_log": (C.c_double, [C.c_double]),
+    "orc_pow": (C.c_double, [C.c_double, C.c_double]),
     "orc_create": (P, [C.POINTER(OrcInput), D]),
     "orc_free": (None, [P]),
     "orc_get_setup": (C.POINTER(OrcSetup), [P]),

@@ -285,6 +285,35 @@ def test_capillary_theta_continuity():
 
 
 # --------------------------------------------------------------------------
+# rounding-reproducible exp / log / pow against libm
+# --------------------------------------------------------------------------
+def ulps(a, b):
+    a, b = np.asarray(a, np.float64), np.asarray(b, np.float64)
+    return np.abs(a - b) / np.spacing(np.abs(b))
+
+
+def test_reproducible_exp_log_pow_within_ulps():
+    L = O.lib()
+    rng = np.random.default_rng(0)
+    xs = np.concatenate([rng.uniform(-700, 700, 2000), rng.uniform(-1, 1, 2000),
+                         [0.0, 1e-300, -1e-300, 0.5, -0.5, 709.0, -708.0]])
+    e = np.array([L.orc_exp(float(x)) for x in xs])
+    assert ulps(e, np.exp(xs)).max() <= 4
+    ys = np.concatenate([10 ** rng.uniform(-300, 300, 2000), rng.uniform(0.5, 2, 2000),
+                         [1.0, 2.0, 0.5, 1e-300, 1e300, 0.70710678118654752]])
+    lg = np.array([L.orc_log(float(y)) for y in ys])
+    ok = np.abs(np.log(ys)) > 1e-300
+    assert (np.abs(lg - np.log(ys))[ok] <= 4 * np.spacing(np.abs(np.log(ys[ok])))
+            + 1e-300).all()
+    assert L.orc_log(1.0) == 0.0
+    # the powers the law uses: 2/3 (plastic), 0.33 and 1/3 (rupture)
+    for p in (2.0 / 3.0, 0.33, 1.0 / 3.0):
+        xs = 10 ** rng.uniform(-12, 2, 2000)
+        v = np.array([L.orc_pow(float(x), p) for x in xs])
+        assert (np.abs(v / xs ** p - 1) < 5e-15).all()
+
+
+# --------------------------------------------------------------------------
 # whole-state invariants
 # --------------------------------------------------------------------------
 def test_free_fall_discrete_closed_form():
