@@ -71,6 +71,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=300)
     ap.add_argument("--n", type=int, default=2_000_000)
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
+    ap.add_argument("--kernel", choices=["auto", "loop"], default="auto",
+                    help="fused step kernel: contact-compacting (auto) or one-thread loop")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -87,6 +89,7 @@ def main():
     log(f"bed of {args.n} spheres built in {time.time() - t0:.1f}s")
     prec = G.PREC_FP64 if args.precision == "fp64" else G.PREC_MIXED
     e = G.Engine(w["inp"], None, prec)
+    e.set_kernel(G.Engine.KERNEL_AUTO if args.kernel == "auto" else G.Engine.KERNEL_LOOP)
     e.set_particles(w["pos"])
     log(f"engine up: {e.device_bytes() / 2**30:.2f} GiB on device")
     t0 = time.time()

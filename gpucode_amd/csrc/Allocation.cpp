@@ -98,6 +98,52 @@ void dem_fill_inst(const dem_setup &S, DevInst &P) {
     }
 }
 
+// Sort-key layout: Morton codes of the cells when every level grid fits 10
+// bits per axis and the key space 31 bits, else the row-major hash.
+static bool morton_layout(const dem_setup &S, unsigned off[DEM_MAXL], int *key_bits) {
+    unsigned long long acc = 0;
+    for (int l = 0; l < S.nLevels; ++l) {
+        int m = S.gridSize[l][0];
+        if (S.gridSize[l][1] > m) m = S.gridSize[l][1];
+        if (S.gridSize[l][2] > m) m = S.gridSize[l][2];
+        int b = 0;
+        while ((1 << b) < m) ++b;
+        if (b > 10) return false;
+        off[l] = (unsigned)acc;
+        acc += 1ull << (3 * b);
+    }
+    if (acc > (1ull << 31)) return false;
+    int bits = 1;
+    while (bits < 31 && (1ull << bits) < acc) ++bits;
+    *key_bits = bits;
+    return true;
+}
+
+void dem_refresh_inst(dem_ctx *c) {
+    dem_fill_inst(c->S, c->P);
+    int bits = 1;
+    while (bits < 31 && ((long long)1 << bits) <= (long long)c->S.sysNCells) ++bits;
+    c->key_bits = bits;
+    c->P.morton = 0;
+    if (c->want_morton) {
+        unsigned off[DEM_MAXL] = {0};
+        int mb = 0;
+        if (morton_layout(c->S, off, &mb)) {
+            c->P.morton = 1;
+            for (int l = 0; l < DEM_MAXL; ++l) c->P.mortonOff[l] = off[l];
+            c->key_bits = mb;
+        }
+    }
+}
+
+extern "C" int dem_set_order(dem_ctx *c, int morton) {
+    if (!c) return dem_fail(DEM_ERR_ARG, "null context");
+    c->want_morton = morton != 0;
+    dem_refresh_inst(c);
+    c->rebuild_pending = true;
+    return DEM_OK;
+}
+
 template <class T>
 static int dalloc(dem_ctx *c, T **p, size_t n) {
     size_t b = (n ? n : 1) * sizeof(T);
@@ -107,35 +153,19 @@ static int dalloc(dem_ctx *c, T **p, size_t n) {
     return DEM_OK;
 }
 
-static int alloc_hist(dem_ctx *c, DevHist &h, size_t n) {
-    int rc;
-    if ((rc = dalloc(c, &h.dx, n)) || (rc = dalloc(c, &h.dy, n)) || (rc = dalloc(c, &h.dz, n)) ||
-        (rc = dalloc(c, &h.nd, n)) || (rc = dalloc(c, &h.ndm, n)) || (rc = dalloc(c, &h.prad, n)) ||
-        (rc = dalloc(c, &h.pdfm, n)) || (rc = dalloc(c, &h.flags, n)))
-        return rc;
-    return DEM_OK;
-}
-
-static void free_hist(DevHist &h) {
-    (void)hipFree(h.dx); (void)hipFree(h.dy); (void)hipFree(h.dz); (void)hipFree(h.nd);
-    (void)hipFree(h.ndm); (void)hipFree(h.prad); (void)hipFree(h.pdfm); (void)hipFree(h.flags);
-}
-
 static int alloc_all(dem_ctx *c) {
     const size_t cap = (size_t)c->cap, K = (size_t)c->K;
     int rc;
-    for (int b = 0; b < 2; ++b)
-        for (int q = 0; q < 9; ++q)
-            if ((rc = dalloc(c, &c->st[b][q], cap))) return rc;
     for (int b = 0; b < 2; ++b) {
-        if ((rc = dalloc(c, &c->lq[b], cap)) || (rc = dalloc(c, &c->rad[b], cap)) ||
-            (rc = dalloc(c, &c->level[b], cap)) || (rc = dalloc(c, &c->id[b], cap)) ||
-            (rc = dalloc(c, &c->nbr[b], cap * K)) || (rc = dalloc(c, &c->nnb[b], cap)) ||
-            (rc = dalloc(c, &c->wl[b], cap * DEM_WALLS)) || (rc = dalloc(c, &c->nwl[b], cap)) ||
-            (rc = alloc_hist(c, c->hp[b], cap * K)) || (rc = alloc_hist(c, c->hw[b], cap * DEM_WALLS)))
+        if ((rc = dalloc(c, &c->P4[b], cap)) || (rc = dalloc(c, &c->VW[b], 2 * cap)) ||
+            (rc = dalloc(c, &c->lq[b], cap)) || (rc = dalloc(c, &c->level[b], cap)) ||
+            (rc = dalloc(c, &c->id[b], cap)) || (rc = dalloc(c, &c->nbr[b], cap * K)) ||
+            (rc = dalloc(c, &c->nnb[b], cap)) || (rc = dalloc(c, &c->wl[b], cap * DEM_WALLS)) ||
+            (rc = dalloc(c, &c->nwl[b], cap)) || (rc = dalloc(c, &c->hp[b], cap * K)) ||
+            (rc = dalloc(c, &c->hw[b], cap * DEM_WALLS)))
             return rc;
     }
-    if ((rc = dalloc(c, &c->pmove, cap)) || (rc = dalloc(c, &c->cstart, (size_t)c->S.sysNCells)) ||
+    if ((rc = dalloc(c, &c->cstart, (size_t)c->S.sysNCells)) ||
         (rc = dalloc(c, &c->cend, (size_t)c->S.sysNCells)) || (rc = dalloc(c, &c->keys, cap)) ||
         (rc = dalloc(c, &c->vals, cap)) || (rc = dalloc(c, &c->keys2, cap)) ||
         (rc = dalloc(c, &c->vals2, cap)) ||
@@ -151,14 +181,13 @@ static int alloc_all(dem_ctx *c) {
 
 static void free_all(dem_ctx *c) {
     for (int b = 0; b < 2; ++b) {
-        for (int q = 0; q < 9; ++q) (void)hipFree(c->st[b][q]);
-        (void)hipFree(c->lq[b]); (void)hipFree(c->rad[b]); (void)hipFree(c->level[b]);
+        (void)hipFree(c->P4[b]); (void)hipFree(c->VW[b]);
+        (void)hipFree(c->lq[b]); (void)hipFree(c->level[b]);
         (void)hipFree(c->id[b]); (void)hipFree(c->nbr[b]); (void)hipFree(c->nnb[b]);
         (void)hipFree(c->wl[b]); (void)hipFree(c->nwl[b]);
-        free_hist(c->hp[b]);
-        free_hist(c->hw[b]);
+        (void)hipFree(c->hp[b]); (void)hipFree(c->hw[b]);
     }
-    (void)hipFree(c->pmove); (void)hipFree(c->cstart); (void)hipFree(c->cend);
+    (void)hipFree(c->cstart); (void)hipFree(c->cend);
     (void)hipFree(c->keys); (void)hipFree(c->vals); (void)hipFree(c->keys2); (void)hipFree(c->vals2);
     (void)hipFree(c->sort_tmp);
     for (int q = 0; q < 7; ++q) (void)hipFree(c->F[q]);
@@ -184,10 +213,7 @@ extern "C" int dem_create(const dem_input *in, const double *dia_m, int precisio
         return rc;
     }
     for (int l = 0; l < c->S.nLevels; ++l) c->K = c->S.levelNNebors[l] > c->K ? c->S.levelNNebors[l] : c->K;
-    int bits = 1;
-    while (bits < 31 && ((long long)1 << bits) <= (long long)c->S.sysNCells) ++bits;
-    c->key_bits = bits;
-    dem_fill_inst(c->S, c->P);
+    dem_refresh_inst(c);
     c->Tw = c->S.Tw;
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
@@ -248,7 +274,7 @@ extern "C" int dem_set_material(dem_ctx *c, const double mat[2][7]) {
         c->S.emod[m] = c->S.ymod[m] / (1 - c->S.pois[m] * c->S.pois[m]);
     }
     dem_calpara_from_material(&c->S);
-    dem_fill_inst(c->S, c->P);
+    dem_refresh_inst(c);
     return DEM_OK;
 }
 
@@ -262,7 +288,7 @@ extern "C" int dem_set_liquid(dem_ctx *c, double stension, double vol, double ga
     c->S.theta = theta;
     c->S.ruptureMode = ruptureMode;
     c->S.forceCap = forceCap;
-    dem_fill_inst(c->S, c->P);
+    dem_refresh_inst(c);
     return DEM_OK;
 }
 
@@ -272,6 +298,6 @@ extern "C" int dem_set_parameters(dem_ctx *c, double dt, double searchGap) {
         return dem_fail(DEM_ERR_ARG, "SearchGap beyond the cell size the grid was built for");
     c->S.dt = dt;
     c->S.searchGap = searchGap;
-    dem_fill_inst(c->S, c->P);
+    dem_refresh_inst(c);
     return DEM_OK;
 }

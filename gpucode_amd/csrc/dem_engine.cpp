@@ -27,14 +27,12 @@ static StepArgs step_args(dem_ctx *c, bool fused) {
         A.in = c->buf(c->cur, 1 - c->Lidx);
         A.out = c->buf(c->cur, c->Lidx);
     }
-    A.rad = c->rad[c->pcur];
-    A.pmove = c->pmove;
     A.nbr = c->nbr[c->hcur];
     A.nnb = c->nnb[c->hcur];
-    A.h = c->hp[c->hcur].ptr();
+    A.h = c->hp[c->hcur];
     A.wl = c->wl[c->hcur];
     A.nwl = c->nwl[c->hcur];
-    A.hw = c->hw[c->hcur].ptr();
+    A.hw = c->hw[c->hcur];
     A.Fx = c->F[0]; A.Fy = c->F[1]; A.Fz = c->F[2];
     A.Mx = c->F[3]; A.My = c->F[4]; A.Mz = c->F[5]; A.Mr = c->F[6];
     A.stop_at = c->stop_at;
@@ -70,22 +68,11 @@ static void reset_lists(dem_ctx *c) {
 static int upload_particles(dem_ctx *c, int off, int n, const double *pos, const double *vel,
                             const double *angv) {
     if (n <= 0) return DEM_OK;
+    // staging: pos, vel, angv (n x 3 each) and radii (n)
     double *tmp = nullptr;
-    HIPCHK(hipMalloc((void **)&tmp, (size_t)n * 3 * sizeof(double)));
-    const double *src[3] = {pos, vel, angv};
-    int rc = DEM_OK;
-    for (int q = 0; q < 3 && rc == DEM_OK; ++q) {
-        double **dst = &c->st[c->cur][3 * q];
-        if (src[q]) {
-            hipError_t e = hipMemcpyAsync(tmp, src[q], (size_t)n * 3 * sizeof(double),
-                                          hipMemcpyHostToDevice, c->stream);
-            if (e != hipSuccess) rc = dem_fail(DEM_ERR_HIP, "upload: %s", hipGetErrorString(e));
-            launch_scatter3(tmp, dst[0], dst[1], dst[2], n, off, c->stream);
-        } else {
-            for (int d = 0; d < 3; ++d) launch_fill_f64(dst[d] + off, 0.0, n, c->stream);
-        }
-    }
-    // ids off..off+n-1 in place, with their radii and levels
+    HIPCHK(hipMalloc((void **)&tmp, (size_t)n * 10 * sizeof(double)));
+    double *dpos = tmp, *dvel = tmp + 3 * (size_t)n, *dang = tmp + 6 * (size_t)n,
+           *drad = tmp + 9 * (size_t)n;
     std::vector<int> ids(n), lv(n);
     std::vector<double> r(n);
     for (int k = 0; k < n; ++k) {
@@ -93,24 +80,32 @@ static int upload_particles(dem_ctx *c, int off, int n, const double *pos, const
         lv[k] = c->level_h[off + k];
         r[k] = c->rad_h[off + k];
     }
-    if (rc == DEM_OK) {
-        hipError_t e = hipMemcpyAsync(c->id[c->pcur] + off, ids.data(), n * sizeof(int),
-                                      hipMemcpyHostToDevice, c->stream);
-        if (e == hipSuccess)
-            e = hipMemcpyAsync(c->level[c->pcur] + off, lv.data(), n * sizeof(int),
-                               hipMemcpyHostToDevice, c->stream);
-        if (e == hipSuccess)
-            e = hipMemcpyAsync(c->rad[c->pcur] + off, r.data(), n * sizeof(double),
-                               hipMemcpyHostToDevice, c->stream);
-        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-        if (e != hipSuccess) rc = dem_fail(DEM_ERR_HIP, "upload: %s", hipGetErrorString(e));
-    }
+    int rc = DEM_OK;
+    hipError_t e = hipMemcpyAsync(dpos, pos, (size_t)n * 3 * sizeof(double), hipMemcpyHostToDevice,
+                                  c->stream);
+    if (e == hipSuccess && vel)
+        e = hipMemcpyAsync(dvel, vel, (size_t)n * 3 * sizeof(double), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess && angv)
+        e = hipMemcpyAsync(dang, angv, (size_t)n * 3 * sizeof(double), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(drad, r.data(), n * sizeof(double), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(c->id[c->pcur] + off, ids.data(), n * sizeof(int), hipMemcpyHostToDevice,
+                           c->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(c->level[c->pcur] + off, lv.data(), n * sizeof(int),
+                           hipMemcpyHostToDevice, c->stream);
+    if (e != hipSuccess) rc = dem_fail(DEM_ERR_HIP, "upload: %s", hipGetErrorString(e));
+    // both buffers get the records, so the radius is valid whichever is read
+    launch_scatter_state(dpos, vel ? dvel : nullptr, angv ? dang : nullptr, drad,
+                         c->buf(c->cur, c->Lidx), n, off, c->stream);
+    launch_scatter_state(dpos, vel ? dvel : nullptr, angv ? dang : nullptr, drad,
+                         c->buf(1 - c->cur, c->Lidx), n, off, c->stream);
     launch_fill_i32(c->lq[0] + off, 0, n, c->stream);
     launch_fill_i32(c->lq[1] + off, 0, n, c->stream);
-    launch_fill_f64(c->pmove + off, 0.0, n, c->stream);
     launch_fill_i32(c->nnb[c->hcur] + off, 0, n, c->stream);
     launch_fill_i32(c->nwl[c->hcur] + off, 0, n, c->stream);
-    hipError_t e = hipStreamSynchronize(c->stream);
+    e = hipStreamSynchronize(c->stream);
     (void)hipFree(tmp);
     if (rc == DEM_OK && e != hipSuccess) rc = dem_fail(DEM_ERR_HIP, "upload: %s", hipGetErrorString(e));
     return rc;
@@ -178,11 +173,23 @@ static int get1i(dem_ctx *c, int *host, const int *src) {
     return rc;
 }
 
+// one field of the particle records, in original order: what = 0 position,
+// 1 velocity, 2 spin (3 values each), 4 path length (one value)
+static int get_state(dem_ctx *c, double *host, int what) {
+    if (!host) return DEM_OK;
+    double *tmp = nullptr;
+    size_t n = (size_t)c->total, w = what < 3 ? 3 : 1;
+    HIPCHK(hipMalloc((void **)&tmp, (n ? n : 1) * w * sizeof(double)));
+    launch_gather_state(c->id[c->pcur], c->buf(c->cur, c->Lidx), what, tmp, c->total, c->stream);
+    int rc = download(c, host, tmp, n * w);
+    (void)hipFree(tmp);
+    return rc;
+}
+
 extern "C" int dem_get_particles(dem_ctx *c, double *pos, double *vel, double *angv) {
     if (!c) return dem_fail(DEM_ERR_ARG, "null context");
     int rc;
-    if ((rc = get3(c, pos, &c->st[c->cur][0])) || (rc = get3(c, vel, &c->st[c->cur][3])) ||
-        (rc = get3(c, angv, &c->st[c->cur][6])))
+    if ((rc = get_state(c, pos, 0)) || (rc = get_state(c, vel, 1)) || (rc = get_state(c, angv, 2)))
         return rc;
     return DEM_OK;
 }
@@ -204,7 +211,7 @@ extern "C" int dem_set_store_forces(dem_ctx *c, int on) {
 
 extern "C" int dem_get_pmove(dem_ctx *c, double *pmove) {
     if (!c) return dem_fail(DEM_ERR_ARG, "null context");
-    return get1(c, pmove, c->pmove);
+    return get_state(c, pmove, 4);
 }
 
 extern "C" int dem_get_lqdcnt(dem_ctx *c, int *lqdCnt) {
@@ -239,36 +246,33 @@ extern "C" int dem_get_nebor(dem_ctx *c, int *num, int *list, int maxK) {
     return bad ? dem_fail(DEM_ERR_ARG, "maxK smaller than a list") : DEM_OK;
 }
 
-static int get_hist(dem_ctx *c, const DevHist &h, const int *dcnt, int slots, int maxK,
+static int get_hist(dem_ctx *c, const HistRec *h, const int *dcnt, int slots, int maxK,
                     double *dispt, double *nDsp, double *nDsp_mx, double *plstRad,
                     double *plstDfm, int *stage, int *lqdBrg) {
     const int n = c->total;
-    std::vector<int> ids, cnt(n), fl(n);
-    std::vector<double> a(n);
+    std::vector<int> ids, cnt(n);
+    std::vector<HistRec> rec(n);
     int rc;
     if ((rc = host_ids(c, ids)) || (rc = download(c, cnt.data(), dcnt, (size_t)n))) return rc;
-    double *dst[7] = {nullptr, nullptr, nullptr, nDsp, nDsp_mx, plstRad, plstDfm};
-    const double *src[7] = {h.dx, h.dy, h.dz, h.nd, h.ndm, h.prad, h.pdfm};
     for (int k = 0; k < maxK; ++k) {
         bool live = k < slots;
-        for (int q = 0; q < 7; ++q) {
-            if (live && (rc = download(c, a.data(), src[q] + (size_t)k * c->cap, (size_t)n))) return rc;
-            for (int i = 0; i < n; ++i) {
-                size_t o = (size_t)ids[i] * maxK + k;
-                double v = (live && k < cnt[i]) ? a[i] : 0.0;
-                if (q < 3) {
-                    if (dispt) dispt[3 * o + q] = v;
-                } else if (dst[q]) {
-                    dst[q][o] = v;
-                }
-            }
-        }
-        if (live && (rc = download(c, fl.data(), h.flags + (size_t)k * c->cap, (size_t)n))) return rc;
+        if (live && (rc = download(c, rec.data(), h + (size_t)k * c->cap, (size_t)n))) return rc;
         for (int i = 0; i < n; ++i) {
             size_t o = (size_t)ids[i] * maxK + k;
-            int f = (live && k < cnt[i]) ? fl[i] : 0;
-            if (stage) stage[o] = f & 0xff;
-            if (lqdBrg) lqdBrg[o] = (f >> 8) & 1;
+            HistRec r;
+            memset(&r, 0, sizeof r);
+            if (live && k < cnt[i]) r = rec[i];
+            if (dispt) {
+                dispt[3 * o] = r.dx;
+                dispt[3 * o + 1] = r.dy;
+                dispt[3 * o + 2] = r.dz;
+            }
+            if (nDsp) nDsp[o] = r.nd;
+            if (nDsp_mx) nDsp_mx[o] = r.ndm;
+            if (plstRad) plstRad[o] = r.prad;
+            if (plstDfm) plstDfm[o] = r.pdfm;
+            if (stage) stage[o] = r.flags & 0xff;
+            if (lqdBrg) lqdBrg[o] = (r.flags >> 8) & 1;
         }
     }
     return DEM_OK;
@@ -307,28 +311,25 @@ extern "C" int dem_get_hist_pw(dem_ctx *c, double *dispt, double *nDsp, double *
 }
 
 // restores lists and history in original order (identity permutation)
-static int put_hist(dem_ctx *c, const DevHist &h, int maxK, int n, const double *dispt,
+static int put_hist(dem_ctx *c, HistRec *h, int maxK, int n, const double *dispt,
                     const double *nDsp, const double *nDsp_mx, const double *plstRad,
                     const double *plstDfm, const int *stage, const int *lqdBrg) {
-    std::vector<double> a(n);
-    std::vector<int> fl(n);
-    const double *src[4] = {nDsp, nDsp_mx, plstRad, plstDfm};
-    double *dst[7] = {h.dx, h.dy, h.dz, h.nd, h.ndm, h.prad, h.pdfm};
+    std::vector<HistRec> rec(n);
     for (int k = 0; k < maxK; ++k) {
-        for (int q = 0; q < 7; ++q) {
-            for (int i = 0; i < n; ++i) {
-                size_t o = (size_t)i * maxK + k;
-                a[i] = q < 3 ? (dispt ? dispt[3 * o + q] : 0.0) : (src[q - 3] ? src[q - 3][o] : 0.0);
-            }
-            HIPCHK(hipMemcpyAsync(dst[q] + (size_t)k * c->cap, a.data(), n * sizeof(double),
-                                  hipMemcpyHostToDevice, c->stream));
-            HIPCHK(hipStreamSynchronize(c->stream));
-        }
         for (int i = 0; i < n; ++i) {
             size_t o = (size_t)i * maxK + k;
-            fl[i] = (stage ? (stage[o] & 0xff) : 0) | ((lqdBrg && lqdBrg[o]) ? 256 : 0);
+            HistRec &r = rec[i];
+            r.dx = dispt ? dispt[3 * o] : 0.0;
+            r.dy = dispt ? dispt[3 * o + 1] : 0.0;
+            r.dz = dispt ? dispt[3 * o + 2] : 0.0;
+            r.nd = nDsp ? nDsp[o] : 0.0;
+            r.ndm = nDsp_mx ? nDsp_mx[o] : 0.0;
+            r.prad = plstRad ? plstRad[o] : 0.0;
+            r.pdfm = plstDfm ? plstDfm[o] : 0.0;
+            r.flags = (stage ? (stage[o] & 0xff) : 0) | ((lqdBrg && lqdBrg[o]) ? 256 : 0);
+            r.pad = 0;
         }
-        HIPCHK(hipMemcpyAsync(h.flags + (size_t)k * c->cap, fl.data(), n * sizeof(int),
+        HIPCHK(hipMemcpyAsync(h + (size_t)k * c->cap, rec.data(), n * sizeof(HistRec),
                               hipMemcpyHostToDevice, c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));
     }
@@ -387,20 +388,43 @@ extern "C" int dem_set_contacts(dem_ctx *c, int maxK, const int *num, const int 
     return DEM_OK;
 }
 
+// The device tables give each cell's range in storage order (row-major or
+// Morton); the reference's tables (thrust-sorted (hash, index) pairs and
+// CellStart/CellEnd into them) follow from walking the cells in row-major
+// order, since a cell's particles are stored by original index either way.
+static int ref_tables(dem_ctx *c, int *hash, int *index, int *cellStart, int *cellEnd) {
+    const size_t nc = (size_t)c->S.sysNCells;
+    std::vector<int> cs(nc), ce(nc), ids;
+    int rc;
+    if ((rc = download(c, cs.data(), c->cstart, nc)) || (rc = download(c, ce.data(), c->cend, nc)) ||
+        (rc = host_ids(c, ids)))
+        return rc;
+    int acc = 0;
+    for (size_t cell = 0; cell < nc; ++cell) {
+        if (cs[cell] == DEM_EMPTY) {
+            if (cellStart) cellStart[cell] = DEM_EMPTY;
+            if (cellEnd) cellEnd[cell] = DEM_EMPTY;
+            continue;
+        }
+        const int n = ce[cell] - cs[cell];
+        if (cellStart) cellStart[cell] = acc;
+        if (cellEnd) cellEnd[cell] = acc + n;
+        for (int q = cs[cell]; q < ce[cell]; ++q, ++acc) {
+            if (hash) hash[acc] = (int)cell;
+            if (index) index[acc] = ids[q];
+        }
+    }
+    return DEM_OK;
+}
+
 extern "C" int dem_get_hash(dem_ctx *c, int *hash, int *index) {
     if (!c) return dem_fail(DEM_ERR_ARG, "null context");
-    // after a build, keys are the sorted hashes and the sorted order is the
-    // particle order, so the sorted original index is the id array
-    int rc;
-    if ((rc = download(c, (unsigned *)hash, c->keys, (size_t)c->total))) return rc;
-    return download(c, index, c->id[c->pcur], (size_t)c->total);
+    return ref_tables(c, hash, index, nullptr, nullptr);
 }
 
 extern "C" int dem_get_cells(dem_ctx *c, int *cellStart, int *cellEnd) {
     if (!c) return dem_fail(DEM_ERR_ARG, "null context");
-    int rc;
-    if ((rc = download(c, cellStart, c->cstart, (size_t)c->S.sysNCells))) return rc;
-    return download(c, cellEnd, c->cend, (size_t)c->S.sysNCells);
+    return ref_tables(c, nullptr, nullptr, cellStart, cellEnd);
 }
 
 // ---------------------------------------------------------------------
@@ -423,24 +447,24 @@ extern "C" int dem_bld_nebor_list(dem_ctx *c) {
     StateBuf a = c->buf(c->cur, c->Lidx), b = c->buf(1 - c->cur, 1 - c->Lidx);
     launch_calc_hash(a, c->level[c->pcur], c->id[c->pcur], c->P, c->keys, c->vals, n, s);
     launch_radix_sort(c->keys, c->vals, c->keys2, c->vals2, n, c->key_bits, c->sort_tmp, s);
-    launch_permute(c->vals, a, b, c->rad[c->pcur], c->rad[1 - c->pcur], c->level[c->pcur],
-                   c->level[1 - c->pcur], c->id[c->pcur], c->id[1 - c->pcur], c->pmove, n, s);
+    launch_permute(c->vals, a, b, c->level[c->pcur], c->level[1 - c->pcur], c->id[c->pcur],
+                   c->id[1 - c->pcur], n, s);
     c->cur = 1 - c->cur;
     c->Lidx = 1 - c->Lidx;
     c->pcur = 1 - c->pcur;
-    launch_cell_range(c->keys, c->cstart, c->cend, c->S.sysNCells, n, s);
+    launch_cell_range(c->keys, c->buf(c->cur, c->Lidx), c->level[c->pcur], c->P, c->cstart,
+                      c->cend, c->S.sysNCells, n, s);
     launch_fill_i32(c->overflow, 0, 1, s);
-    launch_neighbor_find(c->buf(c->cur, c->Lidx), c->rad[c->pcur], c->level[c->pcur], c->P,
-                         c->cstart, c->cend, c->nbr[c->hcur], c->nnb[c->hcur], c->overflow, c->cap,
-                         n, s);
+    launch_neighbor_find(c->buf(c->cur, c->Lidx), c->level[c->pcur], c->P, c->cstart, c->cend,
+                         c->nbr[c->hcur], c->nnb[c->hcur], c->overflow, c->cap, n, s);
     HIPCHK(hipGetLastError());
     return DEM_OK;
 }
 
 extern "C" int dem_bld_cir_wall(dem_ctx *c) {
     if (!c) return dem_fail(DEM_ERR_ARG, "null context");
-    launch_wall_list(c->buf(c->cur, c->Lidx), c->rad[c->pcur], c->P, c->Tw, c->wl[c->hcur],
-                     c->nwl[c->hcur], c->cap, c->total, c->stream);
+    launch_wall_list(c->buf(c->cur, c->Lidx), c->P, c->Tw, c->wl[c->hcur], c->nwl[c->hcur], c->cap,
+                     c->total, c->stream);
     HIPCHK(hipGetLastError());
     return DEM_OK;
 }
@@ -449,8 +473,8 @@ extern "C" int dem_hstry_copy(dem_ctx *c) {
     if (!c) return dem_fail(DEM_ERR_ARG, "null context");
     const int o = 1 - c->hcur;
     launch_hist_remap(c->vals, c->id[c->pcur], c->id[1 - c->pcur], c->nbr[c->hcur], c->nnb[c->hcur],
-                      c->nbr[o], c->nnb[o], c->hp[c->hcur].ptr(), c->hp[o].ptr(), c->wl[c->hcur],
-                      c->nwl[c->hcur], c->wl[o], c->nwl[o], c->hw[c->hcur].ptr(), c->hw[o].ptr(),
+                      c->nbr[o], c->nnb[o], c->hp[c->hcur], c->hp[o], c->wl[c->hcur],
+                      c->nwl[c->hcur], c->wl[o], c->nwl[o], c->hw[c->hcur], c->hw[o],
                       c->cap, c->total, c->saved ? c->old_total : 0, c->stream);
     HIPCHK(hipGetLastError());
     c->saved = false;
@@ -537,8 +561,25 @@ extern "C" int dem_step_unfused(dem_ctx *c) {
 // ---------------------------------------------------------------------
 // the fused path
 // ---------------------------------------------------------------------
+static bool use_compact(const dem_ctx *c) {
+    if (c->S.forceCap == 1 || c->K > 64) return false;
+    return c->variant != DEM_KERNEL_LOOP;
+}
+
+extern "C" int dem_set_kernel(dem_ctx *c, int variant) {
+    if (!c) return dem_fail(DEM_ERR_ARG, "null context");
+    if (variant != DEM_KERNEL_AUTO && variant != DEM_KERNEL_LOOP && variant != DEM_KERNEL_COMPACT)
+        return dem_fail(DEM_ERR_ARG, "unknown kernel variant %d", variant);
+    if (variant == DEM_KERNEL_COMPACT && (c->S.forceCap == 1 || c->K > 64))
+        return dem_fail(DEM_ERR_ARG, "the compacting kernel needs a dry packing with <= 64 slots");
+    c->variant = variant;
+    return DEM_OK;
+}
+
 static int run_chunk(dem_ctx *c, int nsteps, int *executed) {
     const bool wet = c->S.forceCap == 1;
+    const bool compact = use_compact(c);
+    const bool mixed = c->precision == DEM_PREC_MIXED;
     int big = INT_MAX;
     HIPCHK(hipMemcpyAsync(c->stop_at, &big, sizeof(int), hipMemcpyHostToDevice, c->stream));
     StepArgs A = step_args(c, true);
@@ -552,7 +593,10 @@ static int run_chunk(dem_ctx *c, int nsteps, int *executed) {
         A.step = s;
         tw = tw + c->S.topv * c->S.dt;
         A.Tw = tw;
-        launch_step_fused(c->P, A, wet, c->precision == DEM_PREC_MIXED, c->stream);
+        if (compact)
+            launch_step_compact(c->P, A, c->K, mixed, c->stream);
+        else
+            launch_step_fused(c->P, A, wet, mixed, c->stream);
     }
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(c->ev1, c->stream));

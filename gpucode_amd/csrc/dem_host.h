@@ -19,13 +19,6 @@ extern "C" int dem_read_deck(const char *path, dem_input *in);
                             hipGetErrorString(e_));                                     \
     } while (0)
 
-struct DevHist {
-    double *dx = nullptr, *dy = nullptr, *dz = nullptr, *nd = nullptr, *ndm = nullptr,
-           *prad = nullptr, *pdfm = nullptr;
-    int *flags = nullptr;
-    HistPtr ptr() const { return HistPtr{dx, dy, dz, nd, ndm, prad, pdfm, flags}; }
-};
-
 struct dem_ctx {
     dem_input in;
     dem_setup S;
@@ -37,20 +30,19 @@ struct dem_ctx {
     std::vector<double> rad_h;   // reduced radius by original id
     std::vector<int> level_h;    // size level by original id
 
-    // particle state, cell-sorted, two buffers
-    double *st[2][9] = {};       // x y z vx vy vz wx wy wz
+    // particle records, cell-sorted, two buffers
+    double4 *P4[2] = {};         // (x, y, z, r)
+    double4 *VW[2] = {};         // (vx, vy, vz, wx), (wy, wz, pmove, -)
     int cur = 0;
     int *lq[2] = {};             // bridge counts: lq[Lidx] = lqdCnt, other = lqdCnt_Old
     int Lidx = 0;
-    double *rad[2] = {};
     int *level[2] = {}, *id[2] = {};
     int pcur = 0;
-    double *pmove = nullptr;
 
     // lists and history, current = [hcur], old = [1-hcur]
     int *nbr[2] = {}, *nnb[2] = {};
     int *wl[2] = {}, *nwl[2] = {};
-    DevHist hp[2], hw[2];
+    HistRec *hp[2] = {}, *hw[2] = {};
     int hcur = 0;
     bool has_lists = false, saved = false;
     int old_total = 0;           // particles covered by the old lists
@@ -63,6 +55,8 @@ struct dem_ctx {
     // force arrays (separate-kernel path and optional output of the fused path)
     double *F[7] = {};           // Fx Fy Fz Mx My Mz Mr
     int store_forces = 0;
+    int variant = DEM_KERNEL_AUTO;  // fused step kernel choice
+    bool want_morton = true;        // Morton storage order when the grid allows
 
     // device flags and counters
     int *stop_at = nullptr, *overflow = nullptr, *upd_count = nullptr;
@@ -82,9 +76,8 @@ struct dem_ctx {
 
     StateBuf buf(int b, int lqd_idx) const {
         StateBuf s;
-        s.x = st[b][0]; s.y = st[b][1]; s.z = st[b][2];
-        s.vx = st[b][3]; s.vy = st[b][4]; s.vz = st[b][5];
-        s.wx = st[b][6]; s.wy = st[b][7]; s.wz = st[b][8];
+        s.p = P4[b];
+        s.v = VW[b];
         s.lqd = lq[lqd_idx];
         return s;
     }
@@ -92,5 +85,7 @@ struct dem_ctx {
 
 // DevInst from the derived setup (one instance)
 void dem_fill_inst(const dem_setup &S, DevInst &P);
+// dem_fill_inst plus the context's storage order and sort-key width
+void dem_refresh_inst(dem_ctx *c);
 // Internal pieces of the rebuild and step, shared by the C ABI functions.
 int dem_do_rebuild(dem_ctx *c);

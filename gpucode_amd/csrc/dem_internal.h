@@ -1,20 +1,20 @@
 // dem_internal.h -- device-side data layout of the DEM engine (gfx950).
 //
-// Layout in HBM (cap = particle capacity, K = max neighbour slots):
-//   particle state  SoA FP64, in cell-sorted order, two buffers (A read,
-//                   B written by the fused step, swapped every step):
-//                   x y z | vx vy vz | wx wy wz, plus lqdCnt (int) per buffer
-//   rad, pmove      FP64 [cap];  level, inst, id [cap] int
-//   neighbour list  int32 nbr[k*cap + i] (sorted index of j), nnb[cap];
-//                   a current and an old copy (swapped at each rebuild)
-//   p-p history     per slot k*cap+i: dispt x,y,z, nDsp, nDsp_mx, plstRad,
-//                   plstDfm (FP64 SoA) + flags (int: stage | lqdBrg << 8);
-//                   current and old copies
-//   wall list       wl[k*cap + i] (k < 3), nwl[cap]; wall history as p-p
-//                   with 3 slots; current and old copies
-// Slot-major SoA keeps the accesses of one wavefront to slot k of 64
-// consecutive particles contiguous.
+// Layout in HBM (cap = particle capacity, K = max neighbour slots), all in
+// cell-sorted particle order:
+//   P  [cap] double4    (x, y, z, r)                       32 B, two buffers
+//   VW [2*cap] double4  (vx, vy, vz, wx), (wy, wz, pmove, -) 64 B, two buffers
+//   lqd [cap] int        bridge count, two buffers (lqdCnt / lqdCnt_Old)
+//   nbr [K*cap] int      slot-major: nbr[k*cap + i] = sorted index of the
+//                        k-th neighbour of i; nnb[cap] counts
+//   H  [K*cap] HistRec   64-byte contact history per directed slot, slot-major
+//   wl [3*cap] int, nwl [cap], HW [3*cap] HistRec  wall slots
+// Buffers A (read) and B (written) of P/VW swap every fused step.  One
+// record per particle or slot means a gather touches one 32/64-byte sector
+// instead of one line per field; slot-major order keeps slot k of 64
+// consecutive particles contiguous for the wavefront that walks them.
 #pragma once
+#include <hip/hip_runtime.h>
 #include <stdint.h>
 
 #define DEM_PI 3.1415926  // heydem.h:15
@@ -54,28 +54,36 @@ struct DevInst {
     int cellOff[DEM_MAXL];     // hash offset of each level grid
     int levelK[DEM_MAXL];      // nNebors of each level
     int nLevels;
-    int pad2_;
+    // storage order of the particles: 0 = row-major cell hash (the
+    // reference's sort key), 1 = Morton (Z-order) code of the cell, which
+    // keeps 3-D neighbours close in memory.  Either way particles of one
+    // cell are stored by original index.
+    int morton;
+    unsigned mortonOff[DEM_MAXL];  // key offset of each level's Morton range
 };
 
-struct HistPtr {
-    double *dx, *dy, *dz, *nd, *ndm, *prad, *pdfm;
-    int *flags;
+// Seven history fields of one directed contact (heydem.h:153-190) plus the
+// flags word: stage in bits 0-7, lqdBrg in bit 8.
+struct __attribute__((aligned(64))) HistRec {
+    double dx, dy, dz, nd, ndm, prad, pdfm;
+    int flags, pad;
 };
+
+typedef double4 dvec4;
 
 struct StateBuf {
-    double *x, *y, *z, *vx, *vy, *vz, *wx, *wy, *wz;
+    dvec4 *p;    // (x, y, z, r)
+    dvec4 *v;    // 2 records per particle
     int *lqd;
 };
 
 // Everything one step kernel touches.
 struct StepArgs {
     StateBuf in, out;          // read A, write B (in == out for in-place)
-    const double *rad;
-    double *pmove;
     const int *nbr, *nnb;
-    HistPtr h;
+    HistRec *h;
     const int *wl, *nwl;
-    HistPtr hw;
+    HistRec *hw;
     double *Fx, *Fy, *Fz, *Mx, *My, *Mz, *Mr;  // optional force output
     int *stop_at;              // device rebuild request (step index)
     int cap, total, step, store_forces;

@@ -261,75 +261,98 @@ DEV int cell_coord(double p, double o, double cs, int g) {
 // keys[id] = hash, vals[id] = current index: sorting the id-ordered pairs
 // stably by hash yields (hash, original index) order, the order
 // thrust::sort_by_key produces from calcHashG's output.
-__global__ void k_calc_hash(const double *x, const double *y, const double *z, const int *level,
-                            const int *id, DevInst P, unsigned *keys, unsigned *vals, int total) {
+// spreads the low 10 bits of v to every third bit
+DEV unsigned spread3(unsigned v) {
+    v &= 0x3ffu;
+    v = (v | (v << 16)) & 0x030000ffu;
+    v = (v | (v << 8)) & 0x0300f00fu;
+    v = (v | (v << 4)) & 0x030c30c3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+}
+
+// row-major cell hash of the reference (kernel.cu:627-640) and, in Morton
+// mode, the Z-order key used as the storage order
+DEV void cell_of(const DevInst &P, const double4 q, int L, int &hash, unsigned &key) {
+    const double cs = P.cellSize[L];
+    const int cx = cell_coord(q.x, P.origin[0], cs, P.gx[L]);
+    const int cy = cell_coord(q.y, P.origin[1], cs, P.gy[L]);
+    const int cz = cell_coord(q.z, P.origin[2], cs, P.gz[L]);
+    hash = (cz * P.gy[L] * P.gx[L]) + (cy * P.gx[L]) + cx + P.cellOff[L];
+    key = P.morton ? P.mortonOff[L] + (spread3(cx) | (spread3(cy) << 1) | (spread3(cz) << 2))
+                   : (unsigned)hash;
+}
+
+__global__ void k_calc_hash(const double4 *p, const int *level, const int *id, DevInst P,
+                            unsigned *keys, unsigned *vals, int total) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= total) return;
-    int L = level[i];
-    double cs = P.cellSize[L];
-    int cx = cell_coord(x[i], P.origin[0], cs, P.gx[L]);
-    int cy = cell_coord(y[i], P.origin[1], cs, P.gy[L]);
-    int cz = cell_coord(z[i], P.origin[2], cs, P.gz[L]);
-    int h = (cz * P.gy[L] * P.gx[L]) + (cy * P.gx[L]) + cx + P.cellOff[L];
-    keys[id[i]] = (unsigned)h;
+    int h;
+    unsigned key;
+    cell_of(P, p[i], level[i], h, key);
+    keys[id[i]] = key;
     vals[id[i]] = (unsigned)i;
 }
 
 void launch_calc_hash(const StateBuf &s, const int *level, const int *id, const DevInst &P,
                       unsigned *keys, unsigned *vals, int total, hipStream_t st) {
     if (total <= 0) return;
-    hipLaunchKernelGGL(k_calc_hash, dim3(blocks_for(total, 256)), dim3(256), 0, st, s.x, s.y, s.z,
-                       level, id, P, keys, vals, total);
+    hipLaunchKernelGGL(k_calc_hash, dim3(blocks_for(total, 256)), dim3(256), 0, st, s.p, level,
+                       id, P, keys, vals, total);
 }
 
 // ---------------------------------------------------------------------
-// permutation into cell order
+// permutation into cell order; zeroes the path length (main.cpp:492)
 // ---------------------------------------------------------------------
-__global__ void k_permute(const unsigned *perm, StateBuf a, StateBuf b, const double *rad_a,
-                          double *rad_b, const int *lvl_a, int *lvl_b, const int *id_a, int *id_b,
-                          double *pmove, int total) {
+__global__ void k_permute(const unsigned *perm, StateBuf a, StateBuf b, const int *lvl_a,
+                          int *lvl_b, const int *id_a, int *id_b, int total) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= total) return;
     int o = (int)perm[i];
-    b.x[i] = a.x[o]; b.y[i] = a.y[o]; b.z[i] = a.z[o];
-    b.vx[i] = a.vx[o]; b.vy[i] = a.vy[o]; b.vz[i] = a.vz[o];
-    b.wx[i] = a.wx[o]; b.wy[i] = a.wy[o]; b.wz[i] = a.wz[o];
+    b.p[i] = a.p[o];
+    b.v[2 * i] = a.v[2 * o];
+    double4 v1 = a.v[2 * o + 1];
+    v1.z = 0.0;
+    b.v[2 * i + 1] = v1;
     b.lqd[i] = a.lqd[o];
-    rad_b[i] = rad_a[o];
     lvl_b[i] = lvl_a[o];
     id_b[i] = id_a[o];
-    pmove[i] = 0.0;  // main.cpp:492
 }
 
-void launch_permute(const unsigned *perm, const StateBuf &a, const StateBuf &b,
-                    const double *rad_a, double *rad_b, const int *lvl_a, int *lvl_b,
-                    const int *id_a, int *id_b, double *pmove, int total, hipStream_t s) {
+void launch_permute(const unsigned *perm, const StateBuf &a, const StateBuf &b, const int *lvl_a,
+                    int *lvl_b, const int *id_a, int *id_b, int total, hipStream_t s) {
     if (total <= 0) return;
-    hipLaunchKernelGGL(k_permute, dim3(blocks_for(total, 256)), dim3(256), 0, s, perm, a, b, rad_a,
-                       rad_b, lvl_a, lvl_b, id_a, id_b, pmove, total);
+    hipLaunchKernelGGL(k_permute, dim3(blocks_for(total, 256)), dim3(256), 0, s, perm, a, b, lvl_a,
+                       lvl_b, id_a, id_b, total);
 }
 
 // ---------------------------------------------------------------------
 // cell ranges (kernel.cu:682-724)
 // ---------------------------------------------------------------------
-__global__ void k_cell_range(const unsigned *keys, int *cstart, int *cend, int total) {
+// CellStart/CellEnd indexed by the row-major cell hash; a cell's particles
+// are contiguous in storage in either order (equal sort keys = same cell)
+__global__ void k_cell_range(const unsigned *keys, const double4 *p, const int *level, DevInst P,
+                             int *cstart, int *cend, int total) {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= total) return;
-    unsigned h = keys[k];
-    if (k == 0 || h != keys[k - 1]) {
-        cstart[h] = k;
-        if (k > 0) cend[keys[k - 1]] = k;
-    }
-    if (k == total - 1) cend[h] = k + 1;
+    const unsigned key = keys[k];
+    const bool first = k == 0 || key != keys[k - 1];
+    const bool last = k == total - 1 || key != keys[k + 1];
+    if (!first && !last) return;
+    int h;
+    unsigned kk;
+    cell_of(P, p[k], level[k], h, kk);
+    if (first) cstart[h] = k;
+    if (last) cend[h] = k + 1;
 }
 
-void launch_cell_range(const unsigned *keys, int *cstart, int *cend, int ncells, int total,
-                       hipStream_t s) {
-    launch_fill_i32(cstart, DEM_EMPTY, ncells, s);
-    launch_fill_i32(cend, DEM_EMPTY, ncells, s);
+void launch_cell_range(const unsigned *keys, const StateBuf &s, const int *level, const DevInst &P,
+                       int *cstart, int *cend, int ncells, int total, hipStream_t st) {
+    launch_fill_i32(cstart, DEM_EMPTY, ncells, st);
+    launch_fill_i32(cend, DEM_EMPTY, ncells, st);
     if (total <= 0) return;
-    hipLaunchKernelGGL(k_cell_range, dim3(blocks_for(total, 256)), dim3(256), 0, s, keys, cstart,
-                       cend, total);
+    hipLaunchKernelGGL(k_cell_range, dim3(blocks_for(total, 256)), dim3(256), 0, st, keys, s.p,
+                       level, P, cstart, cend, total);
 }
 
 // ---------------------------------------------------------------------
@@ -338,24 +361,23 @@ void launch_cell_range(const unsigned *keys, int *cstart, int *cend, int ncells,
 // The file is compiled with -ffp-contract=off, so the distance test rounds
 // like the host restatement.
 // ---------------------------------------------------------------------
-__global__ void __launch_bounds__(128) k_neighbor_find(const double *x, const double *y,
-                                                       const double *z, const double *rad,
-                                                       const int *level, DevInst P,
-                                                       const int *cstart, const int *cend,
-                                                       int *nbr, int *nnb, int *overflow, int cap,
-                                                       int total) {
+__global__ void __launch_bounds__(128) k_neighbor_find(const double4 *p, const int *level,
+                                                       DevInst P, const int *cstart,
+                                                       const int *cend, int *nbr, int *nnb,
+                                                       int *overflow, int cap, int total) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= total) return;
-    const double ox = x[i], oy = y[i], oz = z[i], ri = rad[i];
+    const double4 o = p[i];
+    const double ri = o.w;
     const int iL = level[i];
     const int K = P.levelK[iL];
     int num = 0;
     for (int L = 0; L < P.nLevels; ++L) {
         const double cs = P.cellSize[L];
         const int gx = P.gx[L], gy = P.gy[L], gz = P.gz[L], coff = P.cellOff[L];
-        const int cx = cell_coord(ox, P.origin[0], cs, gx);
-        const int cy = cell_coord(oy, P.origin[1], cs, gy);
-        const int cz = cell_coord(oz, P.origin[2], cs, gz);
+        const int cx = cell_coord(o.x, P.origin[0], cs, gx);
+        const int cy = cell_coord(o.y, P.origin[1], cs, gy);
+        const int cz = cell_coord(o.z, P.origin[2], cs, gz);
         const int off = (int)ceil(P.levelSize[iL] / P.levelSize[L]);
         for (int dz = -off; dz <= off; ++dz) {
             const int zz = cz + dz;
@@ -367,24 +389,22 @@ __global__ void __launch_bounds__(128) k_neighbor_find(const double *x, const do
                 if (sx < 0) sx = 0;
                 if (ex > gx - 1) ex = gx - 1;
                 const int rowb = coff + zz * gy * gx + yy * gx;
-                while (sx != ex) {
-                    if (cstart[rowb + sx] == DEM_EMPTY) ++sx;
-                    else break;
-                }
-                const int s0 = cstart[rowb + sx];
-                while (sx != ex) {
-                    if (cend[rowb + ex] == DEM_EMPTY) --ex;
-                    else break;
-                }
-                const int e0 = cend[rowb + ex];
-                if (s0 == DEM_EMPTY || e0 == DEM_EMPTY) continue;
-                for (int j = s0; j < e0; ++j) {
-                    if (j == i) continue;
-                    double dx = ox - x[j], dy2 = oy - y[j], dz2 = oz - z[j];
-                    double len = sqrt(dx * dx + dy2 * dy2 + dz2 * dz2);
-                    if (len < rad[j] + ri + P.searchGap) {
-                        if (num < K) nbr[(long)num * cap + i] = j;
-                        ++num;
+                // the reference scans the sorted range from the first to the
+                // last non-empty cell of the row; cell by cell in x order is
+                // the same sequence, and works for either storage order
+                for (int x = sx; x <= ex; ++x) {
+                    const int s0 = cstart[rowb + x];
+                    if (s0 == DEM_EMPTY) continue;
+                    const int e0 = cend[rowb + x];
+                    for (int j = s0; j < e0; ++j) {
+                        if (j == i) continue;
+                        const double4 q = p[j];
+                        double dx = o.x - q.x, dy2 = o.y - q.y, dz2 = o.z - q.z;
+                        double len = sqrt(dx * dx + dy2 * dy2 + dz2 * dz2);
+                        if (len < q.w + ri + P.searchGap) {
+                            if (num < K) nbr[(long)num * cap + i] = j;
+                            ++num;
+                        }
                     }
                 }
             }
@@ -397,36 +417,37 @@ __global__ void __launch_bounds__(128) k_neighbor_find(const double *x, const do
     nnb[i] = num;
 }
 
-void launch_neighbor_find(const StateBuf &s, const double *rad, const int *level,
-                          const DevInst &P, const int *cstart, const int *cend, int *nbr, int *nnb,
-                          int *overflow, int cap, int total, hipStream_t st) {
+void launch_neighbor_find(const StateBuf &s, const int *level, const DevInst &P, const int *cstart,
+                          const int *cend, int *nbr, int *nnb, int *overflow, int cap, int total,
+                          hipStream_t st) {
     if (total <= 0) return;
-    hipLaunchKernelGGL(k_neighbor_find, dim3(blocks_for(total, 128)), dim3(128), 0, st, s.x, s.y,
-                       s.z, rad, level, P, cstart, cend, nbr, nnb, overflow, cap, total);
+    hipLaunchKernelGGL(k_neighbor_find, dim3(blocks_for(total, 128)), dim3(128), 0, st, s.p, level,
+                       P, cstart, cend, nbr, nnb, overflow, cap, total);
 }
 
 // ---------------------------------------------------------------------
 // cylinder wall candidates (kernel.cu:1034-1073)
 // ---------------------------------------------------------------------
-__global__ void k_wall_list(const double *x, const double *y, const double *z, const double *rad,
-                            DevInst P, double Tw, int *wl, int *nwl, int cap, int total) {
+__global__ void k_wall_list(const double4 *p, DevInst P, double Tw, int *wl, int *nwl, int cap,
+                            int total) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= total) return;
-    const double px = x[i], py = y[i], pz = z[i], r = rad[i];
+    const double4 q = p[i];
+    const double r = q.w;
     int n = 0;
-    if (pz >= (Tw - (2.0 * r + P.searchGap))) wl[(long)(n++) * cap + i] = 0;
-    if (pz <= ((2.0 * r + P.searchGap) + P.Bw)) wl[(long)(n++) * cap + i] = 1;
-    double ax = P.cirX - px, ay = P.cirY - py;
+    if (q.z >= (Tw - (2.0 * r + P.searchGap))) wl[(long)(n++) * cap + i] = 0;
+    if (q.z <= ((2.0 * r + P.searchGap) + P.Bw)) wl[(long)(n++) * cap + i] = 1;
+    double ax = P.cirX - q.x, ay = P.cirY - q.y;
     double dc = sqrt(ax * ax + ay * ay);
     if (dc >= P.R - (2.0 * r + P.searchGap)) wl[(long)(n++) * cap + i] = 2;
     nwl[i] = n;
 }
 
-void launch_wall_list(const StateBuf &s, const double *rad, const DevInst &P, double Tw, int *wl,
-                      int *nwl, int cap, int total, hipStream_t st) {
+void launch_wall_list(const StateBuf &s, const DevInst &P, double Tw, int *wl, int *nwl, int cap,
+                      int total, hipStream_t st) {
     if (total <= 0) return;
-    hipLaunchKernelGGL(k_wall_list, dim3(blocks_for(total, 256)), dim3(256), 0, st, s.x, s.y, s.z,
-                       rad, P, Tw, wl, nwl, cap, total);
+    hipLaunchKernelGGL(k_wall_list, dim3(blocks_for(total, 256)), dim3(256), 0, st, s.p, P, Tw, wl,
+                       nwl, cap, total);
 }
 
 // ---------------------------------------------------------------------
@@ -435,22 +456,18 @@ void launch_wall_list(const StateBuf &s, const double *rad, const DevInst &P, do
 // partner in the old list, or zero.  Partners are compared by original id
 // because the particle order changes at every rebuild.
 // ---------------------------------------------------------------------
-DEV void hist_copy(HistPtr d, long ds, HistPtr s, long ss) {
-    d.dx[ds] = s.dx[ss]; d.dy[ds] = s.dy[ss]; d.dz[ds] = s.dz[ss];
-    d.nd[ds] = s.nd[ss]; d.ndm[ds] = s.ndm[ss];
-    d.prad[ds] = s.prad[ss]; d.pdfm[ds] = s.pdfm[ss];
-    d.flags[ds] = s.flags[ss];
-}
-DEV void hist_zero(HistPtr d, long ds) {
-    d.dx[ds] = 0.0; d.dy[ds] = 0.0; d.dz[ds] = 0.0;
-    d.nd[ds] = 0.0; d.ndm[ds] = 0.0; d.prad[ds] = 0.0; d.pdfm[ds] = 0.0;
-    d.flags[ds] = 0;
+DEV HistRec hist_zero() {
+    HistRec r;
+    r.dx = 0.0; r.dy = 0.0; r.dz = 0.0; r.nd = 0.0; r.ndm = 0.0; r.prad = 0.0; r.pdfm = 0.0;
+    r.flags = 0;
+    r.pad = 0;
+    return r;
 }
 
 __global__ void k_hist_remap(const unsigned *perm, const int *id_new, const int *id_old,
                              const int *nbr, const int *nnb, const int *nbr_old,
-                             const int *nnb_old, HistPtr h, HistPtr h_old, int cap, int total,
-                             int old_total) {
+                             const int *nnb_old, HistRec *h, const HistRec *h_old, int cap,
+                             int total, int old_total) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= total) return;
     int p = (int)perm[i];
@@ -462,16 +479,13 @@ __global__ void k_hist_remap(const unsigned *perm, const int *id_new, const int 
         int hit = -1;
         for (int b = 0; b < nold; ++b)
             if (id_old[nbr_old[(long)b * cap + p]] == jid) hit = b;
-        if (hit >= 0)
-            hist_copy(h, sa, h_old, (long)hit * cap + p);
-        else
-            hist_zero(h, sa);
+        h[sa] = hit >= 0 ? h_old[(long)hit * cap + p] : hist_zero();
     }
 }
 
 __global__ void k_hist_remap_pw(const unsigned *perm, const int *wl, const int *nwl,
-                                const int *wl_old, const int *nwl_old, HistPtr h, HistPtr h_old,
-                                int cap, int total, int old_total) {
+                                const int *wl_old, const int *nwl_old, HistRec *h,
+                                const HistRec *h_old, int cap, int total, int old_total) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= total) return;
     int p = (int)perm[i];
@@ -481,17 +495,14 @@ __global__ void k_hist_remap_pw(const unsigned *perm, const int *wl, const int *
         int w = wl[sa], hit = -1;
         for (int b = 0; b < nold; ++b)
             if (wl_old[(long)b * cap + p] == w) hit = b;
-        if (hit >= 0)
-            hist_copy(h, sa, h_old, (long)hit * cap + p);
-        else
-            hist_zero(h, sa);
+        h[sa] = hit >= 0 ? h_old[(long)hit * cap + p] : hist_zero();
     }
 }
 
 void launch_hist_remap(const unsigned *perm, const int *id_new, const int *id_old, const int *nbr,
-                       const int *nnb, const int *nbr_old, const int *nnb_old, HistPtr h,
-                       HistPtr h_old, const int *wl, const int *nwl, const int *wl_old,
-                       const int *nwl_old, HistPtr hw, HistPtr hw_old, int cap, int total,
+                       const int *nnb, const int *nbr_old, const int *nnb_old, HistRec *h,
+                       const HistRec *h_old, const int *wl, const int *nwl, const int *wl_old,
+                       const int *nwl_old, HistRec *hw, const HistRec *hw_old, int cap, int total,
                        int old_total, hipStream_t s) {
     if (total <= 0) return;
     hipLaunchKernelGGL(k_hist_remap, dim3(blocks_for(total, 128)), dim3(128), 0, s, perm, id_new,
@@ -583,27 +594,10 @@ DEV double dm_log(double x) {
 
 DEV double dm_pow(double x, double y) { return dm_exp(y * dm_log(x)); }
 
-struct Hist1 {
-    double dx, dy, dz, nd, ndm, prad, pdfm;
-    int flags;
-};
-DEV Hist1 hist_load(const HistPtr &h, long s) {
-    Hist1 r;
-    r.dx = h.dx[s]; r.dy = h.dy[s]; r.dz = h.dz[s];
-    r.nd = h.nd[s]; r.ndm = h.ndm[s]; r.prad = h.prad[s]; r.pdfm = h.pdfm[s];
-    r.flags = h.flags[s];
-    return r;
-}
-DEV void hist_store(const HistPtr &h, long s, const Hist1 &r) {
-    h.dx[s] = r.dx; h.dy[s] = r.dy; h.dz[s] = r.dz;
-    h.nd[s] = r.nd; h.ndm[s] = r.ndm; h.prad[s] = r.prad; h.pdfm[s] = r.pdfm;
-    h.flags[s] = r.flags;
-}
-
 // normal force with the load/unload/reload verdict (kernel.cu:104-144 p-p,
 // 250-290 p-w; YIELD_LE selects the p-p yield test "<=")
 template <bool YIELD_LE>
-DEV double normal_law(const DevLaw &c, Hist1 &h, double d, double R) {
+DEV double normal_law(const DevLaw &c, HistRec &h, double d, double R) {
     int stage;
     if (d < h.pdfm) stage = 1;
     else if (d >= h.pdfm && d < h.nd) stage = 3;
@@ -633,7 +627,7 @@ DEV double normal_law(const DevLaw &c, Hist1 &h, double d, double R) {
 // tangential spring (kernel.cu:172-217 p-p with 1e-7 tolerances,
 // 306-350 p-w with 1e-10)
 template <bool PP>
-DEV V3 tangential_law(const DevLaw &c, Hist1 &h, double fN, double d, double R, V3 n, V3 vt,
+DEV V3 tangential_law(const DevLaw &c, HistRec &h, double fN, double d, double R, V3 n, V3 vt,
                       double dt) {
     const double tol = PP ? 1.0e-7 : 1.0e-10;
     V3 old = v3(h.dx, h.dy, h.dz);
@@ -669,7 +663,7 @@ DEV V3 tangential_law(const DevLaw &c, Hist1 &h, double fN, double d, double R, 
 
 // one directed contact; cpv = contact-point relative velocity
 template <bool PP>
-DEV void contact(const DevLaw &c, double rolling, double dt, Hist1 &h, double d, V3 n, double R,
+DEV void contact(const DevLaw &c, double rolling, double dt, HistRec &h, double d, V3 n, double R,
                  double ri, V3 cpv, V3 &F, V3 &M, double &Mr) {
     double fN = normal_law<PP>(c, h, d, R);
     double cpvn = dot(cpv, n);
@@ -693,7 +687,7 @@ DEV void contact(const DevLaw &c, double rolling, double dt, Hist1 &h, double d,
 // square roots and powers (Hertz, plastic radius, damping, xi_max, Mindlin
 // ratio) and the force/moment vectors of this pair.
 template <bool PP>
-DEV void contact_mixed(const DevLaw &c, double rolling, double dt, Hist1 &h, double d, V3 n,
+DEV void contact_mixed(const DevLaw &c, double rolling, double dt, HistRec &h, double d, V3 n,
                        double R, double ri, V3 cpv, V3 &F, V3 &M, double &Mr) {
     const float tol = PP ? 1.0e-7f : 1.0e-10f;
     int stage;
@@ -799,15 +793,22 @@ DEV double rupture(const DevInst &P, double V) {
                               : 0.99 * dm_pow(V, 0.33);
 }
 
+DEV V3 vel_of(const double4 *v, int i) {
+    const double4 a = v[2 * i];
+    return v3(a.x, a.y, a.z);
+}
+DEV V3 spin_of(const double4 *v, int i) {
+    const double4 a = v[2 * i], b = v[2 * i + 1];
+    return v3(a.w, b.x, b.y);
+}
+
 // gravity + all p-p slots of particle i (kernel.cu:424-551)
 template <bool WET, bool MIXED>
-DEV void pp_particle(const DevInst &P, const StepArgs &A, int i, V3 &F, V3 &M, double &Mr,
-                     int &nbridge) {
+DEV void pp_particle(const DevInst &P, const StepArgs &A, int i, const double4 pi, V3 iv, V3 iw,
+                     V3 &F, V3 &M, double &Mr, int &nbridge) {
     const StateBuf &s = A.in;
-    const V3 ip = v3(s.x[i], s.y[i], s.z[i]);
-    const V3 iv = v3(s.vx[i], s.vy[i], s.vz[i]);
-    const V3 iw = v3(s.wx[i], s.wy[i], s.wz[i]);
-    const double ri = A.rad[i];
+    const V3 ip = v3(pi.x, pi.y, pi.z);
+    const double ri = pi.w;
     double iVol = 0.0;
     if (WET) {
         int inl = s.lqd[i];
@@ -821,8 +822,9 @@ DEV void pp_particle(const DevInst &P, const StepArgs &A, int i, V3 &F, V3 &M, d
     for (int k = 0; k < n; ++k) {
         const long sk = (long)k * A.cap + i;
         const int j = A.nbr[sk];
-        const V3 jp = v3(s.x[j], s.y[j], s.z[j]);
-        const double rj = A.rad[j];
+        const double4 pj = s.p[j];
+        const V3 jp = v3(pj.x, pj.y, pj.z);
+        const double rj = pj.w;
         const V3 dv = sub(ip, jp);
         if (MIXED && !WET) {
             // dry: pairs apart need no square root
@@ -838,17 +840,19 @@ DEV void pp_particle(const DevInst &P, const StepArgs &A, int i, V3 &F, V3 &M, d
             nv = divv(dv, l);
         }
         const double gap = l - ri - rj;
+        HistRec h;
+        bool loaded = false;
         if (gap < 0.0) {
-            const V3 jv = v3(s.vx[j], s.vy[j], s.vz[j]);
-            const V3 jw = v3(s.wx[j], s.wy[j], s.wz[j]);
-            Hist1 h = hist_load(A.h, sk);
+            const V3 jv = vel_of(s.v, j);
+            const V3 jw = spin_of(s.v, j);
+            h = A.h[sk];
+            loaded = true;
             double R = ri * rj / (ri + rj);
             V3 cpv = sub(sub(iv, jv), cross(add(mulv(iw, ri), mulv(jw, rj)), nv));
             if (MIXED)
                 contact_mixed<true>(P.pp, P.rolling, P.dt, h, -gap, nv, R, ri, cpv, F, M, Mr);
             else
                 contact<true>(P.pp, P.rolling, P.dt, h, -gap, nv, R, ri, cpv, F, M, Mr);
-            hist_store(A.h, sk, h);
         }
         if (WET && P.forceCap == 1) {
             int jnl = s.lqd[j];
@@ -857,12 +861,15 @@ DEV void pp_particle(const DevInst &P, const StepArgs &A, int i, V3 &F, V3 &M, d
             double jVol = P.vol * (dj * dj * dj) / jnl;
             double V = iVol + jVol;
             double rup = rupture(P, V);
-            int fl = A.h.flags[sk];
+            int fl = loaded ? h.flags : A.h[sk].flags;
             int brg = (fl >> 8) & 1;
             if (gap < P.layer2 * (ri + rj)) brg = 1;
             else if (gap > rup) brg = 0;
             int nfl = (fl & 0xff) | (brg << 8);
-            if (nfl != fl) A.h.flags[sk] = nfl;
+            if (loaded)
+                h.flags = nfl;
+            else if (nfl != fl)
+                A.h[sk].flags = nfl;
             double Sp = gap < 0 ? 0.0 : gap;
             if (brg == 1) {
                 double effR = 2 * ri * rj / (ri + rj);
@@ -872,19 +879,18 @@ DEV void pp_particle(const DevInst &P, const StepArgs &A, int i, V3 &F, V3 &M, d
                 nbridge = nbridge + 1;
             }
         }
+        if (loaded) A.h[sk] = h;
     }
 }
 
 // walls of particle i (kernel.cu:958-1023, 370-403)
 template <bool MIXED>
-DEV void pw_particle(const DevInst &P, const StepArgs &A, int i, V3 &F, V3 &M, double &Mr) {
-    const StateBuf &s = A.in;
+DEV void pw_particle(const DevInst &P, const StepArgs &A, int i, const double4 pi, V3 iv, V3 iw,
+                     V3 &F, V3 &M, double &Mr) {
     const int n = A.nwl[i];
     if (n == 0) return;
-    const V3 ip = v3(s.x[i], s.y[i], s.z[i]);
-    const V3 iv = v3(s.vx[i], s.vy[i], s.vz[i]);
-    const V3 iw = v3(s.wx[i], s.wy[i], s.wz[i]);
-    const double ri = A.rad[i];
+    const V3 ip = v3(pi.x, pi.y, pi.z);
+    const double ri = pi.w;
     V3 jp = v3(0.0, 0.0, 0.0), jv = v3(0.0, 0.0, 0.0);
     for (int k = 0; k < n; ++k) {
         const long sk = (long)k * A.cap + i;
@@ -907,45 +913,50 @@ DEV void pw_particle(const DevInst &P, const StepArgs &A, int i, V3 &F, V3 &M, d
         const V3 nv = divv(dv, l);
         const double gap = l - ri;
         if (gap < 0.0) {
-            Hist1 h = hist_load(A.hw, sk);
+            HistRec h = A.hw[sk];
             V3 cpv = sub(sub(iv, jv), cross(mulv(iw, ri), nv));
             if (MIXED)
                 contact_mixed<false>(P.pw, P.rolling, P.dt, h, -gap, nv, ri, ri, cpv, F, M, Mr);
             else
                 contact<false>(P.pw, P.rolling, P.dt, h, -gap, nv, ri, ri, cpv, F, M, Mr);
-            hist_store(A.hw, sk, h);
+            A.hw[sk] = h;
         }
     }
 }
 
 // semi-implicit Euler (kernel.cu:563-606); returns true when the particle's
 // path length passed the rebuild threshold
-DEV bool integrate(const DevInst &P, const StepArgs &A, int i, V3 F, V3 M, double Mr) {
-    const StateBuf &a = A.in, &b = A.out;
+DEV bool integrate(const DevInst &P, const StepArgs &A, int i, const double4 pi, const double4 va,
+                   const double4 vb, V3 F, V3 M, double Mr) {
     const double dt = P.dt;
-    const double di = 2.0 * A.rad[i];
+    const double di = 2.0 * pi.w;
     const double mass = di * di * di;
-    double vx = a.vx[i] + F.x * dt / mass;
-    double vy = a.vy[i] + F.y * dt / mass;
-    double vz = a.vz[i] + F.z * dt / mass;
-    b.vx[i] = vx; b.vy[i] = vy; b.vz[i] = vz;
-    b.x[i] = a.x[i] + vx * dt;
-    b.y[i] = a.y[i] + vy * dt;
-    b.z[i] = a.z[i] + vz * dt;
-    double pm = A.pmove[i] + sqrt(vx * vx + vy * vy + vz * vz) * dt;
-    A.pmove[i] = pm;
+    double vx = va.x + F.x * dt / mass;
+    double vy = va.y + F.y * dt / mass;
+    double vz = va.z + F.z * dt / mass;
+    double4 np;
+    np.x = pi.x + vx * dt;
+    np.y = pi.y + vy * dt;
+    np.z = pi.z + vz * dt;
+    np.w = pi.w;
+    double pm = vb.z + sqrt(vx * vx + vy * vy + vz * vz) * dt;
     const double d2 = di * di;
     const double inert = 0.1 * (d2 * d2 * di);
-    double wx = a.wx[i] + M.x * dt / inert;
-    double wy = a.wy[i] + M.y * dt / inert;
-    double wz = a.wz[i] + M.z * dt / inert;
+    double wx = va.w + M.x * dt / inert;
+    double wy = vb.x + M.y * dt / inert;
+    double wz = vb.y + M.z * dt / inert;
     double mr = Mr * dt / inert;
     double al = sqrt(wx * wx + wy * wy + wz * wz);
     if (al > 1.0e-7) {
         double f = (1.0 - fmin(1.0, mr / al));
         wx = wx * f; wy = wy * f; wz = wz * f;
     }
-    b.wx[i] = wx; b.wy[i] = wy; b.wz[i] = wz;
+    A.out.p[i] = np;
+    double4 o0, o1;
+    o0.x = vx; o0.y = vy; o0.z = vz; o0.w = wx;
+    o1.x = wy; o1.y = wz; o1.z = pm; o1.w = 0.0;
+    A.out.v[2 * i] = o0;
+    A.out.v[2 * i + 1] = o1;
     if (A.store_forces) A.Mr[i] = mr;
     return pm > P.rebuildThr;
 }
@@ -967,17 +978,20 @@ __global__ void __launch_bounds__(256) k_step_fused(DevInst P, StepArgs A) {
     const int i = (int)(xcd_block(nb) * blockDim.x + threadIdx.x);
     bool over = false;
     if (i < A.total) {
+        const double4 pi = A.in.p[i];
+        const double4 va = A.in.v[2 * i], vb = A.in.v[2 * i + 1];
+        const V3 iv = v3(va.x, va.y, va.z), iw = v3(va.w, vb.x, vb.y);
         V3 F = v3(0.0, 0.0, 0.0), M = v3(0.0, 0.0, 0.0);
         double Mr = 0.0;
         int nbridge = 0;
-        pp_particle<WET, MIXED>(P, A, i, F, M, Mr, nbridge);
+        pp_particle<WET, MIXED>(P, A, i, pi, iv, iw, F, M, Mr, nbridge);
         if (WET) A.out.lqd[i] = nbridge;
-        pw_particle<MIXED>(P, A, i, F, M, Mr);
+        pw_particle<MIXED>(P, A, i, pi, iv, iw, F, M, Mr);
         if (A.store_forces) {
             A.Fx[i] = F.x; A.Fy[i] = F.y; A.Fz[i] = F.z;
             A.Mx[i] = M.x; A.My[i] = M.y; A.Mz[i] = M.z;
         }
-        over = integrate(P, A, i, F, M, Mr);
+        over = integrate(P, A, i, pi, va, vb, F, M, Mr);
     }
     request_rebuild(A, over);
 }
@@ -995,16 +1009,176 @@ void launch_step_fused(const DevInst &P, const StepArgs &A, bool wet, bool mixed
         hipLaunchKernelGGL((k_step_fused<false, false>), dim3(nb), dim3(256), 0, s, P, A);
 }
 
+// ---------------------------------------------------------------------
+// Dry fused step with contact compaction.
+// A block owns 256 consecutive particles.  Phase A: each thread tests the
+// overlap of all its neighbour slots (cheap) and records the contacting
+// slots in a bitmask.  Phase B: a block scan packs every (particle, slot)
+// contact into an LDS list, particle by particle in slot order.  Phase C:
+// all 256 lanes take list entries in rounds and evaluate the contact law
+// (the expensive, branchy part) with every lane busy, writing the pair's
+// force, moment and rolling term to LDS; after each round the owner of each
+// particle adds its entries in list order, i.e. gravity first, then its
+// contacts in slot order, then its walls: the summation order of
+// kernel.cu:439-498, so the FP64 path stays bit-identical to the one-thread
+// loop.  Requires K <= 64 (slot bitmask).
+// ---------------------------------------------------------------------
+#define CP_BS 256
+
+template <bool MIXED>
+DEV bool pair_geometry(const double4 pi, const double4 pj, V3 &nv, double &gap) {
+    const V3 dv = v3(pi.x - pj.x, pi.y - pj.y, pi.z - pj.z);
+    if (MIXED) {
+        const double rs = pi.w + pj.w;
+        if (!(dot(dv, dv) < rs * rs)) return false;
+        const double l = len(dv);
+        nv = mulv(dv, 1.0 / l);
+        gap = l - pi.w - pj.w;
+    } else {
+        const double l = len(dv);
+        gap = l - pi.w - pj.w;
+        if (!(gap < 0.0)) return false;
+        nv = divv(dv, l);
+    }
+    return gap < 0.0;
+}
+
+template <bool MIXED>
+__global__ void __launch_bounds__(CP_BS) k_step_compact(DevInst P, StepArgs A) {
+    if (__hip_atomic_load(A.stop_at, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <= A.step) return;
+    extern __shared__ double smem_d[];
+    double *res = smem_d;                                     // [7][CP_BS]
+    unsigned short *list = (unsigned short *)(smem_d + 7 * CP_BS);
+    __shared__ unsigned scan_lds[4];
+    __shared__ unsigned total_s;
+
+    const unsigned nb = gridDim.x;
+    const int base = (int)(xcd_block(nb) * CP_BS);
+    const int tid = threadIdx.x;
+    const int i = base + tid;
+    const bool own = i < A.total;
+
+    // phase A: own record and the contact bitmask
+    double4 pi = make_double4(0.0, 0.0, 0.0, 0.0), va = pi, vb = pi;
+    unsigned long long mask = 0ull;
+    if (own) {
+        pi = A.in.p[i];
+        va = A.in.v[2 * i];
+        vb = A.in.v[2 * i + 1];
+        const int n = A.nnb[i];
+        for (int k = 0; k < n; ++k) {
+            const int j = A.nbr[(long)k * A.cap + i];
+            V3 nv;
+            double gap;
+            if (pair_geometry<MIXED>(pi, A.in.p[j], nv, gap)) mask |= 1ull << k;
+        }
+    }
+    // phase B: pack (particle, slot) pairs in slot order
+    const unsigned cnt = (unsigned)__popcll(mask);
+    unsigned tot;
+    const unsigned off = block_excl_scan(cnt, &tot, scan_lds);
+    {
+        unsigned long long m = mask;
+        unsigned q = off;
+        while (m) {
+            const int k = __ffsll((long long)m) - 1;
+            m &= m - 1ull;
+            list[q++] = (unsigned short)((tid << 8) | k);
+        }
+    }
+    if (tid == 0) total_s = tot;
+    __syncthreads();
+    const unsigned T = total_s;
+
+    double Fx = 0.0, Fy = 0.0, Fz = 0.0, Mx = 0.0, My = 0.0, Mz = 0.0, Mr = 0.0;
+    if (own) {
+        const double di = 2.0 * pi.w;
+        Fz = Fz - di * di * di;  // gravity first (kernel.cu:439)
+    }
+    // phase C: evaluate the packed contacts in rounds of CP_BS
+    for (unsigned r0 = 0; r0 < T; r0 += CP_BS) {
+        const unsigned t = r0 + tid;
+        if (t < T) {
+            const unsigned e = list[t];
+            const int il = (int)(e >> 8), k = (int)(e & 0xff);
+            const int ii = base + il;
+            const long sk = (long)k * A.cap + ii;
+            const int j = A.nbr[sk];
+            const double4 qi = A.in.p[ii], qj = A.in.p[j];
+            V3 nv;
+            double gap;
+            pair_geometry<MIXED>(qi, qj, nv, gap);
+            const V3 iv = vel_of(A.in.v, ii), iw = spin_of(A.in.v, ii);
+            const V3 jv = vel_of(A.in.v, j), jw = spin_of(A.in.v, j);
+            HistRec h = A.h[sk];
+            const double ri = qi.w, rj = qj.w;
+            const double R = ri * rj / (ri + rj);
+            const V3 cpv = sub(sub(iv, jv), cross(add(mulv(iw, ri), mulv(jw, rj)), nv));
+            V3 dF = v3(0.0, 0.0, 0.0), dM = v3(0.0, 0.0, 0.0);
+            double dMr = 0.0;
+            if (MIXED)
+                contact_mixed<true>(P.pp, P.rolling, P.dt, h, -gap, nv, R, ri, cpv, dF, dM, dMr);
+            else
+                contact<true>(P.pp, P.rolling, P.dt, h, -gap, nv, R, ri, cpv, dF, dM, dMr);
+            A.h[sk] = h;
+            res[0 * CP_BS + tid] = dF.x; res[1 * CP_BS + tid] = dF.y; res[2 * CP_BS + tid] = dF.z;
+            res[3 * CP_BS + tid] = dM.x; res[4 * CP_BS + tid] = dM.y; res[5 * CP_BS + tid] = dM.z;
+            res[6 * CP_BS + tid] = dMr;
+        }
+        __syncthreads();
+        if (cnt) {
+            unsigned a = off > r0 ? off : r0;
+            unsigned b = off + cnt < r0 + CP_BS ? off + cnt : r0 + CP_BS;
+            for (unsigned q = a; q < b; ++q) {
+                const unsigned s = q - r0;
+                Fx = Fx + res[0 * CP_BS + s]; Fy = Fy + res[1 * CP_BS + s];
+                Fz = Fz + res[2 * CP_BS + s];
+                Mx = Mx + res[3 * CP_BS + s]; My = My + res[4 * CP_BS + s];
+                Mz = Mz + res[5 * CP_BS + s];
+                Mr = Mr + res[6 * CP_BS + s];
+            }
+        }
+        __syncthreads();
+    }
+    // phase D: walls and integration by the owner
+    bool over = false;
+    if (own) {
+        V3 F = v3(Fx, Fy, Fz), M = v3(Mx, My, Mz);
+        const V3 iv = v3(va.x, va.y, va.z), iw = v3(va.w, vb.x, vb.y);
+        pw_particle<MIXED>(P, A, i, pi, iv, iw, F, M, Mr);
+        if (A.store_forces) {
+            A.Fx[i] = F.x; A.Fy[i] = F.y; A.Fz[i] = F.z;
+            A.Mx[i] = M.x; A.My[i] = M.y; A.Mz[i] = M.z;
+        }
+        over = integrate(P, A, i, pi, va, vb, F, M, Mr);
+    }
+    request_rebuild(A, over);
+}
+
+size_t compact_lds_bytes(int K) { return (size_t)7 * CP_BS * sizeof(double) + (size_t)CP_BS * K * 2; }
+
+void launch_step_compact(const DevInst &P, const StepArgs &A, int K, bool mixed, hipStream_t s) {
+    if (A.total <= 0) return;
+    unsigned nb = blocks_for(A.total, CP_BS);
+    size_t lds = compact_lds_bytes(K);
+    if (mixed)
+        hipLaunchKernelGGL((k_step_compact<true>), dim3(nb), dim3(CP_BS), lds, s, P, A);
+    else
+        hipLaunchKernelGGL((k_step_compact<false>), dim3(nb), dim3(CP_BS), lds, s, P, A);
+}
+
 // --- the reference's separate kernels (PpInterCalc, CirPWCalc,
 // UpdateSystem), accumulating into the force arrays -----------------------
 template <bool WET, bool MIXED>
 __global__ void __launch_bounds__(256) k_pp_force(DevInst P, StepArgs A) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= A.total) return;
+    const double4 pi = A.in.p[i];
+    const V3 iv = vel_of(A.in.v, i), iw = spin_of(A.in.v, i);
     V3 F = v3(A.Fx[i], A.Fy[i], A.Fz[i]), M = v3(A.Mx[i], A.My[i], A.Mz[i]);
     double Mr = A.Mr[i];
     int nbridge = 0;
-    pp_particle<WET, MIXED>(P, A, i, F, M, Mr, nbridge);
+    pp_particle<WET, MIXED>(P, A, i, pi, iv, iw, F, M, Mr, nbridge);
     if (WET) A.out.lqd[i] = A.out.lqd[i] + nbridge;
     A.Fx[i] = F.x; A.Fy[i] = F.y; A.Fz[i] = F.z;
     A.Mx[i] = M.x; A.My[i] = M.y; A.Mz[i] = M.z;
@@ -1015,9 +1189,11 @@ template <bool MIXED>
 __global__ void __launch_bounds__(256) k_pw_force(DevInst P, StepArgs A) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= A.total) return;
+    const double4 pi = A.in.p[i];
+    const V3 iv = vel_of(A.in.v, i), iw = spin_of(A.in.v, i);
     V3 F = v3(A.Fx[i], A.Fy[i], A.Fz[i]), M = v3(A.Mx[i], A.My[i], A.Mz[i]);
     double Mr = A.Mr[i];
-    pw_particle<MIXED>(P, A, i, F, M, Mr);
+    pw_particle<MIXED>(P, A, i, pi, iv, iw, F, M, Mr);
     A.Fx[i] = F.x; A.Fy[i] = F.y; A.Fz[i] = F.z;
     A.Mx[i] = M.x; A.My[i] = M.y; A.Mz[i] = M.z;
     A.Mr[i] = Mr;
@@ -1027,8 +1203,10 @@ __global__ void __launch_bounds__(256) k_update(DevInst P, StepArgs A, int *coun
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     bool over = false;
     if (i < A.total) {
+        const double4 pi = A.in.p[i];
+        const double4 va = A.in.v[2 * i], vb = A.in.v[2 * i + 1];
         V3 F = v3(A.Fx[i], A.Fy[i], A.Fz[i]), M = v3(A.Mx[i], A.My[i], A.Mz[i]);
-        over = integrate(P, A, i, F, M, A.Mr[i]);
+        over = integrate(P, A, i, pi, va, vb, F, M, A.Mr[i]);
     }
     unsigned long long b = __ballot(over);
     if (b != 0ull && (threadIdx.x & 63) == __ffsll((long long)b) - 1)
@@ -1057,6 +1235,26 @@ void launch_update(const DevInst &P, const StepArgs &A, int *count, hipStream_t 
 // ---------------------------------------------------------------------
 // import / export (original particle order)
 // ---------------------------------------------------------------------
+// what: 0 position, 1 velocity, 2 spin, 3 radius (one value)
+__global__ void k_gather_state(const int *id, StateBuf s, int what, double *out, int total) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    const long o = (long)id[i];
+    if (what == 0) {
+        const double4 q = s.p[i];
+        out[3 * o] = q.x; out[3 * o + 1] = q.y; out[3 * o + 2] = q.z;
+    } else if (what == 1) {
+        const double4 a = s.v[2 * i];
+        out[3 * o] = a.x; out[3 * o + 1] = a.y; out[3 * o + 2] = a.z;
+    } else if (what == 2) {
+        const double4 a = s.v[2 * i], b = s.v[2 * i + 1];
+        out[3 * o] = a.w; out[3 * o + 1] = b.x; out[3 * o + 2] = b.y;
+    } else if (what == 3) {
+        out[o] = s.p[i].w;
+    } else {
+        out[o] = s.v[2 * i + 1].z;
+    }
+}
 __global__ void k_gather3(const int *id, const double *a, const double *b, const double *c,
                           double *out, int total) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1074,10 +1272,33 @@ __global__ void k_gather1i(const int *id, const int *a, int *out, int total) {
     if (i >= total) return;
     out[id[i]] = a[i];
 }
-__global__ void k_scatter3(const double *in, double *a, double *b, double *c, int n, int off) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    a[off + i] = in[3 * (long)i]; b[off + i] = in[3 * (long)i + 1]; c[off + i] = in[3 * (long)i + 2];
+// particles off..off+n-1 from host AoS arrays (vel/angv may be null)
+__global__ void k_scatter_state(const double *pos, const double *vel, const double *angv,
+                                const double *rad, StateBuf s, int n, int off) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int i = off + k;
+    double4 q;
+    q.x = pos[3 * (long)k]; q.y = pos[3 * (long)k + 1]; q.z = pos[3 * (long)k + 2];
+    q.w = rad[k];
+    s.p[i] = q;
+    double4 a, b;
+    a.x = vel ? vel[3 * (long)k] : 0.0;
+    a.y = vel ? vel[3 * (long)k + 1] : 0.0;
+    a.z = vel ? vel[3 * (long)k + 2] : 0.0;
+    a.w = angv ? angv[3 * (long)k] : 0.0;
+    b.x = angv ? angv[3 * (long)k + 1] : 0.0;
+    b.y = angv ? angv[3 * (long)k + 2] : 0.0;
+    b.z = 0.0;
+    b.w = 0.0;
+    s.v[2 * i] = a;
+    s.v[2 * i + 1] = b;
+}
+void launch_gather_state(const int *id, const StateBuf &s, int what, double *out, int total,
+                         hipStream_t st) {
+    if (total > 0)
+        hipLaunchKernelGGL(k_gather_state, dim3(blocks_for(total, 256)), dim3(256), 0, st, id, s,
+                           what, out, total);
 }
 void launch_gather3(const int *id, const double *a, const double *b, const double *c, double *out,
                     int total, hipStream_t s) {
@@ -1095,11 +1316,11 @@ void launch_gather1i(const int *id, const int *a, int *out, int total, hipStream
         hipLaunchKernelGGL(k_gather1i, dim3(blocks_for(total, 256)), dim3(256), 0, s, id, a, out,
                            total);
 }
-void launch_scatter3(const double *in, double *a, double *b, double *c, int n, int off,
-                     hipStream_t s) {
+void launch_scatter_state(const double *pos, const double *vel, const double *angv,
+                          const double *rad, const StateBuf &s, int n, int off, hipStream_t st) {
     if (n > 0)
-        hipLaunchKernelGGL(k_scatter3, dim3(blocks_for(n, 256)), dim3(256), 0, s, in, a, b, c, n,
-                           off);
+        hipLaunchKernelGGL(k_scatter_state, dim3(blocks_for(n, 256)), dim3(256), 0, st, pos, vel,
+                           angv, rad, s, n, off);
 }
 
 // ---------------------------------------------------------------------
@@ -1128,24 +1349,26 @@ __global__ void __launch_bounds__(256) k_stats(DevInst P, StepArgs A, unsigned l
     unsigned long long nn = 0, nc = 0, nw = 0, nwc = 0, nbr = 0;
     if (i < A.total) {
         const StateBuf &s = A.in;
+        const double4 pi = s.p[i];
         nn = A.nnb[i];
         for (int k = 0; k < (int)nn; ++k) {
             long sk = (long)k * A.cap + i;
             int j = A.nbr[sk];
-            double dx = s.x[i] - s.x[j], dy = s.y[i] - s.y[j], dz = s.z[i] - s.z[j];
-            double gap = sqrt(dx * dx + dy * dy + dz * dz) - A.rad[i] - A.rad[j];
+            const double4 pj = s.p[j];
+            double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+            double gap = sqrt(dx * dx + dy * dy + dz * dz) - pi.w - pj.w;
             if (gap < 0.0) nc++;
-            if ((A.h.flags[sk] >> 8) & 1) nbr++;
+            if ((A.h[sk].flags >> 8) & 1) nbr++;
         }
         nw = A.nwl[i];
         for (int k = 0; k < (int)nw; ++k) {
             int w = A.wl[(long)k * A.cap + i];
             double gap;
-            if (w == 0) gap = (A.Tw - s.z[i]) - A.rad[i];
-            else if (w == 1) gap = (s.z[i] - P.Bw) - A.rad[i];
+            if (w == 0) gap = (A.Tw - pi.z) - pi.w;
+            else if (w == 1) gap = (pi.z - P.Bw) - pi.w;
             else {
-                double ax = P.cirX - s.x[i], ay = P.cirY - s.y[i];
-                gap = (P.R - sqrt(ax * ax + ay * ay)) - A.rad[i];
+                double ax = P.cirX - pi.x, ay = P.cirY - pi.y;
+                gap = (P.R - sqrt(ax * ax + ay * ay)) - pi.w;
             }
             if (gap < 0.0) nwc++;
         }

@@ -15,32 +15,36 @@ void launch_radix_sort(unsigned *k, unsigned *v, unsigned *k2, unsigned *v2, lon
 
 void launch_calc_hash(const StateBuf &s, const int *level, const int *id, const DevInst &P,
                       unsigned *keys, unsigned *vals, int total, hipStream_t st);
-void launch_permute(const unsigned *perm, const StateBuf &a, const StateBuf &b,
-                    const double *rad_a, double *rad_b, const int *lvl_a, int *lvl_b,
-                    const int *id_a, int *id_b, double *pmove, int total, hipStream_t s);
-void launch_cell_range(const unsigned *keys, int *cstart, int *cend, int ncells, int total,
-                       hipStream_t s);
-void launch_neighbor_find(const StateBuf &s, const double *rad, const int *level,
-                          const DevInst &P, const int *cstart, const int *cend, int *nbr, int *nnb,
-                          int *overflow, int cap, int total, hipStream_t st);
-void launch_wall_list(const StateBuf &s, const double *rad, const DevInst &P, double Tw, int *wl,
-                      int *nwl, int cap, int total, hipStream_t st);
+void launch_permute(const unsigned *perm, const StateBuf &a, const StateBuf &b, const int *lvl_a,
+                    int *lvl_b, const int *id_a, int *id_b, int total, hipStream_t s);
+void launch_cell_range(const unsigned *keys, const StateBuf &s, const int *level, const DevInst &P,
+                       int *cstart, int *cend, int ncells, int total, hipStream_t st);
+void launch_neighbor_find(const StateBuf &s, const int *level, const DevInst &P, const int *cstart,
+                          const int *cend, int *nbr, int *nnb, int *overflow, int cap, int total,
+                          hipStream_t st);
+void launch_wall_list(const StateBuf &s, const DevInst &P, double Tw, int *wl, int *nwl, int cap,
+                      int total, hipStream_t st);
 void launch_hist_remap(const unsigned *perm, const int *id_new, const int *id_old, const int *nbr,
-                       const int *nnb, const int *nbr_old, const int *nnb_old, HistPtr h,
-                       HistPtr h_old, const int *wl, const int *nwl, const int *wl_old,
-                       const int *nwl_old, HistPtr hw, HistPtr hw_old, int cap, int total,
+                       const int *nnb, const int *nbr_old, const int *nnb_old, HistRec *h,
+                       const HistRec *h_old, const int *wl, const int *nwl, const int *wl_old,
+                       const int *nwl_old, HistRec *hw, const HistRec *hw_old, int cap, int total,
                        int old_total, hipStream_t s);
 
 void launch_step_fused(const DevInst &P, const StepArgs &A, bool wet, bool mixed, hipStream_t s);
+// dry step with contact compaction; needs K <= 64
+void launch_step_compact(const DevInst &P, const StepArgs &A, int K, bool mixed, hipStream_t s);
 void launch_pp_force(const DevInst &P, const StepArgs &A, bool wet, bool mixed, hipStream_t s);
 void launch_pw_force(const DevInst &P, const StepArgs &A, bool mixed, hipStream_t s);
 void launch_update(const DevInst &P, const StepArgs &A, int *count, hipStream_t s);
 
+// what: 0 position, 1 velocity, 2 spin (AoS [id][3]), 3 radius, 4 path length
+void launch_gather_state(const int *id, const StateBuf &s, int what, double *out, int total,
+                         hipStream_t st);
 void launch_gather3(const int *id, const double *a, const double *b, const double *c, double *out,
                     int total, hipStream_t s);
 void launch_gather1(const int *id, const double *a, double *out, int total, hipStream_t s);
 void launch_gather1i(const int *id, const int *a, int *out, int total, hipStream_t s);
-void launch_scatter3(const double *in, double *a, double *b, double *c, int n, int off,
-                     hipStream_t s);
+void launch_scatter_state(const double *pos, const double *vel, const double *angv,
+                          const double *rad, const StateBuf &s, int n, int off, hipStream_t st);
 void launch_stats(const DevInst &P, const StepArgs &A, unsigned long long *out, hipStream_t s);
 void launch_stream_copy(const void *a, void *b, long nbytes, hipStream_t s);

@@ -212,6 +212,15 @@ class Engine:
     def sync(self):
         check(A.lib().dem_sync(self._ctx))
 
+    KERNEL_AUTO, KERNEL_LOOP, KERNEL_COMPACT = 0, 1, 2
+
+    def set_kernel(self, variant):
+        check(A.lib().dem_set_kernel(self._ctx, int(variant)))
+
+    def set_order(self, morton):
+        """Storage order between rebuilds: Morton (True) or row-major."""
+        check(A.lib().dem_set_order(self._ctx, int(bool(morton))))
+
     @property
     def iter(self):
         return A.lib().dem_iter(self._ctx)

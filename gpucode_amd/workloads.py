@@ -94,12 +94,37 @@ def wet_poly(n=200_000, seed=0xC2, R=25e-3, H=60e-3, steps=50_000, theta_deg=20.
                 steps=steps)
 
 
-def pocket_bed(n, R, d=1.0, a=1.10, jitter=0.03, seed=0, z0=0.0):
+def _trilaterate(p1, p2, p3, d):
+    """Centres at distance d from three centres (rows), on the upper side;
+    NaN where no such point exists."""
+    ex = p2 - p1
+    d12 = np.linalg.norm(ex, axis=1)
+    ex = ex / d12[:, None]
+    t = p3 - p1
+    i = (ex * t).sum(1)
+    ey = t - i[:, None] * ex
+    nj = np.linalg.norm(ey, axis=1)
+    ey = ey / nj[:, None]
+    ez = np.cross(ex, ey)
+    flip = ez[:, 2] < 0
+    ez[flip] = -ez[flip]
+    j = (ey * t).sum(1)
+    x = d12 / 2
+    y = (i * i + j * j) / (2 * j) - i * x / j
+    z2 = d * d - x * x - y * y
+    z = np.sqrt(np.where(z2 > 0, z2, np.nan))
+    return p1 + x[:, None] * ex + y[:, None] * ey + z[:, None] * ez
+
+
+def pocket_bed(n, R, d=1.0, a=1.10, jitter=0.03, seed=0, z0=0.0, pockets=True):
     """Dense bed of n spheres (diameter d, reduced units) in a cylinder of
-    radius R centred at the origin, floor at z0.  Layers are triangular
-    lattices of spacing a stacked into each other's pockets (ABC order);
-    every site is jittered in the plane and then lowered onto the highest of
-    the spheres below it, so each sphere rests on at least one support."""
+    radius R centred at the origin, floor at z0.  The floor layer is a
+    triangular lattice of spacing a with every site jittered in the plane.
+    Each further layer sits in the pockets of the one below (ABC order): a
+    sphere is placed touching the three spheres nearest to its lattice site
+    (pockets=True), so every sphere rests on three supports and the bed
+    starts with about six contacts per sphere; where three supports are not
+    available (at the wall) it is lowered onto the highest sphere below."""
     from scipy.spatial import cKDTree
 
     rng = np.random.default_rng(seed)
@@ -110,19 +135,20 @@ def pocket_bed(n, R, d=1.0, a=1.10, jitter=0.03, seed=0, z0=0.0):
     ii, jj = ii.ravel().astype(float), jj.ravel().astype(float)
     out = []
     total = 0
-    prev_xy = prev_z = tree = None
+    prev = tree = None
     L = 0
     h = math.sqrt(max(d * d - a * a / 3.0, 0.25 * d * d))
     while total < n:
         ox, oy = (L % 3) * a / 2, (L % 3) * a / (2 * s3)
         x = ii * a + jj * a / 2 + ox
         y = jj * a * s3 / 2 + oy
-        x = x + jitter * a * rng.uniform(-1, 1, x.shape)
-        y = y + jitter * a * rng.uniform(-1, 1, y.shape)
+        if L == 0 or not pockets:
+            x = x + jitter * a * rng.uniform(-1, 1, x.shape)
+            y = y + jitter * a * rng.uniform(-1, 1, y.shape)
         keep = x * x + y * y <= (R - r - 1e-9) ** 2
         x, y = x[keep], y[keep]
         if L == 0:
-            z = np.full(x.shape, z0 + r)
+            layer = np.stack([x, y, np.full(x.shape, z0 + r)], 1)
         else:
             xy = np.stack([x, y], 1)
             dist, idx = tree.query(xy, k=8, distance_upper_bound=d)
@@ -130,21 +156,42 @@ def pocket_bed(n, R, d=1.0, a=1.10, jitter=0.03, seed=0, z0=0.0):
             for c in range(dist.shape[1]):
                 ok = np.isfinite(dist[:, c])
                 cand = np.full(x.shape, -np.inf)
-                j = idx[ok, c]
+                jn = idx[ok, c]
                 rho = dist[ok, c]
-                cand[ok] = prev_z[j] + np.sqrt(np.maximum(d * d - rho * rho, 0.0))
+                cand[ok] = prev[jn, 2] + np.sqrt(np.maximum(d * d - rho * rho, 0.0))
                 z = np.maximum(z, cand)
             nominal = z0 + r + L * h
             z = np.where(np.isfinite(z), z, nominal)
-        layer = np.stack([x, y, z], 1)
+            layer = np.stack([x, y, z], 1)
+            if pockets:
+                ok3 = np.isfinite(dist[:, 2])
+                sel = np.where(ok3)[0]
+                if len(sel):
+                    q = _trilaterate(prev[idx[sel, 0]], prev[idx[sel, 1]], prev[idx[sel, 2]], d)
+                    good = np.isfinite(q).all(1)
+                    good &= q[:, 0] ** 2 + q[:, 1] ** 2 <= (R - r - 1e-9) ** 2
+                    # no overlap with the other spheres below (the three
+                    # supports sit at exactly d)
+                    if good.any():
+                        d4, _ = tree3.query(q[good], k=4)
+                        ok = d4[:, 3] >= d * (1 - 1e-12)
+                        gi = np.where(good)[0][ok]
+                        layer[sel[gi]] = q[gi]
+                # keep the layer free of in-plane overlaps: drop the later of
+                # any two spheres closer than d
+                lt = cKDTree(layer)
+                bad = np.zeros(len(layer), bool)
+                for a_, b_ in lt.query_pairs(d * (1 - 1e-12)):
+                    bad[max(a_, b_)] = True
+                layer = layer[~bad]
         if total + len(layer) > n:
             pick = np.sort(rng.choice(len(layer), n - total, replace=False))
             layer = layer[pick]
         out.append(layer)
         total += len(layer)
-        prev_xy = np.stack([x, y], 1)
-        prev_z = z
-        tree = cKDTree(prev_xy)
+        prev = layer
+        tree = cKDTree(layer[:, :2])
+        tree3 = cKDTree(layer)
         L += 1
     return np.concatenate(out)[:n]
 

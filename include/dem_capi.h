@@ -44,6 +44,11 @@ enum {
  * both modes. */
 enum { DEM_PREC_FP64 = 0, DEM_PREC_MIXED = 1 };
 
+/* Fused step kernel: AUTO picks the contact-compacting kernel for dry
+ * packings whose lists have at most 64 slots, the one-thread-per-particle
+ * loop otherwise.  Both give identical results. */
+enum { DEM_KERNEL_AUTO = 0, DEM_KERNEL_LOOP = 1, DEM_KERNEL_COMPACT = 2 };
+
 /* The heyi.in keywords (ReadData.cpp:75-188), SI units, plus overrides. */
 typedef struct dem_input {
     int NP;                 /* NUMPARTICLE (capacity) */
@@ -220,6 +225,13 @@ int dem_step(dem_ctx *ctx, int nsteps);
  * stream.  Raises DEM_ERR_STATE if a rebuild would have been needed. */
 int dem_time_steps(dem_ctx *ctx, int nsteps, double *ms_per_step, int *rebuild_needed);
 int dem_sync(dem_ctx *ctx);
+/* DEM_KERNEL_* selection for dem_step; COMPACT is refused for wet
+ * packings and lists longer than 64 slots. */
+int dem_set_kernel(dem_ctx *ctx, int variant);
+/* Storage order of the particles between rebuilds: 1 (default) = Morton
+ * order of the cells, 0 = the reference's row-major cell hash.  Lists,
+ * forces and the exported hash/cell tables are identical either way. */
+int dem_set_order(dem_ctx *ctx, int morton);
 int dem_iter(dem_ctx *ctx);
 double dem_ctime(dem_ctx *ctx);
 int dem_n_rebuild(dem_ctx *ctx);

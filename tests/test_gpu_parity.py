@@ -184,6 +184,56 @@ def test_fused_equals_separate_kernels():
     assert a.n_rebuild == b.n_rebuild
 
 
+@pytest.mark.parametrize("precision", [G.PREC_FP64, G.PREC_MIXED])
+def test_compacting_kernel_equals_loop_kernel(precision):
+    # the compacting kernel sums each particle's contacts in slot order, so
+    # it must reproduce the one-thread loop bit for bit, rebuilds included
+    w = W.small_bed(n=3000)
+    rng = np.random.default_rng(4)
+    vel = rng.normal(size=(3000, 3)) * 50.0  # about 5 m/s: a rebuild every ~150 steps
+    a = G.Engine(w["inp"], None, precision)
+    b = G.Engine(w["inp"], None, precision)
+    a.set_kernel(G.Engine.KERNEL_COMPACT)
+    b.set_kernel(G.Engine.KERNEL_LOOP)
+    for x in (a, b):
+        x.set_particles(w["pos"], vel)
+        x.store_forces(True)
+    a.step(400)
+    b.step(400)
+    assert a.n_rebuild == b.n_rebuild and a.n_rebuild > 1
+    for u, v in zip(a.particles(), b.particles()):
+        assert np.array_equal(u, v)
+    for u, v in zip(a.forces(), b.forces()):
+        assert np.array_equal(u, v)
+    ha, hb = a.hist_pp(), b.hist_pp()
+    for f in ha:
+        assert np.array_equal(ha[f], hb[f]), f
+
+
+def test_morton_storage_equals_row_major_storage():
+    # storage order changes memory locality only: lists (as original ids),
+    # the reference's hash/cell tables and the trajectory are identical
+    w = W.wet_bed(n=3000, poly=True, reference_law=False, theta_deg=20.0, liquid=0.2)
+    a = G.Engine(w["inp"], w["dia"])
+    b = G.Engine(w["inp"], w["dia"])
+    a.set_order(True)
+    b.set_order(False)
+    for x in (a, b):
+        x.set_particles(w["pos"])
+    a.rebuild()
+    b.rebuild()
+    assert np.array_equal(a.hash_index()[0], b.hash_index()[0])
+    assert np.array_equal(a.hash_index()[1], b.hash_index()[1])
+    for u, v in zip(a.cells(), b.cells()):
+        assert np.array_equal(u, v)
+    a.step(200)
+    b.step(200)
+    for u, v in zip(a.particles(), b.particles()):
+        assert np.array_equal(u, v)
+    for u, v in zip(a.nebor(), b.nebor()):
+        assert np.array_equal(u, v)
+
+
 def test_dry_column_20000_steps():
     w = W.dry_column()
     e, o = pair(w)
