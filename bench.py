@@ -29,17 +29,18 @@ def log(msg):
 def algorithmic_bytes(st, n):
     """Compulsory HBM bytes per particle-step of the fused step kernel:
     read x,v,omega (72) + r (8), write x,v,omega (72), path length r/w (16),
-    list counts (8), 4 B per neighbour index, and 60 B read + 60 B write of
-    FP64 history (dispt 24, nDsp/nDsp_mx/plstRad/plstDfm 32, flags 4) per
-    contacting p-p or wall slot."""
+    list counts (8), 4 B per neighbour index, and 64 B read + 64 B write of
+    contact history (dispt 24, nDsp/nDsp_mx/plstRad/plstDfm 32, stage 4,
+    lqdBrg 4) per contacting p-p or wall slot.  Neighbour state re-reads
+    count as cache hits; forces are never stored."""
     n_nb = st["n_nebor"] / n
     n_c = st["n_contact"] / n
     n_w = st["n_wall_contact"] / n
-    return 176.0 + 4.0 * n_nb + 120.0 * (n_c + n_w), n_nb, n_c, n_w
+    return 176.0 + 4.0 * n_nb + 128.0 * (n_c + n_w), n_nb, n_c, n_w
 
 
-def load_traffic():
-    path = os.path.join(ROOT, "profiles", "traffic_fused_step_packed_bed_2M.json")
+def load_traffic(kernel):
+    path = os.path.join(ROOT, "profiles", f"traffic_{kernel}_packed_bed_2M.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
@@ -71,8 +72,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=300)
     ap.add_argument("--n", type=int, default=2_000_000)
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
-    ap.add_argument("--kernel", choices=["auto", "loop"], default="auto",
-                    help="fused step kernel: contact-compacting (auto) or one-thread loop")
+    ap.add_argument("--kernel", choices=["auto", "wave", "compact", "loop"], default="auto",
+                    help="fused step kernel: per-wave contact compaction (auto, wave), "
+                         "block-wide compaction (compact) or the one-thread loop (loop)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -89,7 +91,8 @@ def main():
     log(f"bed of {args.n} spheres built in {time.time() - t0:.1f}s")
     prec = G.PREC_FP64 if args.precision == "fp64" else G.PREC_MIXED
     e = G.Engine(w["inp"], None, prec)
-    e.set_kernel(G.Engine.KERNEL_AUTO if args.kernel == "auto" else G.Engine.KERNEL_LOOP)
+    e.set_kernel({"auto": G.Engine.KERNEL_AUTO, "wave": G.Engine.KERNEL_WAVE,
+                  "compact": G.Engine.KERNEL_COMPACT, "loop": G.Engine.KERNEL_LOOP}[args.kernel])
     e.set_particles(w["pos"])
     log(f"engine up: {e.device_bytes() / 2**30:.2f} GiB on device")
     t0 = time.time()
@@ -113,13 +116,14 @@ def main():
     launches = tm1["fused_launches"] - tm0["fused_launches"]
     kern_ms = (tm1["fused_ms"] - tm0["fused_ms"]) / max(launches, 1)
     achieved = B * n / (kern_ms * 1e-3) / 1e9
+    kname = e.step_kernel_name()
     traffic = None
-    tr = load_traffic()
+    tr = load_traffic(kname)
     if tr:
         traffic = tr.get("bytes_per_launch")
 
     copy_gbs = G.engine.stream_copy(1 << 30, 20)
-    log(f"plain copy {copy_gbs:.0f} GB/s; fused kernel {kern_ms:.4f} ms, {achieved:.0f} GB/s "
+    log(f"plain copy {copy_gbs:.0f} GB/s; {kname} {kern_ms:.4f} ms, {achieved:.0f} GB/s "
         f"algorithmic ({B:.1f} B/particle-step)")
 
     base = None
@@ -153,7 +157,7 @@ def main():
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-            "kernel": "k_step_fused", "kernel_ms": kern_ms, "bytes_per_particle": B,
+            "kernel": kname, "kernel_ms": kern_ms, "bytes_per_particle": B,
             "attainable_copy_gbs": copy_gbs,
         },
         "cpu_baseline": base,

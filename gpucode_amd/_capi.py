@@ -80,6 +80,7 @@ _LIB = None
 P = C.c_void_p
 D = C.POINTER(C.c_double)
 I = C.POINTER(C.c_int)
+F = C.POINTER(C.c_float)
 
 _SIGS = {
     "dem_last_error": (C.c_char_p, []),
@@ -128,6 +129,7 @@ _SIGS = {
     "dem_contact_stats": (C.c_int, [P, C.POINTER(DemStats)]),
     "dem_get_timing": (C.c_int, [P, D, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong), D, I]),
     "dem_device_bytes": (C.c_longlong, [P]),
+    "dem_step_kernel_name": (C.c_char_p, [P]),
     "dem_stream_copy": (C.c_int, [C.c_longlong, C.c_int, D]),
     "dem_read_deck": (C.c_int, [C.c_char_p, C.POINTER(DemInput)]),
     "dem_particle_feed": (C.c_int, [C.POINTER(DemSetup), D, D, C.c_int, C.c_int,
@@ -138,6 +140,25 @@ _SIGS = {
     "dem_dump_read": (C.c_int, [P, C.c_char_p, I]),
     "dem_set_lqdcnt": (C.c_int, [P, I]),
     "dem_set_clock": (C.c_int, [P, C.c_int, C.c_double, C.c_double]),
+    # dem_batch.h
+    "dem_batch_create": (C.c_int, [C.c_int, C.POINTER(DemInput), C.POINTER(D), C.c_int,
+                                   C.POINTER(P)]),
+    "dem_batch_destroy": (C.c_int, [P]),
+    "dem_batch_size": (C.c_int, [P]),
+    "dem_batch_instance": (P, [P, C.c_int]),
+    "dem_batch_step": (C.c_int, [P, C.c_int]),
+    "dem_batch_timing": (C.c_int, [P, D, C.POINTER(C.c_longlong), D, C.POINTER(C.c_longlong)]),
+    "dem_batch_analyze": (C.c_int, [P, D, D, D]),
+    "dem_analyze": (C.c_int, [P, D, D, D]),
+    # dem_surrogate.h
+    "sur_nparams": (C.c_int, []),
+    "sur_create": (C.c_int, [C.c_ulonglong, C.POINTER(P)]),
+    "sur_destroy": (C.c_int, [P]),
+    "sur_get_params": (C.c_int, [P, F]),
+    "sur_set_params": (C.c_int, [P, F]),
+    "sur_forward": (C.c_int, [P, C.c_int, F, F]),
+    "sur_loss_grad": (C.c_int, [P, C.c_int, F, F, F, F]),
+    "sur_train": (C.c_int, [P, C.c_int, F, F, C.c_int, C.c_int, C.c_float, C.c_ulonglong, F]),
 }
 
 
@@ -173,6 +194,13 @@ def dptr(a):
         return None
     assert a.dtype == np.float64 and a.flags.c_contiguous
     return a.ctypes.data_as(D)
+
+
+def fptr(a):
+    if a is None:
+        return None
+    assert a.dtype == np.float32 and a.flags.c_contiguous
+    return a.ctypes.data_as(F)
 
 
 def iptr(a):

@@ -23,6 +23,8 @@ SOURCES = [
     "ParticleFeed.cpp",
     "WriteTec.cpp",
     "DumpLoad.cpp",
+    "dem_batch.cpp",
+    "surrogate.hip",
 ]
 EXTRA_SOURCES = []  # appended by later modules (batched engine, surrogate)
 

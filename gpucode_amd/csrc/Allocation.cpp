@@ -195,10 +195,12 @@ static void free_all(dem_ctx *c) {
     (void)hipFree(c->stats);
 }
 
-extern "C" int dem_create(const dem_input *in, const double *dia_m, int precision, dem_ctx **out) {
+int dem_create_on(const dem_input *in, const double *dia_m, int precision, hipStream_t stream,
+                  dem_ctx **out) {
     if (!in || !out) return dem_fail(DEM_ERR_ARG, "null argument");
     if (precision != DEM_PREC_FP64 && precision != DEM_PREC_MIXED)
         return dem_fail(DEM_ERR_ARG, "unknown precision %d", precision);
+    if (in->NP < 1) return dem_fail(DEM_ERR_ARG, "NP must be positive");
     *out = nullptr;
     dem_ctx *c = new dem_ctx();
     c->in = *in;
@@ -215,24 +217,23 @@ extern "C" int dem_create(const dem_input *in, const double *dia_m, int precisio
     for (int l = 0; l < c->S.nLevels; ++l) c->K = c->S.levelNNebors[l] > c->K ? c->S.levelNNebors[l] : c->K;
     dem_refresh_inst(c);
     c->Tw = c->S.Tw;
-    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
-    if (e != hipSuccess) {
-        delete c;
-        return dem_fail(DEM_ERR_HIP, "hipStreamCreate: %s", hipGetErrorString(e));
+    if (stream) {
+        c->stream = stream;
+        c->own_stream = false;
+    } else {
+        hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+        if (e != hipSuccess) {
+            delete c;
+            return dem_fail(DEM_ERR_HIP, "hipStreamCreate: %s", hipGetErrorString(e));
+        }
     }
-    if ((rc = alloc_all(c)) != DEM_OK) {
-        free_all(c);
-        (void)hipStreamDestroy(c->stream);
-        delete c;
+    if ((rc = alloc_all(c)) != DEM_OK ||
+        (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess)) {
+        if (rc == DEM_OK) rc = dem_fail(DEM_ERR_HIP, "hipEventCreate failed");
+        dem_destroy(c);
         return rc;
     }
-    if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
-        free_all(c);
-        (void)hipStreamDestroy(c->stream);
-        delete c;
-        return dem_fail(DEM_ERR_HIP, "hipEventCreate failed");
-    }
-    e = hipStreamSynchronize(c->stream);
+    hipError_t e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) {
         dem_destroy(c);
         return dem_fail(DEM_ERR_HIP, "allocation sync: %s", hipGetErrorString(e));
@@ -241,13 +242,17 @@ extern "C" int dem_create(const dem_input *in, const double *dia_m, int precisio
     return DEM_OK;
 }
 
+extern "C" int dem_create(const dem_input *in, const double *dia_m, int precision, dem_ctx **out) {
+    return dem_create_on(in, dia_m, precision, nullptr, out);
+}
+
 extern "C" int dem_destroy(dem_ctx *c) {
     if (!c) return DEM_OK;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     free_all(c);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
-    if (c->stream) (void)hipStreamDestroy(c->stream);
+    if (c->stream && c->own_stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return DEM_OK;
 }

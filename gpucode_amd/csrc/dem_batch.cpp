@@ -1,0 +1,340 @@
+// dem_batch.cpp -- batched stepping of many instances and packing analysis.
+//
+// The reference runs one packing per process: its constants are process
+// globals (kernel.cu:15-19, heydem.h:196-204) and main.cpp:425-623 steps
+// that one packing.  A batch holds n ordinary contexts on one stream.  Per
+// chunk of steps it uploads one table entry per instance (buffers, lists,
+// step limit, parities) and launches one step kernel per step for all dry
+// instances (per-wave compaction) and one for all others (one thread per
+// particle); the blocks of each launch are tasks (instance, block) listed
+// largest instance first.  Rebuild requests stay per instance: an instance
+// whose particles passed the path-length threshold skips the rest of the
+// chunk and is rebuilt before the next one, exactly as dem_step does.
+#include <limits.h>
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "../../include/dem_batch.h"
+#include "dem_host.h"
+#include "dem_launch.h"
+
+static const int kBatchChunk = 64;
+
+struct dem_batch {
+    std::vector<dem_ctx *> inst;
+    hipStream_t stream = nullptr;
+    int precision = DEM_PREC_FP64;
+    DevInst *d_inst = nullptr;
+    BatchInst *d_tab = nullptr, *h_tab = nullptr;  // h_tab pinned
+    AnaArgs *d_ana = nullptr, *h_ana = nullptr;    // h_ana pinned
+    double *d_out = nullptr, *h_out = nullptr;     // analysis results, h_out pinned
+    int *d_stop = nullptr, *h_stop = nullptr;      // h_stop pinned
+    int *h_ovf = nullptr;                          // pinned
+    int2 *d_tasks = nullptr;
+    int ntask_wave = 0, ntask_loop = 0, task_cap = 0;
+    std::vector<int> task_totals;                  // totals the task table was built for
+    std::vector<int> task_kernel;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
+    double step_ms = 0.0, rebuild_ms = 0.0;
+    long long launches = 0, n_rebuild = 0;
+};
+
+static void batch_free(dem_batch *b) {
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    for (dem_ctx *c : b->inst) dem_destroy(c);
+    (void)hipFree(b->d_inst);
+    (void)hipFree(b->d_tab);
+    (void)hipFree(b->d_ana);
+    (void)hipFree(b->d_out);
+    (void)hipFree(b->d_stop);
+    (void)hipFree(b->d_tasks);
+    (void)hipHostFree(b->h_tab);
+    (void)hipHostFree(b->h_ana);
+    (void)hipHostFree(b->h_out);
+    (void)hipHostFree(b->h_stop);
+    (void)hipHostFree(b->h_ovf);
+    if (b->ev0) (void)hipEventDestroy(b->ev0);
+    if (b->ev1) (void)hipEventDestroy(b->ev1);
+    if (b->ev2) (void)hipEventDestroy(b->ev2);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+}
+
+extern "C" int dem_batch_create(int n, const dem_input *in, const double *const *dia, int precision,
+                                dem_batch **out) {
+    if (n < 1 || !in || !out) return dem_fail(DEM_ERR_ARG, "bad argument");
+    *out = nullptr;
+    dem_batch *b = new dem_batch();
+    b->precision = precision;
+    int rc = DEM_OK;
+    hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete b;
+        return dem_fail(DEM_ERR_HIP, "hipStreamCreate: %s", hipGetErrorString(e));
+    }
+    for (int k = 0; k < n && rc == DEM_OK; ++k) {
+        dem_ctx *c = nullptr;
+        rc = dem_create_on(&in[k], dia ? dia[k] : nullptr, precision, b->stream, &c);
+        if (rc == DEM_OK) b->inst.push_back(c);
+    }
+    const size_t un = (size_t)n;
+    if (rc == DEM_OK) {
+        e = hipMalloc((void **)&b->d_inst, un * sizeof(DevInst));
+        if (e == hipSuccess) e = hipMalloc((void **)&b->d_tab, un * sizeof(BatchInst));
+        if (e == hipSuccess) e = hipMalloc((void **)&b->d_ana, un * sizeof(AnaArgs));
+        if (e == hipSuccess) e = hipMalloc((void **)&b->d_out, un * 3 * sizeof(double));
+        if (e == hipSuccess) e = hipMalloc((void **)&b->d_stop, un * sizeof(int));
+        if (e == hipSuccess) e = hipHostMalloc((void **)&b->h_tab, un * sizeof(BatchInst), 0);
+        if (e == hipSuccess) e = hipHostMalloc((void **)&b->h_ana, un * sizeof(AnaArgs), 0);
+        if (e == hipSuccess) e = hipHostMalloc((void **)&b->h_out, un * 3 * sizeof(double), 0);
+        if (e == hipSuccess) e = hipHostMalloc((void **)&b->h_stop, un * sizeof(int), 0);
+        if (e == hipSuccess) e = hipHostMalloc((void **)&b->h_ovf, un * sizeof(int), 0);
+        if (e == hipSuccess) e = hipEventCreate(&b->ev0);
+        if (e == hipSuccess) e = hipEventCreate(&b->ev1);
+        if (e == hipSuccess) e = hipEventCreate(&b->ev2);
+        if (e != hipSuccess) rc = dem_fail(DEM_ERR_HIP, "batch allocation: %s", hipGetErrorString(e));
+    }
+    if (rc != DEM_OK) {
+        batch_free(b);
+        return rc;
+    }
+    *out = b;
+    return DEM_OK;
+}
+
+extern "C" int dem_batch_destroy(dem_batch *b) {
+    if (b) batch_free(b);
+    return DEM_OK;
+}
+
+extern "C" int dem_batch_size(dem_batch *b) { return b ? (int)b->inst.size() : 0; }
+
+extern "C" dem_ctx *dem_batch_instance(dem_batch *b, int k) {
+    if (!b || k < 0 || k >= (int)b->inst.size()) return nullptr;
+    return b->inst[k];
+}
+
+// constants of every instance (they change with dem_set_material & co.)
+static int upload_insts(dem_batch *b) {
+    std::vector<DevInst> h(b->inst.size());
+    for (size_t k = 0; k < b->inst.size(); ++k) h[k] = b->inst[k]->P;
+    HIPCHK(hipMemcpyAsync(b->d_inst, h.data(), h.size() * sizeof(DevInst), hipMemcpyHostToDevice,
+                          b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));  // h goes out of scope
+    return DEM_OK;
+}
+
+// task table: (instance, block) for the wave-kernel instances, then for the
+// loop-kernel instances; largest instance first within each group
+static int build_tasks(dem_batch *b) {
+    const int n = (int)b->inst.size();
+    std::vector<int> tot(n), kern(n);
+    for (int k = 0; k < n; ++k) {
+        tot[k] = b->inst[k]->total;
+        kern[k] = dem_step_kernel(b->inst[k]) == DEM_KERNEL_LOOP ? DEM_KERNEL_LOOP : DEM_KERNEL_WAVE;
+    }
+    if (tot == b->task_totals && kern == b->task_kernel && b->d_tasks) return DEM_OK;
+    std::vector<int> order(n);
+    for (int k = 0; k < n; ++k) order[k] = k;
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return tot[x] > tot[y]; });
+    std::vector<int2> t;
+    int nw = 0;
+    for (int pass = 0; pass < 2; ++pass) {
+        for (int k : order) {
+            const bool wave = kern[k] == DEM_KERNEL_WAVE;
+            if (wave != (pass == 0)) continue;
+            const int nblk = (tot[k] + 255) / 256;
+            for (int q = 0; q < nblk; ++q) t.push_back(make_int2(k, q));
+        }
+        if (pass == 0) nw = (int)t.size();
+    }
+    if ((int)t.size() > b->task_cap) {
+        (void)hipFree(b->d_tasks);
+        b->d_tasks = nullptr;
+        b->task_cap = 0;
+        HIPCHK(hipMalloc((void **)&b->d_tasks, (t.size() ? t.size() : 1) * sizeof(int2)));
+        b->task_cap = (int)t.size();
+    }
+    if (!t.empty()) {
+        HIPCHK(hipMemcpyAsync(b->d_tasks, t.data(), t.size() * sizeof(int2), hipMemcpyHostToDevice,
+                              b->stream));
+        HIPCHK(hipStreamSynchronize(b->stream));
+    }
+    b->ntask_wave = nw;
+    b->ntask_loop = (int)t.size() - nw;
+    b->task_totals = tot;
+    b->task_kernel = kern;
+    return DEM_OK;
+}
+
+// rebuilds every instance in need among those with steps left, then checks
+// their overflow flags with one synchronisation
+static int batch_rebuilds(dem_batch *b, const std::vector<int> &left) {
+    const int n = (int)b->inst.size();
+    std::vector<int> done;
+    HIPCHK(hipEventRecord(b->ev0, b->stream));
+    for (int k = 0; k < n; ++k) {
+        dem_ctx *c = b->inst[k];
+        if (left[k] <= 0 || !c->rebuild_pending) continue;
+        int rc = dem_do_rebuild(c);
+        if (rc) return rc;
+        done.push_back(k);
+    }
+    if (done.empty()) return DEM_OK;
+    HIPCHK(hipEventRecord(b->ev1, b->stream));
+    for (size_t q = 0; q < done.size(); ++q)
+        HIPCHK(hipMemcpyAsync(&b->h_ovf[q], b->inst[done[q]]->overflow, sizeof(int),
+                              hipMemcpyDeviceToHost, b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+    b->rebuild_ms += ms;
+    b->n_rebuild += (long long)done.size();
+    for (size_t q = 0; q < done.size(); ++q)
+        if (b->h_ovf[q] > 0)
+            return dem_fail(DEM_ERR_OVERFLOW,
+                            "instance %d: a neighbour list needed %d slots, more than its size level "
+                            "allows", done[q], b->h_ovf[q]);
+    return DEM_OK;
+}
+
+extern "C" int dem_batch_step(dem_batch *b, int nsteps) {
+    if (!b || nsteps < 0) return dem_fail(DEM_ERR_ARG, "bad argument");
+    const int n = (int)b->inst.size();
+    for (dem_ctx *c : b->inst)
+        if (c->precision != b->precision) return dem_fail(DEM_ERR_STATE, "mixed precisions in batch");
+    int rc;
+    if ((rc = upload_insts(b))) return rc;
+    std::vector<int> left(n, nsteps);
+    const bool mixed = b->precision == DEM_PREC_MIXED;
+    for (;;) {
+        bool any = false;
+        for (int k = 0; k < n; ++k) any = any || left[k] > 0;
+        if (!any) break;
+        if ((rc = batch_rebuilds(b, left))) return rc;
+        if ((rc = build_tasks(b))) return rc;
+        int S = 0;
+        for (int k = 0; k < n; ++k) {
+            dem_ctx *c = b->inst[k];
+            BatchInst &T = b->h_tab[k];
+            memset(&T, 0, sizeof(T));
+            T.A = dem_step_args(c, true);
+            T.A.stop_at = b->d_stop + k;
+            for (int q = 0; q < 2; ++q) {
+                T.P4[q] = c->P4[q];
+                T.VW[q] = c->VW[q];
+                T.lq[q] = c->lq[q];
+            }
+            T.cur = c->cur;
+            T.Lidx = c->Lidx;
+            T.limit = left[k] < kBatchChunk ? (left[k] > 0 ? left[k] : 0) : kBatchChunk;
+            T.Tw0 = c->Tw;
+            T.dTw = c->S.topv * c->S.dt;
+            S = T.limit > S ? T.limit : S;
+        }
+        HIPCHK(hipMemcpyAsync(b->d_tab, b->h_tab, (size_t)n * sizeof(BatchInst), hipMemcpyHostToDevice,
+                              b->stream));
+        launch_fill_i32(b->d_stop, INT_MAX, n, b->stream);
+        HIPCHK(hipEventRecord(b->ev0, b->stream));
+        for (int s = 0; s < S; ++s) {
+            launch_step_batch(b->d_inst, b->d_tab, b->d_tasks, b->ntask_wave, true, mixed, s, b->stream);
+            launch_step_batch(b->d_inst, b->d_tab, b->d_tasks + b->ntask_wave, b->ntask_loop, false, mixed,
+                              s, b->stream);
+        }
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipEventRecord(b->ev1, b->stream));
+        HIPCHK(hipMemcpyAsync(b->h_stop, b->d_stop, (size_t)n * sizeof(int), hipMemcpyDeviceToHost,
+                              b->stream));
+        HIPCHK(hipStreamSynchronize(b->stream));
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+        b->step_ms += ms;
+        b->launches += S;
+        for (int k = 0; k < n; ++k) {
+            const int lim = b->h_tab[k].limit;
+            if (lim <= 0) continue;
+            dem_ctx *c = b->inst[k];
+            const int ex = dem_finish_chunk(c, lim, b->h_stop[k]);
+            c->forces_valid = c->store_forces != 0 && ex > 0;
+            left[k] -= ex;
+        }
+    }
+    return DEM_OK;
+}
+
+extern "C" int dem_batch_timing(dem_batch *b, double *step_ms, long long *launches, double *rebuild_ms,
+                                long long *n_rebuild) {
+    if (!b) return dem_fail(DEM_ERR_ARG, "null batch");
+    if (step_ms) *step_ms = b->step_ms;
+    if (launches) *launches = b->launches;
+    if (rebuild_ms) *rebuild_ms = b->rebuild_ms;
+    if (n_rebuild) *n_rebuild = b->n_rebuild;
+    return DEM_OK;
+}
+
+// ---------------------------------------------------------------------
+// analysis
+// ---------------------------------------------------------------------
+static int analyze(const std::vector<dem_ctx *> &cs, hipStream_t st, DevInst *d_inst, AnaArgs *d_ana,
+                   AnaArgs *h_ana, double *d_out, double *h_out, double *phi, double *Z, double *top) {
+    const int n = (int)cs.size();
+    std::vector<DevInst> hi(n);
+    for (int k = 0; k < n; ++k) {
+        dem_ctx *c = cs[k];
+        if (c->rebuild_pending || !c->has_lists) {
+            int rc = dem_do_rebuild(c);  // lists of the current positions
+            if (rc) return rc;
+        }
+        hi[k] = c->P;
+        AnaArgs &G = h_ana[k];
+        G.p = c->P4[c->cur];
+        G.nbr = c->nbr[c->hcur];
+        G.nnb = c->nnb[c->hcur];
+        G.wl = c->wl[c->hcur];
+        G.nwl = c->nwl[c->hcur];
+        G.cap = c->cap;
+        G.total = c->total;
+        G.Tw = c->Tw;
+    }
+    HIPCHK(hipMemcpyAsync(d_inst, hi.data(), (size_t)n * sizeof(DevInst), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(d_ana, h_ana, (size_t)n * sizeof(AnaArgs), hipMemcpyHostToDevice, st));
+    launch_analyze(d_inst, d_ana, d_out, n, st);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(h_out, d_out, (size_t)n * 3 * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    for (int k = 0; k < n; ++k) {
+        if (phi) phi[k] = h_out[3 * k];
+        if (Z) Z[k] = h_out[3 * k + 1];
+        if (top) top[k] = h_out[3 * k + 2];
+    }
+    return DEM_OK;
+}
+
+extern "C" int dem_batch_analyze(dem_batch *b, double *phi, double *Z, double *top) {
+    if (!b) return dem_fail(DEM_ERR_ARG, "null batch");
+    return analyze(b->inst, b->stream, b->d_inst, b->d_ana, b->h_ana, b->d_out, b->h_out, phi, Z, top);
+}
+
+extern "C" int dem_analyze(dem_ctx *c, double *phi, double *Z, double *top) {
+    if (!c) return dem_fail(DEM_ERR_ARG, "null context");
+    DevInst *d_inst = nullptr;
+    AnaArgs *d_ana = nullptr;
+    double *d_out = nullptr;
+    AnaArgs h_ana;
+    double h_out[3];
+    int rc = DEM_OK;
+    hipError_t e = hipMalloc((void **)&d_inst, sizeof(DevInst));
+    if (e == hipSuccess) e = hipMalloc((void **)&d_ana, sizeof(AnaArgs));
+    if (e == hipSuccess) e = hipMalloc((void **)&d_out, 3 * sizeof(double));
+    if (e != hipSuccess) rc = dem_fail(DEM_ERR_HIP, "analysis allocation: %s", hipGetErrorString(e));
+    if (rc == DEM_OK)
+        rc = analyze(std::vector<dem_ctx *>{c}, c->stream, d_inst, d_ana, &h_ana, d_out, h_out, phi, Z,
+                     top);
+    (void)hipStreamSynchronize(c->stream);
+    (void)hipFree(d_inst);
+    (void)hipFree(d_ana);
+    (void)hipFree(d_out);
+    return rc;
+}

@@ -16,7 +16,7 @@
 
 static const int kChunk = 64;
 
-static StepArgs step_args(dem_ctx *c, bool fused) {
+StepArgs dem_step_args(dem_ctx *c, bool fused) {
     StepArgs A;
     memset(&A, 0, sizeof(A));
     if (fused) {
@@ -37,6 +37,7 @@ static StepArgs step_args(dem_ctx *c, bool fused) {
     A.Mx = c->F[3]; A.My = c->F[4]; A.Mz = c->F[5]; A.Mr = c->F[6];
     A.stop_at = c->stop_at;
     A.cap = c->cap;
+    A.K = c->K;
     A.total = c->total;
     A.step = 0;
     A.store_forces = fused ? c->store_forces : 1;
@@ -515,7 +516,7 @@ extern "C" int dem_var_init(dem_ctx *c) {
 
 extern "C" int dem_pp_inter_calc(dem_ctx *c) {
     if (!c) return dem_fail(DEM_ERR_ARG, "null context");
-    StepArgs A = step_args(c, false);
+    StepArgs A = dem_step_args(c, false);
     launch_pp_force(c->P, A, c->S.forceCap == 1, c->precision == DEM_PREC_MIXED, c->stream);
     HIPCHK(hipGetLastError());
     return DEM_OK;
@@ -524,7 +525,7 @@ extern "C" int dem_pp_inter_calc(dem_ctx *c) {
 extern "C" int dem_cir_pw_calc(dem_ctx *c) {
     if (!c) return dem_fail(DEM_ERR_ARG, "null context");
     c->Tw = c->Tw + c->S.topv * c->S.dt;  // kernel.cu:966-969
-    StepArgs A = step_args(c, false);
+    StepArgs A = dem_step_args(c, false);
     launch_pw_force(c->P, A, c->precision == DEM_PREC_MIXED, c->stream);
     HIPCHK(hipGetLastError());
     return DEM_OK;
@@ -532,7 +533,7 @@ extern "C" int dem_cir_pw_calc(dem_ctx *c) {
 
 extern "C" int dem_update_system(dem_ctx *c, int *reNebld) {
     if (!c) return dem_fail(DEM_ERR_ARG, "null context");
-    StepArgs A = step_args(c, false);
+    StepArgs A = dem_step_args(c, false);
     launch_fill_i32(c->upd_count, 0, 1, c->stream);
     launch_update(c->P, A, c->upd_count, c->stream);
     HIPCHK(hipGetLastError());
@@ -561,30 +562,31 @@ extern "C" int dem_step_unfused(dem_ctx *c) {
 // ---------------------------------------------------------------------
 // the fused path
 // ---------------------------------------------------------------------
-static bool use_compact(const dem_ctx *c) {
-    if (c->S.forceCap == 1 || c->K > 64) return false;
-    return c->variant != DEM_KERNEL_LOOP;
+// the step kernel that dem_step launches: DEM_KERNEL_LOOP, _COMPACT or _WAVE
+int dem_step_kernel(const dem_ctx *c) {
+    if (c->S.forceCap == 1 || c->K > 64) return DEM_KERNEL_LOOP;
+    return c->variant == DEM_KERNEL_AUTO ? DEM_KERNEL_WAVE : c->variant;
 }
 
 extern "C" int dem_set_kernel(dem_ctx *c, int variant) {
     if (!c) return dem_fail(DEM_ERR_ARG, "null context");
-    if (variant != DEM_KERNEL_AUTO && variant != DEM_KERNEL_LOOP && variant != DEM_KERNEL_COMPACT)
+    if (variant < DEM_KERNEL_AUTO || variant > DEM_KERNEL_WAVE)
         return dem_fail(DEM_ERR_ARG, "unknown kernel variant %d", variant);
-    if (variant == DEM_KERNEL_COMPACT && (c->S.forceCap == 1 || c->K > 64))
-        return dem_fail(DEM_ERR_ARG, "the compacting kernel needs a dry packing with <= 64 slots");
+    if ((variant == DEM_KERNEL_COMPACT || variant == DEM_KERNEL_WAVE) &&
+        (c->S.forceCap == 1 || c->K > 64))
+        return dem_fail(DEM_ERR_ARG, "the compacting kernels need a dry packing with <= 64 slots");
     c->variant = variant;
     return DEM_OK;
 }
 
 static int run_chunk(dem_ctx *c, int nsteps, int *executed) {
     const bool wet = c->S.forceCap == 1;
-    const bool compact = use_compact(c);
+    const int kern = dem_step_kernel(c);
     const bool mixed = c->precision == DEM_PREC_MIXED;
     int big = INT_MAX;
     HIPCHK(hipMemcpyAsync(c->stop_at, &big, sizeof(int), hipMemcpyHostToDevice, c->stream));
-    StepArgs A = step_args(c, true);
-    const double tw0 = c->Tw;
-    double tw = tw0;
+    StepArgs A = dem_step_args(c, true);
+    double tw = c->Tw;
     HIPCHK(hipEventRecord(c->ev0, c->stream));
     for (int s = 0; s < nsteps; ++s) {
         const int b = (c->cur + s) & 1, l = (c->Lidx + s) & 1;
@@ -593,7 +595,9 @@ static int run_chunk(dem_ctx *c, int nsteps, int *executed) {
         A.step = s;
         tw = tw + c->S.topv * c->S.dt;
         A.Tw = tw;
-        if (compact)
+        if (kern == DEM_KERNEL_WAVE)
+            launch_step_wave(c->P, A, mixed, c->stream);
+        else if (kern == DEM_KERNEL_COMPACT)
             launch_step_compact(c->P, A, c->K, mixed, c->stream);
         else
             launch_step_fused(c->P, A, wet, mixed, c->stream);
@@ -605,19 +609,22 @@ static int run_chunk(dem_ctx *c, int nsteps, int *executed) {
     HIPCHK(hipStreamSynchronize(c->stream));
     float ms = 0.f;
     HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
-    int done = stop < nsteps ? stop : nsteps;
     c->fused_ms += ms;
-    c->fused_steps += done;
     c->fused_launches += nsteps;
+    *executed = dem_finish_chunk(c, nsteps, stop);
+    return DEM_OK;
+}
+
+int dem_finish_chunk(dem_ctx *c, int nsteps, int stop) {
+    const int done = stop < nsteps ? stop : nsteps;
+    c->fused_steps += done;
     c->cur = (c->cur + done) & 1;
     c->Lidx = (c->Lidx + done) & 1;
-    c->Tw = tw0;
     for (int s = 0; s < done; ++s) c->Tw = c->Tw + c->S.topv * c->S.dt;
     for (int s = 0; s < done; ++s) c->ctime = c->ctime + c->S.dt;
     c->iter += done;
     if (stop <= nsteps) c->rebuild_pending = true;
-    *executed = done;
-    return DEM_OK;
+    return done;
 }
 
 extern "C" int dem_step(dem_ctx *c, int nsteps) {
@@ -665,6 +672,15 @@ extern "C" int dem_get_timing(dem_ctx *c, double *fused_ms, long long *fused_ste
     if (rebuild_ms) *rebuild_ms = c->rebuild_ms;
     if (n_rebuild) *n_rebuild = c->n_rebuild;
     return DEM_OK;
+}
+
+extern "C" const char *dem_step_kernel_name(dem_ctx *c) {
+    if (!c) return "";
+    switch (dem_step_kernel(c)) {
+    case DEM_KERNEL_WAVE: return "k_step_wave";
+    case DEM_KERNEL_COMPACT: return "k_step_compact";
+    default: return "k_step_fused";
+    }
 }
 
 extern "C" int dem_sync(dem_ctx *c) {
@@ -719,7 +735,7 @@ extern "C" int dem_contact_stats(dem_ctx *c, dem_stats *out) {
     if (c->rebuild_pending && (rc = dem_rebuild(c))) return rc;
     unsigned long long h[8] = {0};
     HIPCHK(hipMemcpyAsync(c->stats, h, sizeof h, hipMemcpyHostToDevice, c->stream));
-    StepArgs A = step_args(c, true);
+    StepArgs A = dem_step_args(c, true);
     launch_stats(c->P, A, c->stats, c->stream);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(h, c->stats, sizeof h, hipMemcpyDeviceToHost, c->stream));

@@ -25,6 +25,7 @@ struct dem_ctx {
     int precision = DEM_PREC_FP64;
     int cap = 0, total = 0, K = 0;
     hipStream_t stream = nullptr;
+    bool own_stream = true;      // false when a batch lends its stream
     DevInst P;
     double Tw = 0.0;
     std::vector<double> rad_h;   // reduced radius by original id
@@ -89,3 +90,13 @@ void dem_fill_inst(const dem_setup &S, DevInst &P);
 void dem_refresh_inst(dem_ctx *c);
 // Internal pieces of the rebuild and step, shared by the C ABI functions.
 int dem_do_rebuild(dem_ctx *c);
+// StepArgs of the context's current buffers (fused: read cur, write 1-cur)
+StepArgs dem_step_args(dem_ctx *c, bool fused);
+// DEM_KERNEL_LOOP, _COMPACT or _WAVE: the kernel dem_step launches
+int dem_step_kernel(const dem_ctx *c);
+// host bookkeeping after a chunk of nsteps launches whose device stop
+// index read back as stop; returns the steps executed
+int dem_finish_chunk(dem_ctx *c, int nsteps, int stop);
+// dem_create on a stream owned by the caller
+int dem_create_on(const dem_input *in, const double *dia_m, int precision, hipStream_t stream,
+                  dem_ctx **out);

@@ -87,5 +87,27 @@ struct StepArgs {
     double *Fx, *Fy, *Fz, *Mx, *My, *Mz, *Mr;  // optional force output
     int *stop_at;              // device rebuild request (step index)
     int cap, total, step, store_forces;
+    int K;                     // neighbour slots per particle (list stride)
     double Tw;                 // lid height for this step
+};
+
+// One instance of a batched launch: its step arguments apart from the
+// per-step buffers, both record buffers, the buffer parities at the start
+// of the chunk, the number of steps it may take in this chunk and its lid
+// height (Tw after step s of the chunk is Tw0 + (s+1) increments of dTw,
+// added one at a time as the single-instance driver does).
+struct BatchInst {
+    StepArgs A;
+    double4 *P4[2], *VW[2];
+    int *lq[2];
+    int cur, Lidx, limit, pad;
+    double Tw0, dTw;
+};
+
+// Inputs of the packing analysis of one instance (current records).
+struct AnaArgs {
+    const double4 *p;
+    const int *nbr, *nnb, *wl, *nwl;
+    int cap, total;
+    double Tw;
 };

@@ -883,7 +883,33 @@ DEV void pp_particle(const DevInst &P, const StepArgs &A, int i, const double4 p
     }
 }
 
-// walls of particle i (kernel.cu:958-1023, 370-403)
+// contact point, normal and gap of particle (ip, ri) against wall w: lid
+// (x, y, Tw), floor (x, y, Bw), or the lateral cylinder point at radius R
+// (kernel.cu:987-1014)
+DEV void wall_geometry(const DevInst &P, double Tw, const V3 ip, double ri, int w, V3 &nv,
+                       double &gap) {
+    V3 jp;
+    if (w == 0) {
+        jp = v3(ip.x, ip.y, Tw);
+    } else if (w == 1) {
+        jp = v3(ip.x, ip.y, P.Bw);
+    } else {
+        double ax = P.cirX - ip.x, ay = P.cirY - ip.y;
+        double dc = sqrt(ax * ax + ay * ay);
+        double cs = (ip.x - P.cirX) / dc, sn = (ip.y - P.cirY) / dc;
+        jp.x = P.R * cs + P.cirX;
+        jp.y = P.R * sn + P.cirY;
+        jp.z = ip.z;
+    }
+    const V3 dv = sub(ip, jp);
+    const double l = len(dv);
+    nv = divv(dv, l);
+    gap = l - ri;
+}
+
+// walls of particle i (kernel.cu:958-1023, 370-403).  The wall velocity is
+// not reset between slots (kernel.cu:979-1020): once the lid has been seen,
+// later slots use the lid velocity.
 template <bool MIXED>
 DEV void pw_particle(const DevInst &P, const StepArgs &A, int i, const double4 pi, V3 iv, V3 iw,
                      V3 &F, V3 &M, double &Mr) {
@@ -891,27 +917,14 @@ DEV void pw_particle(const DevInst &P, const StepArgs &A, int i, const double4 p
     if (n == 0) return;
     const V3 ip = v3(pi.x, pi.y, pi.z);
     const double ri = pi.w;
-    V3 jp = v3(0.0, 0.0, 0.0), jv = v3(0.0, 0.0, 0.0);
+    V3 jv = v3(0.0, 0.0, 0.0);
     for (int k = 0; k < n; ++k) {
         const long sk = (long)k * A.cap + i;
         const int w = A.wl[sk];
-        if (w == 0) {
-            jp = v3(ip.x, ip.y, A.Tw);
-            jv = v3(0.0, 0.0, P.topv);
-        } else if (w == 1) {
-            jp = v3(ip.x, ip.y, P.Bw);
-        } else {
-            double ax = P.cirX - ip.x, ay = P.cirY - ip.y;
-            double dc = sqrt(ax * ax + ay * ay);
-            double cs = (ip.x - P.cirX) / dc, sn = (ip.y - P.cirY) / dc;
-            jp.x = P.R * cs + P.cirX;
-            jp.y = P.R * sn + P.cirY;
-            jp.z = ip.z;
-        }
-        const V3 dv = sub(ip, jp);
-        const double l = len(dv);
-        const V3 nv = divv(dv, l);
-        const double gap = l - ri;
+        if (w == 0) jv = v3(0.0, 0.0, P.topv);
+        V3 nv;
+        double gap;
+        wall_geometry(P, A.Tw, ip, ri, w, nv, gap);
         if (gap < 0.0) {
             HistRec h = A.hw[sk];
             V3 cpv = sub(sub(iv, jv), cross(mulv(iw, ri), nv));
@@ -970,12 +983,10 @@ DEV void request_rebuild(const StepArgs &A, bool mine) {
         atomicMin(A.stop_at, A.step + 1);
 }
 
+// one thread per particle: block blk of 256 particles
 template <bool WET, bool MIXED>
-__global__ void __launch_bounds__(256) k_step_fused(DevInst P, StepArgs A) {
-    // a rebuild is due before this step
-    if (__hip_atomic_load(A.stop_at, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <= A.step) return;
-    const unsigned nb = gridDim.x;
-    const int i = (int)(xcd_block(nb) * blockDim.x + threadIdx.x);
+DEV void step_loop_block(const DevInst &P, const StepArgs &A, unsigned blk) {
+    const int i = (int)(blk * 256u + threadIdx.x);
     bool over = false;
     if (i < A.total) {
         const double4 pi = A.in.p[i];
@@ -994,6 +1005,13 @@ __global__ void __launch_bounds__(256) k_step_fused(DevInst P, StepArgs A) {
         over = integrate(P, A, i, pi, va, vb, F, M, Mr);
     }
     request_rebuild(A, over);
+}
+
+template <bool WET, bool MIXED>
+__global__ void __launch_bounds__(256) k_step_fused(DevInst P, StepArgs A) {
+    // a rebuild is due before this step
+    if (__hip_atomic_load(A.stop_at, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <= A.step) return;
+    step_loop_block<WET, MIXED>(P, A, xcd_block(gridDim.x));
 }
 
 void launch_step_fused(const DevInst &P, const StepArgs &A, bool wet, bool mixed, hipStream_t s) {
@@ -1165,6 +1183,466 @@ void launch_step_compact(const DevInst &P, const StepArgs &A, int K, bool mixed,
         hipLaunchKernelGGL((k_step_compact<true>), dim3(nb), dim3(CP_BS), lds, s, P, A);
     else
         hipLaunchKernelGGL((k_step_compact<false>), dim3(nb), dim3(CP_BS), lds, s, P, A);
+}
+
+// ---------------------------------------------------------------------
+// Dry fused step with per-wave contact compaction (no block barriers, no
+// LDS contact list).  Each wave owns 64 consecutive particles.
+//  A: the owner tests its neighbour slots eight at a time (eight index
+//     loads, then eight position loads in flight) and its wall slots,
+//     keeping a 64-bit pair mask and a 3-bit wall mask.
+//  B: a wave prefix sum numbers the wave's contacts 0..T-1, particle by
+//     particle, pairs in slot order, then walls in slot order.
+//  C: in rounds of 64, lane t finds its contact's owner by binary search
+//     over the lanes' offsets and its slot by selecting the r-th set bit of
+//     the owner's masks (register shuffles only), evaluates the contact law
+//     and puts the force, moment and rolling term in the wave's LDS slice;
+//     each owner then adds its entries in list order.  The next round's
+//     neighbour indices load while a round computes.
+//  D: integration by the owner.
+// The owners' records stay in an LDS stash from A to D (not in registers,
+// which keeps the kernel at 4 waves per SIMD, and not re-read from memory).
+// The summation order per particle is gravity, contacts in slot order,
+// walls in slot order (kernel.cu:439-498, 958-1023), as in the one-thread
+// loop, so the FP64 results are bit-identical to it.  Requires K <= 64.
+// ---------------------------------------------------------------------
+#define WV_BS 256
+#define WV_NB 8
+
+// per-wave LDS copy of the owners' records (read by the contact lanes of
+// phase C and by the owners in phase D instead of re-reading them)
+struct WvStash {
+    double4 p[64], va[64], vb[64];
+};
+
+DEV void wave_sync_lds() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// position of the r-th (0-based) set bit of m; m has more than r bits set
+DEV int select_bit(unsigned long long m, unsigned r) {
+    int pos = 0;
+    unsigned c = (unsigned)__popc((unsigned)m);
+    if (r >= c) { r -= c; m >>= 32; pos += 32; }
+    c = (unsigned)__popc((unsigned)(m & 0xffffull));
+    if (r >= c) { r -= c; m >>= 16; pos += 16; }
+    c = (unsigned)__popc((unsigned)(m & 0xffull));
+    if (r >= c) { r -= c; m >>= 8; pos += 8; }
+    c = (unsigned)__popc((unsigned)(m & 0xfull));
+    if (r >= c) { r -= c; m >>= 4; pos += 4; }
+    c = (unsigned)__popc((unsigned)(m & 0x3ull));
+    if (r >= c) { r -= c; m >>= 2; pos += 2; }
+    c = (unsigned)(m & 1ull);
+    if (r >= c) pos += 1;
+    return pos;
+}
+
+// one particle-particle entry of the compacted list: slot k of particle
+// ii, neighbour j
+template <bool MIXED>
+DEV void pair_entry(const DevInst &P, const StepArgs &A, int ii, int k, int j, const double4 qi,
+                    const double4 ai, const double4 bi, V3 &dF, V3 &dM, double &dMr) {
+    const long sk = (long)k * A.cap + ii;
+    HistRec h = A.h[sk];
+    const double4 qj = A.in.p[j];
+    const V3 iv = v3(ai.x, ai.y, ai.z), iw = v3(ai.w, bi.x, bi.y);
+    const V3 jv = vel_of(A.in.v, j), jw = spin_of(A.in.v, j);
+    V3 nv;
+    double gap;
+    pair_geometry<MIXED>(qi, qj, nv, gap);
+    const double ri = qi.w, rj = qj.w;
+    const double R = ri * rj / (ri + rj);
+    const V3 cpv = sub(sub(iv, jv), cross(add(mulv(iw, ri), mulv(jw, rj)), nv));
+    if (MIXED)
+        contact_mixed<true>(P.pp, P.rolling, P.dt, h, -gap, nv, R, ri, cpv, dF, dM, dMr);
+    else
+        contact<true>(P.pp, P.rolling, P.dt, h, -gap, nv, R, ri, cpv, dF, dM, dMr);
+    A.h[sk] = h;
+}
+
+// one wall entry: slot kw of particle ii, wall w.  The wall velocity is the
+// lid's once the lid has been seen in this or an earlier slot
+// (kernel.cu:979-1020).
+template <bool MIXED>
+DEV void wall_entry(const DevInst &P, const StepArgs &A, int ii, int kw, int w, const double4 qi,
+                    const double4 ai, const double4 bi, V3 &dF, V3 &dM, double &dMr) {
+    const long cap = A.cap;
+    const long sk = (long)kw * cap + ii;
+    HistRec h = A.hw[sk];
+    const V3 iv = v3(ai.x, ai.y, ai.z), iw = v3(ai.w, bi.x, bi.y);
+    bool lid = w == 0;
+    for (int k = 0; k < kw; ++k) lid = lid || A.wl[(long)k * cap + ii] == 0;
+    const V3 jv = v3(0.0, 0.0, lid ? P.topv : 0.0);
+    const double ri = qi.w;
+    V3 nv;
+    double gap;
+    wall_geometry(P, A.Tw, v3(qi.x, qi.y, qi.z), ri, w, nv, gap);
+    const V3 cpv = sub(sub(iv, jv), cross(mulv(iw, ri), nv));
+    if (MIXED)
+        contact_mixed<false>(P.pw, P.rolling, P.dt, h, -gap, nv, ri, ri, cpv, dF, dM, dMr);
+    else
+        contact<false>(P.pw, P.rolling, P.dt, h, -gap, nv, ri, ri, cpv, dF, dM, dMr);
+    A.hw[sk] = h;
+}
+
+// a compacted entry: particle ii (wave-relative o), slot k, pair or wall
+struct WvEntry {
+    int ii, k;
+    bool wall, valid;
+};
+
+// entry t of the wave's list; every lane of the wave must call this (it
+// reads the other lanes' offsets and masks through shuffles)
+DEV WvEntry wave_entry(unsigned t, unsigned T, unsigned off, unsigned mlo, unsigned mhi,
+                       unsigned wmask, int wbase) {
+    const unsigned tq = t < T ? t : T - 1;
+    int o = 0;
+#pragma unroll
+    for (int s = 32; s > 0; s >>= 1) {
+        const unsigned om = (unsigned)__shfl((int)off, o + s, 64);
+        if (om <= tq) o += s;
+    }
+    const unsigned olo = (unsigned)__shfl((int)mlo, o, 64);
+    const unsigned ohi = (unsigned)__shfl((int)mhi, o, 64);
+    const unsigned ow = (unsigned)__shfl((int)wmask, o, 64);
+    const unsigned ooff = (unsigned)__shfl((int)off, o, 64);
+    const unsigned long long om = ((unsigned long long)ohi << 32) | olo;
+    const unsigned r = tq - ooff, np = (unsigned)__popcll(om);
+    WvEntry e;
+    e.ii = wbase + o;
+    e.valid = t < T;
+    e.wall = r >= np;
+    e.k = e.wall ? select_bit(ow, r - np) : select_bit(om, r);
+    return e;
+}
+
+// block blk of WV_BS particles; res is this wave's [7][64] LDS slice
+template <bool MIXED>
+DEV void step_wave_block(const DevInst &P, const StepArgs &A, unsigned blk, double (*res)[64],
+                         WvStash *st) {
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int i = (int)(blk * WV_BS) + tid;
+    const int wbase = i - lane;
+    const bool own = i < A.total;
+    const long cap = A.cap;
+
+    // phase A: contact masks of the pair slots and the wall slots.  The
+    // first batch of neighbour indices is read together with the counts (in
+    // bounds whatever the count is: the list holds K slots per particle);
+    // entries past the count are masked after.
+    unsigned long long mask = 0ull;
+    unsigned wmask = 0u;
+    double grav = 0.0;
+    if (own) {
+        int jj[WV_NB];
+        const int kb = A.K < WV_NB ? A.K : WV_NB;
+#pragma unroll
+        for (int u = 0; u < WV_NB; ++u) jj[u] = u < kb ? A.nbr[(long)u * cap + i] : i;
+        const double4 pi = A.in.p[i];
+        const int n = A.nnb[i], nw = A.nwl[i];
+        st->p[lane] = pi;
+        st->va[lane] = A.in.v[2 * i];
+        st->vb[lane] = A.in.v[2 * i + 1];
+        for (int k0 = 0; k0 < n; k0 += WV_NB) {
+            if (k0 > 0) {
+#pragma unroll
+                for (int u = 0; u < WV_NB; ++u)
+                    jj[u] = k0 + u < n ? A.nbr[(long)(k0 + u) * cap + i] : i;
+            } else {
+#pragma unroll
+                for (int u = 0; u < WV_NB; ++u) jj[u] = u < n ? jj[u] : i;
+            }
+            double4 q[WV_NB];
+#pragma unroll
+            for (int u = 0; u < WV_NB; ++u) q[u] = A.in.p[jj[u]];
+#pragma unroll
+            for (int u = 0; u < WV_NB; ++u) {
+                V3 nv;
+                double gap;
+                if (k0 + u < n && pair_geometry<MIXED>(pi, q[u], nv, gap)) mask |= 1ull << (k0 + u);
+            }
+        }
+        const V3 ip = v3(pi.x, pi.y, pi.z);
+        for (int k = 0; k < nw; ++k) {
+            V3 nv;
+            double gap;
+            wall_geometry(P, A.Tw, ip, pi.w, A.wl[(long)k * cap + i], nv, gap);
+            if (gap < 0.0) wmask |= 1u << k;
+        }
+        const double di = 2.0 * pi.w;
+        grav = di * di * di;
+    }
+    wave_sync_lds();  // the stash is read by other lanes in phase C
+    // phase B: number the wave's entries, pairs before walls per particle
+    const unsigned cnt = (unsigned)__popcll(mask) + (unsigned)__popc(wmask);
+    const unsigned incl = wave_incl_scan(cnt);
+    const unsigned off = incl - cnt;
+    const unsigned T = __shfl(incl, 63, 64);
+
+    double Fx = 0.0, Fy = 0.0, Fz = 0.0, Mx = 0.0, My = 0.0, Mz = 0.0, Mr = 0.0;
+    Fz = Fz - grav;  // gravity first (kernel.cu:439)
+    const unsigned mlo = (unsigned)mask, mhi = (unsigned)(mask >> 32);
+
+    // phase C, software-pipelined: the neighbour index (or wall id) of
+    // round r+1 is in flight while round r computes, so a round waits for
+    // one load (its history record and the two particles' state) only
+    WvEntry e;
+    int j = 0;
+    if (T > 0) {
+        e = wave_entry(lane, T, off, mlo, mhi, wmask, wbase);
+        if (e.valid) j = (e.wall ? A.wl : A.nbr)[(long)e.k * cap + e.ii];
+    }
+    for (unsigned r0 = 0; r0 < T; r0 += 64) {
+        const WvEntry ec = e;
+        const int jc = j;
+        // next round's entry and its neighbour index
+        if (r0 + 64 < T) {
+            e = wave_entry(r0 + 64 + lane, T, off, mlo, mhi, wmask, wbase);
+            if (e.valid) j = (e.wall ? A.wl : A.nbr)[(long)e.k * cap + e.ii];
+        } else {
+            e.valid = false;
+        }
+        if (ec.valid) {
+            V3 dF = v3(0.0, 0.0, 0.0), dM = v3(0.0, 0.0, 0.0);
+            double dMr = 0.0;
+            const int o = ec.ii - wbase;
+            const double4 qi = st->p[o], ai = st->va[o], bi = st->vb[o];
+            if (!ec.wall)
+                pair_entry<MIXED>(P, A, ec.ii, ec.k, jc, qi, ai, bi, dF, dM, dMr);
+            else
+                wall_entry<MIXED>(P, A, ec.ii, ec.k, jc, qi, ai, bi, dF, dM, dMr);
+            res[0][lane] = dF.x; res[1][lane] = dF.y; res[2][lane] = dF.z;
+            res[3][lane] = dM.x; res[4][lane] = dM.y; res[5][lane] = dM.z;
+            res[6][lane] = dMr;
+        }
+        wave_sync_lds();
+        if (cnt) {
+            const unsigned a = off > r0 ? off : r0;
+            const unsigned b = off + cnt < r0 + 64 ? off + cnt : r0 + 64;
+            for (unsigned q = a; q < b; ++q) {
+                const unsigned s = q - r0;
+                Fx = Fx + res[0][s]; Fy = Fy + res[1][s]; Fz = Fz + res[2][s];
+                Mx = Mx + res[3][s]; My = My + res[4][s]; Mz = Mz + res[5][s];
+                Mr = Mr + res[6][s];
+            }
+        }
+        wave_sync_lds();
+    }
+    // phase D: integration by the owner
+    bool over = false;
+    if (own) {
+        const double4 pi = st->p[lane], va = st->va[lane], vb = st->vb[lane];
+        const V3 F = v3(Fx, Fy, Fz), M = v3(Mx, My, Mz);
+        if (A.store_forces) {
+            A.Fx[i] = F.x; A.Fy[i] = F.y; A.Fz[i] = F.z;
+            A.Mx[i] = M.x; A.My[i] = M.y; A.Mz[i] = M.z;
+        }
+        over = integrate(P, A, i, pi, va, vb, F, M, Mr);
+    }
+    request_rebuild(A, over);
+}
+
+template <bool MIXED>
+__global__ void __launch_bounds__(WV_BS) k_step_wave(DevInst P, StepArgs A) {
+    if (__hip_atomic_load(A.stop_at, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <= A.step) return;
+    __shared__ double res_all[WV_BS / 64][7][64];
+    __shared__ WvStash stash[WV_BS / 64];
+    step_wave_block<MIXED>(P, A, xcd_block(gridDim.x), res_all[threadIdx.x >> 6],
+                           &stash[threadIdx.x >> 6]);
+}
+
+void launch_step_wave(const DevInst &P, const StepArgs &A, bool mixed, hipStream_t s) {
+    if (A.total <= 0) return;
+    unsigned nb = blocks_for(A.total, WV_BS);
+    if (mixed)
+        hipLaunchKernelGGL((k_step_wave<true>), dim3(nb), dim3(WV_BS), 0, s, P, A);
+    else
+        hipLaunchKernelGGL((k_step_wave<false>), dim3(nb), dim3(WV_BS), 0, s, P, A);
+}
+
+// ---------------------------------------------------------------------
+// Batched step over many instances in one launch.  Block t of the grid
+// (after the XCD remap) takes task t = (instance, block of that instance);
+// tasks list the instances largest first, so an instance's blocks are
+// contiguous and run on one XCD.  The instance's constants (DevInst) and
+// lists come from per-instance tables in device memory instead of kernel
+// arguments; each instance keeps its own rebuild flag and step limit, so
+// every instance follows exactly the steps it would take on its own.
+// ---------------------------------------------------------------------
+DEV bool batch_step_args(const BatchInst &B, int step, StepArgs &A) {
+    if (step >= B.limit) return false;
+    A = B.A;
+    if (__hip_atomic_load(A.stop_at, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <= step) return false;
+    const int b = (B.cur + step) & 1, l = (B.Lidx + step) & 1;
+    A.in.p = B.P4[b]; A.in.v = B.VW[b]; A.in.lqd = B.lq[l];
+    A.out.p = B.P4[1 - b]; A.out.v = B.VW[1 - b]; A.out.lqd = B.lq[1 - l];
+    A.step = step;
+    double tw = B.Tw0;
+    for (int s = 0; s <= step; ++s) tw = tw + B.dTw;
+    A.Tw = tw;
+    return true;
+}
+
+template <bool MIXED>
+__global__ void __launch_bounds__(WV_BS) k_step_batch_wave(const DevInst *insts, const BatchInst *tab,
+                                                           const int2 *tasks, int step) {
+    const int2 tk = tasks[xcd_block(gridDim.x)];
+    StepArgs A;
+    if (!batch_step_args(tab[tk.x], step, A)) return;
+    __shared__ double res_all[WV_BS / 64][7][64];
+    __shared__ WvStash stash[WV_BS / 64];
+    step_wave_block<MIXED>(insts[tk.x], A, (unsigned)tk.y, res_all[threadIdx.x >> 6],
+                           &stash[threadIdx.x >> 6]);
+}
+
+template <bool MIXED>
+__global__ void __launch_bounds__(256) k_step_batch_loop(const DevInst *insts, const BatchInst *tab,
+                                                         const int2 *tasks, int step) {
+    const int2 tk = tasks[xcd_block(gridDim.x)];
+    StepArgs A;
+    if (!batch_step_args(tab[tk.x], step, A)) return;
+    const DevInst &P = insts[tk.x];
+    if (P.forceCap == 1)
+        step_loop_block<true, MIXED>(P, A, (unsigned)tk.y);
+    else
+        step_loop_block<false, MIXED>(P, A, (unsigned)tk.y);
+}
+
+void launch_step_batch(const DevInst *insts, const BatchInst *tab, const int2 *tasks, int ntasks,
+                       bool wave, bool mixed, int step, hipStream_t s) {
+    if (ntasks <= 0) return;
+    if (wave) {
+        if (mixed)
+            hipLaunchKernelGGL((k_step_batch_wave<true>), dim3(ntasks), dim3(WV_BS), 0, s, insts, tab,
+                               tasks, step);
+        else
+            hipLaunchKernelGGL((k_step_batch_wave<false>), dim3(ntasks), dim3(WV_BS), 0, s, insts, tab,
+                               tasks, step);
+    } else {
+        if (mixed)
+            hipLaunchKernelGGL((k_step_batch_loop<true>), dim3(ntasks), dim3(256), 0, s, insts, tab,
+                               tasks, step);
+        else
+            hipLaunchKernelGGL((k_step_batch_loop<false>), dim3(ntasks), dim3(256), 0, s, insts, tab,
+                               tasks, step);
+    }
+}
+
+// ---------------------------------------------------------------------
+// Packing analysis: solid fraction phi of the core cylinder and mean
+// contact number Z, one block per instance, deterministic reductions.
+// ---------------------------------------------------------------------
+// positive nodes and weights of 32-point Gauss-Legendre on [-1, 1]
+__constant__ double c_gl_x[16] = {0.04830766568773831, 0.14447196158279649, 0.23928736225213706, 0.33186860228212767, 0.42135127613063533, 0.50689990893222936, 0.5877157572407623, 0.66304426693021523, 0.73218211874028971, 0.79448379596794239, 0.84936761373256997, 0.8963211557660522, 0.93490607593773967, 0.96476225558750639, 0.98561151154526838, 0.99726386184948157};
+__constant__ double c_gl_w[16] = {0.096540088514727812, 0.095638720079274833, 0.093844399080804566, 0.091173878695763863, 0.087652093004403908, 0.083311924226946846, 0.078193895787070311, 0.072345794108848449, 0.065822222776361752, 0.058684093478535704, 0.050998059262376244, 0.042835898022226426, 0.034273862913021626, 0.025392065309262427, 0.016274394730905965, 0.0070186100094692984};
+
+// area of the intersection of a disc of radius s (centre at distance d from
+// the axis) with the disc of radius Rc around the axis
+DEV double lens_area(double s, double Rc, double d) {
+    if (s <= 0.0) return 0.0;
+    if (d >= s + Rc) return 0.0;
+    if (d <= fabs(Rc - s)) {
+        const double m = fmin(s, Rc);
+        return M_PI * m * m;
+    }
+    const double c1 = fmin(1.0, fmax(-1.0, (d * d + s * s - Rc * Rc) / (2.0 * d * s)));
+    const double c2 = fmin(1.0, fmax(-1.0, (d * d + Rc * Rc - s * s) / (2.0 * d * Rc)));
+    const double k = (-d + s + Rc) * (d + s - Rc) * (d - s + Rc) * (d + s + Rc);
+    return s * s * acos(c1) + Rc * Rc * acos(c2) - 0.5 * sqrt(fmax(0.0, k));
+}
+
+// volume of the sphere (centre at axial distance rho, height z, radius r)
+// inside {axial distance <= Rc, zlo <= z <= zhi}
+DEV double core_volume(double rho, double z, double r, double Rc, double zlo, double zhi) {
+    const double a = fmax(z - r, zlo), b = fmin(z + r, zhi);
+    if (!(a < b) || rho - r >= Rc) return 0.0;
+    const double u1 = a - z, u2 = b - z;
+    if (rho + r <= Rc)
+        return M_PI * ((r * r * u2 - u2 * u2 * u2 / 3.0) - (r * r * u1 - u1 * u1 * u1 / 3.0));
+    const double h = 0.5 * (u2 - u1), m = 0.5 * (u2 + u1);
+    double v = 0.0;
+    for (int q = 0; q < 16; ++q) {
+        const double up = m + h * c_gl_x[q], um = m - h * c_gl_x[q];
+        const double sp = sqrt(fmax(0.0, r * r - up * up)), sm = sqrt(fmax(0.0, r * r - um * um));
+        v += c_gl_w[q] * (lens_area(sp, Rc, rho) + lens_area(sm, Rc, rho));
+    }
+    return h * v;
+}
+
+DEV double block_sum_f64(double v, double *sh) {
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) sh[threadIdx.x] = sh[threadIdx.x] + sh[threadIdx.x + s];
+        __syncthreads();
+    }
+    const double r = sh[0];
+    __syncthreads();
+    return r;
+}
+
+DEV double block_max_f64(double v, double *sh) {
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + s]);
+        __syncthreads();
+    }
+    const double r = sh[0];
+    __syncthreads();
+    return r;
+}
+
+__global__ void __launch_bounds__(256) k_analyze(const DevInst *insts, const AnaArgs *args,
+                                                 double *out /* [n][3] */) {
+    __shared__ double sh[256];
+    const DevInst &P = insts[blockIdx.x];
+    const AnaArgs &G = args[blockIdx.x];
+    const long cap = G.cap;
+    double contacts = 0.0, top = -1.0e300;
+    for (int i = threadIdx.x; i < G.total; i += blockDim.x) {
+        const double4 pi = G.p[i];
+        const int n = G.nnb[i], nw = G.nwl[i];
+        int c = 0;
+        for (int k = 0; k < n; ++k) {
+            V3 nv;
+            double gap;
+            if (pair_geometry<false>(pi, G.p[G.nbr[(long)k * cap + i]], nv, gap)) ++c;
+        }
+        bool touch = c > 0;
+        for (int k = 0; k < nw; ++k) {
+            V3 nv;
+            double gap;
+            wall_geometry(P, G.Tw, v3(pi.x, pi.y, pi.z), pi.w, G.wl[(long)k * cap + i], nv, gap);
+            touch = touch || gap < 0.0;
+        }
+        contacts += (double)c;
+        if (touch) top = fmax(top, pi.z + pi.w);
+    }
+    const double C = block_sum_f64(contacts, sh);
+    const double T = block_max_f64(top, sh);
+    const double Rc = P.R - 2.0, zlo = P.Bw + 2.0, zhi = T - 2.0;
+    double vol = 0.0;
+    if (zhi > zlo) {
+        for (int i = threadIdx.x; i < G.total; i += blockDim.x) {
+            const double4 pi = G.p[i];
+            const double dx = pi.x - P.cirX, dy = pi.y - P.cirY;
+            vol += core_volume(sqrt(dx * dx + dy * dy), pi.z, pi.w, Rc, zlo, zhi);
+        }
+    }
+    const double V = block_sum_f64(vol, sh);
+    if (threadIdx.x == 0) {
+        const double core = (zhi > zlo && Rc > 0.0) ? M_PI * Rc * Rc * (zhi - zlo) : 0.0;
+        out[3 * blockIdx.x + 0] = core > 0.0 ? V / core : 0.0;
+        out[3 * blockIdx.x + 1] = G.total > 0 ? C / (double)G.total : 0.0;
+        out[3 * blockIdx.x + 2] = T;
+    }
+}
+
+void launch_analyze(const DevInst *insts, const AnaArgs *args, double *out, int n, hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL(k_analyze, dim3(n), dim3(256), 0, s, insts, args, out);
 }
 
 // --- the reference's separate kernels (PpInterCalc, CirPWCalc,

@@ -33,6 +33,10 @@ void launch_hist_remap(const unsigned *perm, const int *id_new, const int *id_ol
 void launch_step_fused(const DevInst &P, const StepArgs &A, bool wet, bool mixed, hipStream_t s);
 // dry step with contact compaction; needs K <= 64
 void launch_step_compact(const DevInst &P, const StepArgs &A, int K, bool mixed, hipStream_t s);
+void launch_step_wave(const DevInst &P, const StepArgs &A, bool mixed, hipStream_t s);
+void launch_analyze(const DevInst *insts, const AnaArgs *args, double *out, int n, hipStream_t s);
+void launch_step_batch(const DevInst *insts, const BatchInst *tab, const int2 *tasks, int ntasks,
+                       bool wave, bool mixed, int step, hipStream_t s);
 void launch_pp_force(const DevInst &P, const StepArgs &A, bool wet, bool mixed, hipStream_t s);
 void launch_pw_force(const DevInst &P, const StepArgs &A, bool mixed, hipStream_t s);
 void launch_update(const DevInst &P, const StepArgs &A, int *count, hipStream_t s);

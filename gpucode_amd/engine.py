@@ -63,10 +63,14 @@ def load_para(path):
 class Engine:
     """One packing on the GPU (a dem_ctx)."""
 
-    def __init__(self, inp, dia=None, precision=A.PREC_FP64):
-        self._ctx = C.c_void_p()
-        d = None if dia is None else np.ascontiguousarray(dia, np.float64)
-        check(A.lib().dem_create(C.byref(inp), dptr(d), int(precision), C.byref(self._ctx)))
+    def __init__(self, inp, dia=None, precision=A.PREC_FP64, _ctx=None):
+        self._owned = _ctx is None
+        if _ctx is None:
+            self._ctx = C.c_void_p()
+            d = None if dia is None else np.ascontiguousarray(dia, np.float64)
+            check(A.lib().dem_create(C.byref(inp), dptr(d), int(precision), C.byref(self._ctx)))
+        else:
+            self._ctx = _ctx
         self.inp = inp
         self.NP = inp.NP
         self.setup = A.DemSetup()
@@ -74,9 +78,9 @@ class Engine:
         self.K = max(self.setup.levelNNebors[l] for l in range(self.setup.nLevels))
 
     def close(self):
-        if self._ctx:
+        if self._ctx and self._owned:
             A.lib().dem_destroy(self._ctx)
-            self._ctx = C.c_void_p()
+        self._ctx = C.c_void_p()
 
     def __del__(self):
         try:
@@ -212,7 +216,7 @@ class Engine:
     def sync(self):
         check(A.lib().dem_sync(self._ctx))
 
-    KERNEL_AUTO, KERNEL_LOOP, KERNEL_COMPACT = 0, 1, 2
+    KERNEL_AUTO, KERNEL_LOOP, KERNEL_COMPACT, KERNEL_WAVE = 0, 1, 2, 3
 
     def set_kernel(self, variant):
         check(A.lib().dem_set_kernel(self._ctx, int(variant)))
@@ -249,6 +253,16 @@ class Engine:
     def device_bytes(self):
         return A.lib().dem_device_bytes(self._ctx)
 
+    def analyze(self):
+        """(phi, Z, top) of the current packing (GPU analysis kernel)."""
+        v = [C.c_double(), C.c_double(), C.c_double()]
+        check(A.lib().dem_analyze(self._ctx, *[C.byref(x) for x in v]))
+        return tuple(x.value for x in v)
+
+    def step_kernel_name(self):
+        """Name of the step kernel dem_step launches for this packing."""
+        return A.lib().dem_step_kernel_name(self._ctx).decode()
+
     # --- restart --------------------------------------------------------------
     def dump(self, path, nBatch=1):
         check(A.lib().dem_dump_write(self._ctx, str(path).encode(), int(nBatch)))
@@ -257,3 +271,66 @@ class Engine:
         nb = C.c_int()
         check(A.lib().dem_dump_read(self._ctx, str(path).encode(), C.byref(nb)))
         return nb.value
+
+
+class Batch:
+    """Many packings stepped together (a dem_batch); instance k is an Engine
+    view owned by the batch."""
+
+    def __init__(self, inputs, dias=None, precision=A.PREC_FP64):
+        n = len(inputs)
+        arr = (A.DemInput * n)(*inputs)
+        self._keep = []
+        dptrs = None
+        if dias is not None:
+            dptrs = (A.D * n)()
+            for k, d in enumerate(dias):
+                if d is not None:
+                    a = np.ascontiguousarray(d, np.float64)
+                    self._keep.append(a)
+                    dptrs[k] = dptr(a)
+        self._b = C.c_void_p()
+        check(A.lib().dem_batch_create(n, arr, dptrs, int(precision), C.byref(self._b)))
+        self.inst = [Engine(inputs[k], None, precision, _ctx=C.c_void_p(
+            A.lib().dem_batch_instance(self._b, k))) for k in range(n)]
+
+    def __len__(self):
+        return len(self.inst)
+
+    def __getitem__(self, k):
+        return self.inst[k]
+
+    def close(self):
+        if self._b:
+            for e in self.inst:
+                e._ctx = C.c_void_p()
+            A.lib().dem_batch_destroy(self._b)
+            self._b = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def step(self, n):
+        check(A.lib().dem_batch_step(self._b, int(n)))
+
+    def analyze(self):
+        n = len(self.inst)
+        phi, Z, top = np.zeros(n), np.zeros(n), np.zeros(n)
+        check(A.lib().dem_batch_analyze(self._b, dptr(phi), dptr(Z), dptr(top)))
+        return phi, Z, top
+
+    def timing(self):
+        ms, la, rms, nr = C.c_double(), C.c_longlong(), C.c_double(), C.c_longlong()
+        check(A.lib().dem_batch_timing(self._b, C.byref(ms), C.byref(la), C.byref(rms),
+                                       C.byref(nr)))
+        return dict(step_ms=ms.value, launches=la.value, rebuild_ms=rms.value,
+                    n_rebuild=nr.value)

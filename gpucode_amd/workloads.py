@@ -233,3 +233,48 @@ def small_bed(n=3000, R_d=8.0, mu=0.5, seed=7, dt=None, jitter=0.03):
     inp = make_input(n, d, R_d * d, H, mu, cutGap=0.1, dt=dt)
     pos = pocket_bed(n, R_d, 1.0, jitter=jitter, seed=seed)
     return dict(name="small_bed", inp=inp, dia=None, pos=pos, steps=2000)
+
+
+# --- parameter sweeps (batched instances) -------------------------------------
+SWEEP_PARAMS = ("mu", "mu_wall", "rfrc", "liquid", "theta_deg", "tension", "restitution",
+                "poly_width")
+SWEEP_RANGES = {
+    "mu": (0.05, 0.8),           # particle-particle sliding friction
+    "mu_wall": (0.05, 0.8),      # particle-wall sliding friction
+    "rfrc": (0.0, 0.1),          # rolling friction coefficient
+    "liquid": (0.001, 0.03),     # liquid volume per particle volume
+    "theta_deg": (0.0, 40.0),    # contact angle
+    "tension": (0.02, 0.08),     # surface tension [N/m]
+    "restitution": (0.3, 0.9),   # normal restitution at 0.5 m/s
+    "poly_width": (0.0, 0.3),    # diameters uniform in [(1-w) d, d]
+}
+
+
+def sweep_point(rng):
+    """One seeded sweep point: the 8 parameters in SWEEP_PARAMS order."""
+    return np.array([rng.uniform(*SWEEP_RANGES[k]) for k in SWEEP_PARAMS])
+
+
+def sweep_instance(x, n, seed, *, d=1.0e-3, E=1.0e6, bed_height_d=20.0, fill=3.0):
+    """A wet pour-and-settle packing for sweep point x (SWEEP_PARAMS order):
+    n spheres of largest diameter d, soft (E = 1 MPa) so that dt is about
+    1e-5 s, fed at random into a cylinder sized for a settled bed of about
+    bed_height_d diameters (radius clamped to 5-15 d), filling fill times
+    that height; capillary bridges on (contact angle theta, rupture
+    (1 + theta/2) V^(1/3))."""
+    mu, mu_w, rfrc, liquid, theta_deg, tension, e, w = [float(v) for v in x]
+    rng = np.random.default_rng(seed)
+    dia = None
+    nLevels = 1
+    if w > 1e-3:
+        dia = d * rng.uniform(1.0 - w, 1.0, n)
+        dia[0] = d
+        nLevels = 2 if w > 0.15 else 1
+    vol_per = (math.pi / 6) / 0.6 * (1.0 if dia is None else float(np.mean((dia / d) ** 3)))
+    R_d = min(15.0, max(5.0, math.sqrt(n * vol_per / (math.pi * bed_height_d))))
+    H = d * fill * (n * vol_per / (math.pi * R_d * R_d) + 4)
+    inp = make_input(n, d, R_d * d, H, mu, E=E, e=e, rfrc=rfrc, cutGap=0.2, nLevels=nLevels,
+                     forceCap=1, liquid=liquid, tension=tension, theta=math.radians(theta_deg),
+                     rupture_mode=1, mu_wall=mu_w)
+    pos = feed_positions(inp, dia, n, seed + 1)
+    return dict(name="sweep", inp=inp, dia=dia, pos=pos, x=np.asarray(x, float))

@@ -1,0 +1,59 @@
+/*
+ * dem_batch.h -- batched multi-instance stepping and packing analysis.
+ *
+ * No counterpart in the reference, which runs one packing per process with
+ * its constants in process-global __constant__ symbols (kernel.cu:15-19)
+ * and PARAMETER (heydem.h:196-204).  Here each instance is an ordinary
+ * dem_ctx (every dem_* call of dem_capi.h works on it) whose constants and
+ * list pointers also sit in per-instance device tables, so one kernel
+ * launch steps all instances.  Each instance keeps its own rebuild trigger
+ * (path length > 0.3 SearchGap, kernel.cu:582, 601) and takes exactly the
+ * steps and rebuilds it would take alone: dem_batch_step gives results
+ * bit-identical to dem_step on each instance separately.
+ */
+#ifndef DEM_BATCH_H
+#define DEM_BATCH_H
+
+#include "dem_capi.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct dem_batch dem_batch;
+
+/* n instances; in[k] and dia[k] (may be NULL, or dia[k] NULL) as for
+ * dem_create.  All instances share one HIP stream and one precision. */
+int dem_batch_create(int n, const dem_input *in, const double *const *dia, int precision,
+                     dem_batch **out);
+int dem_batch_destroy(dem_batch *b);
+int dem_batch_size(dem_batch *b);
+/* The k-th instance, for dem_set_particles, dem_get_* and the rest of the
+ * single-instance API.  Owned by the batch. */
+dem_ctx *dem_batch_instance(dem_batch *b, int k);
+/* Advances every instance by nsteps fused steps, rebuilding each instance
+ * when its own trigger fires; one launch per step covers all instances. */
+int dem_batch_step(dem_batch *b, int nsteps);
+/* Event timings of dem_batch_step: step-kernel time, launches, rebuild
+ * time, instance rebuilds. */
+int dem_batch_timing(dem_batch *b, double *step_ms, long long *launches, double *rebuild_ms,
+                     long long *n_rebuild);
+
+/* Packing analysis of the current state, one value per instance.
+ * phi: solid fraction of the core cylinder of radius R - 2 d_max around the
+ * container axis, between 2 d_max above the floor and 2 d_max below the bed
+ * top; the bed top is the highest z + r of the particles that touch a
+ * particle or a wall.  A sphere inside the cylinder counts with its exact
+ * intersection with the slab; a sphere crossing the cylinder wall counts
+ * with the area of its z-slices inside the cylinder (disc-disc lens area)
+ * integrated over its z extent in the slab by 32-point Gauss-Legendre
+ * quadrature.
+ * Z: mean number of particle-particle contacts (gap < 0) per particle.
+ * top: the bed top [reduced length]. */
+int dem_batch_analyze(dem_batch *b, double *phi, double *Z, double *top);
+int dem_analyze(dem_ctx *ctx, double *phi, double *Z, double *top);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

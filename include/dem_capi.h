@@ -9,9 +9,10 @@
  *   - all state lives in an opaque dem_ctx (no process globals, no
  *     __constant__ symbols), so several contexts and batched instances can
  *     coexist;
- *   - particle state is SoA and permuted into cell order at every rebuild;
- *     AoS double3 arrays in original particle order appear only at the
- *     explicit import/export calls below;
+ *   - particle state is held as packed per-particle records (position and
+ *     radius; velocity, spin and path length) permuted into cell order at
+ *     every rebuild; double3 arrays in original particle order appear only
+ *     at the explicit import/export calls below;
  *   - every call returns a status code (DEM_OK = 0) instead of exit();
  *     dem_last_error() gives the message;
  *   - each context owns one HIP stream; calls on one context must come from
@@ -44,10 +45,10 @@ enum {
  * both modes. */
 enum { DEM_PREC_FP64 = 0, DEM_PREC_MIXED = 1 };
 
-/* Fused step kernel: AUTO picks the contact-compacting kernel for dry
- * packings whose lists have at most 64 slots, the one-thread-per-particle
- * loop otherwise.  Both give identical results. */
-enum { DEM_KERNEL_AUTO = 0, DEM_KERNEL_LOOP = 1, DEM_KERNEL_COMPACT = 2 };
+/* Fused step kernel: AUTO picks the per-wave contact-compacting kernel for
+ * dry packings whose lists have at most 64 slots, the one-thread-per-
+ * particle loop otherwise.  All variants give identical results. */
+enum { DEM_KERNEL_AUTO = 0, DEM_KERNEL_LOOP = 1, DEM_KERNEL_COMPACT = 2, DEM_KERNEL_WAVE = 3 };
 
 /* The heyi.in keywords (ReadData.cpp:75-188), SI units, plus overrides. */
 typedef struct dem_input {
@@ -225,8 +226,10 @@ int dem_step(dem_ctx *ctx, int nsteps);
  * stream.  Raises DEM_ERR_STATE if a rebuild would have been needed. */
 int dem_time_steps(dem_ctx *ctx, int nsteps, double *ms_per_step, int *rebuild_needed);
 int dem_sync(dem_ctx *ctx);
-/* DEM_KERNEL_* selection for dem_step; COMPACT is refused for wet
- * packings and lists longer than 64 slots. */
+/* DEM_KERNEL_* selection for dem_step; COMPACT (block-wide contact
+ * compaction) and WAVE (per-wave compaction) are refused for wet packings
+ * and lists longer than 64 slots.  AUTO takes WAVE where it applies, else
+ * LOOP. */
 int dem_set_kernel(dem_ctx *ctx, int variant);
 /* Storage order of the particles between rebuilds: 1 (default) = Morton
  * order of the cells, 0 = the reference's row-major cell hash.  Lists,
@@ -242,6 +245,10 @@ int dem_get_timing(dem_ctx *ctx, double *fused_ms, long long *fused_steps,
                    long long *fused_launches, double *rebuild_ms, int *n_rebuild);
 /* Bytes of device memory held by the context. */
 long long dem_device_bytes(dem_ctx *ctx);
+/* Name of the step kernel dem_step launches for this context's packing
+ * ("k_step_wave", "k_step_compact" or "k_step_fused"); a measurement aid
+ * with no reference counterpart. */
+const char *dem_step_kernel_name(dem_ctx *ctx);
 /* Measurement utility with no reference counterpart: a plain device copy
  * of nbytes, timed over iters launches with HIP events; *gbps = read +
  * write bytes per second / 1e9 (the attainable HBM bandwidth, and the

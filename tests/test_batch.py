@@ -1,0 +1,193 @@
+"""Batched multi-instance stepping and the packing analysis.
+
+GPU bars:
+  * a batch of uneven instances (dry and wet, mono- and polydisperse, with
+    rebuilds) steps every instance bit-identically to the same instance
+    stepped alone (same kernels, same per-instance rebuild steps);
+  * phi, Z and the bed top from the analysis kernel equal an independent
+    CPU evaluation (scipy pair search for Z, fine midpoint quadrature of the
+    sphere/core-region intersection for phi) within 1e-6 relative.
+The CPU reference of the intersection volume is itself pinned here against
+closed forms (full sphere, hemisphere, spherical cap, disc-disc lens limits).
+"""
+import math
+
+import numpy as np
+import pytest
+
+import gpucode_amd as G
+from gpucode_amd import workloads as W
+
+
+# --- CPU reference of the analysis ------------------------------------------
+def lens_area(s, Rc, d):
+    """Area of disc(radius s, centre at distance d) inside disc(Rc, origin)."""
+    s = np.asarray(s, float)
+    out = np.zeros_like(s)
+    inside = d <= np.abs(Rc - s)
+    out[inside] = np.pi * np.minimum(s[inside], Rc) ** 2
+    part = (~inside) & (d < s + Rc) & (s > 0)
+    sp = s[part]
+    c1 = np.clip((d * d + sp * sp - Rc * Rc) / (2 * d * sp), -1, 1)
+    c2 = np.clip((d * d + Rc * Rc - sp * sp) / (2 * d * Rc), -1, 1)
+    k = (-d + sp + Rc) * (d + sp - Rc) * (d - sp + Rc) * (d + sp + Rc)
+    out[part] = sp * sp * np.arccos(c1) + Rc * Rc * np.arccos(c2) - 0.5 * np.sqrt(np.maximum(k, 0))
+    return out
+
+
+def core_volume_ref(rho, z, r, Rc, zlo, zhi, nslice=20000):
+    a, b = max(z - r, zlo), min(z + r, zhi)
+    if a >= b or rho - r >= Rc:
+        return 0.0
+    t = a + (np.arange(nslice) + 0.5) * (b - a) / nslice
+    s = np.sqrt(np.maximum(r * r - (t - z) ** 2, 0.0))
+    return float(lens_area(s, Rc, rho).sum() * (b - a) / nslice)
+
+
+def analyze_ref(pos, rad, S, Tw):
+    from scipy.spatial import cKDTree
+
+    n = len(pos)
+    tree = cKDTree(pos)
+    pairs = np.array(sorted(tree.query_pairs(2 * rad.max())), dtype=np.int64).reshape(-1, 2)
+    if len(pairs):
+        d = np.linalg.norm(pos[pairs[:, 0]] - pos[pairs[:, 1]], axis=1)
+        pairs = pairs[d < rad[pairs[:, 0]] + rad[pairs[:, 1]]]
+    touch = np.zeros(n, bool)
+    touch[pairs.ravel()] = True
+    rho = np.hypot(pos[:, 0] - S.cirX, pos[:, 1] - S.cirY)
+    touch |= pos[:, 2] - S.Bw - rad < 0          # floor
+    touch |= (S.R - rho) - rad < 0                # lateral wall
+    touch |= (Tw - pos[:, 2]) - rad < 0           # lid
+    if not touch.any():
+        return 0.0, 2.0 * len(pairs) / n, -1.0e300
+    top = (pos[:, 2] + rad)[touch].max()
+    Rc, zlo, zhi = S.R - 2.0, S.Bw + 2.0, top - 2.0
+    vol = sum(core_volume_ref(rho[i], pos[i, 2], rad[i], Rc, zlo, zhi) for i in range(n))
+    phi = vol / (math.pi * Rc * Rc * (zhi - zlo))
+    return phi, 2.0 * len(pairs) / n, top
+
+
+def test_volume_reference_closed_forms():
+    r = 0.5
+    full = 4 / 3 * math.pi * r ** 3
+    assert abs(core_volume_ref(0.0, 5.0, r, 3.0, 0.0, 10.0) - full) < 1e-7 * full
+    # cut through the centre by the slab's lower face: a hemisphere
+    assert abs(core_volume_ref(0.0, 5.0, r, 3.0, 5.0, 10.0) - full / 2) < 1e-7 * full
+    # a cap of height h = 0.2: pi h^2 (3r - h) / 3
+    h = 0.2
+    cap = math.pi * h * h * (3 * r - h) / 3
+    assert abs(core_volume_ref(0.0, 5.0, r, 3.0, 5.0 + r - h, 10.0) - cap) < 1e-6 * cap
+    # outside the cylinder: nothing; centred on a huge cylinder's wall: about half
+    assert core_volume_ref(3.6, 5.0, r, 3.0, 0.0, 10.0) == 0.0
+    v = core_volume_ref(1000.0, 5.0, r, 1000.0, 0.0, 10.0)
+    assert abs(v - full / 2) < 1e-3 * full
+    # lens limits: disjoint, contained, and two equal discs at distance 0
+    assert lens_area(np.array([1.0]), 2.0, 3.5)[0] == 0.0
+    assert abs(lens_area(np.array([0.5]), 2.0, 1.0)[0] - math.pi * 0.25) < 1e-15
+    assert abs(lens_area(np.array([1.0]), 1.0, 0.0)[0] - math.pi) < 1e-15
+    # equal discs at distance s: 2 s^2 acos(1/2) - s^2 sqrt(3)/2
+    s = 1.0
+    want = 2 * s * s * math.acos(0.5) - 0.5 * s * s * math.sqrt(3.0)
+    assert abs(lens_area(np.array([s]), s, s)[0] - want) < 1e-14
+
+
+# --- GPU ----------------------------------------------------------------------
+def _instances():
+    rng = np.random.default_rng(11)
+    out = []
+    w = W.small_bed(n=2000, seed=3)
+    out.append((w, rng.normal(size=(2000, 3)) * 50.0))
+    w = W.small_bed(n=1500, seed=4, mu=0.2)
+    out.append((w, rng.normal(size=(1500, 3)) * 20.0))
+    w = W.wet_bed(n=1200, poly=True, reference_law=False, theta_deg=20.0, liquid=0.2)
+    out.append((w, None))
+    w = W.dry_column(n=2000)
+    out.append((w, None))
+    w = W.small_bed(n=300, R_d=4.0, seed=5)
+    out.append((w, rng.normal(size=(300, 3)) * 30.0))
+    return out
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", [G.PREC_FP64, G.PREC_MIXED])
+def test_batch_equals_separate_runs(precision):
+    cases = _instances()
+    b = G.Batch([w["inp"] for w, _ in cases], [w["dia"] for w, _ in cases], precision)
+    singles = []
+    for k, (w, vel) in enumerate(cases):
+        b[k].set_particles(w["pos"], vel)
+        e = G.Engine(w["inp"], w["dia"], precision)
+        e.set_particles(w["pos"], vel)
+        singles.append(e)
+    kernels = [e.step_kernel_name() for e in singles]
+    assert "k_step_wave" in kernels and "k_step_fused" in kernels
+    steps = 300
+    b.step(steps)
+    for e in singles:
+        e.step(steps)
+    for k, e in enumerate(singles):
+        print(f"instance {k}: {kernels[k]}, rebuilds {e.n_rebuild}")
+        assert b[k].iter == e.iter == steps
+        assert b[k].n_rebuild == e.n_rebuild
+        assert b[k].ctime == e.ctime
+        for u, v in zip(b[k].particles(), e.particles()):
+            assert np.array_equal(u, v), k
+        hb, he = b[k].hist_pp(), e.hist_pp()
+        for f in hb:
+            assert np.array_equal(hb[f], he[f]), (k, f)
+        assert np.array_equal(b[k].lqdcnt(), e.lqdcnt())
+    assert sum(e.n_rebuild for e in singles) > 2 * len(singles)
+    t = b.timing()
+    assert t["launches"] >= steps and t["n_rebuild"] >= len(singles)
+    # a second call continues from where the first stopped
+    b.step(50)
+    for e in singles:
+        e.step(50)
+    for k, e in enumerate(singles):
+        for u, v in zip(b[k].particles(), e.particles()):
+            assert np.array_equal(u, v), k
+    b.close()
+
+
+@pytest.mark.gpu
+def test_analysis_matches_cpu_reference():
+    # a bed settled for a while: contacts, wall contacts and a free top
+    w = W.small_bed(n=3000, seed=9)
+    e = G.Engine(w["inp"], w["dia"])
+    e.set_particles(w["pos"])
+    e.step(400)
+    phi, Z, top = e.analyze()
+    pos = e.particles()[0]
+    rad = np.full(len(pos), 0.5)
+    S = e.setup
+    phi_r, Z_r, top_r = analyze_ref(pos, rad, S, S.Tw)
+    print(phi, phi_r, Z, Z_r, top, top_r)
+    assert top == top_r
+    assert Z == Z_r
+    assert abs(phi - phi_r) <= 1e-6 * phi_r
+    assert 0.45 < phi < 0.75 and 3.0 < Z < 8.0
+
+
+@pytest.mark.gpu
+def test_batch_analysis_equals_single():
+    cases = _instances()[:3]
+    b = G.Batch([w["inp"] for w, _ in cases], [w["dia"] for w, _ in cases])
+    for k, (w, vel) in enumerate(cases):
+        b[k].set_particles(w["pos"])
+    b.step(100)
+    phi, Z, top = b.analyze()
+    for k in range(len(cases)):
+        p1, z1, t1 = b[k].analyze()
+        assert (phi[k], Z[k], top[k]) == (p1, z1, t1)
+    for k in (0, 2):
+        pos = b[k].particles()[0]
+        S = b[k].setup
+        _, rad, _, _ = G.derive(cases[k][0]["inp"], cases[k][0]["dia"])
+        phi_r, Z_r, top_r = analyze_ref(pos, rad, S, S.Tw)
+        assert Z[k] == Z_r and top[k] == top_r, k
+        assert abs(phi[k] - phi_r) <= 1e-6 * max(phi_r, 1e-300), k
+    # the polydisperse bed sits on lattice sites spaced for the largest
+    # size: after 100 steps nothing touches yet, so there is no bed top
+    assert top[2] == -1.0e300 and phi[2] == 0.0
+    b.close()

@@ -184,16 +184,17 @@ def test_fused_equals_separate_kernels():
     assert a.n_rebuild == b.n_rebuild
 
 
+@pytest.mark.parametrize("kernel", [G.Engine.KERNEL_COMPACT, G.Engine.KERNEL_WAVE])
 @pytest.mark.parametrize("precision", [G.PREC_FP64, G.PREC_MIXED])
-def test_compacting_kernel_equals_loop_kernel(precision):
-    # the compacting kernel sums each particle's contacts in slot order, so
-    # it must reproduce the one-thread loop bit for bit, rebuilds included
+def test_compacting_kernel_equals_loop_kernel(precision, kernel):
+    # the compacting kernels sum each particle's contacts in slot order, so
+    # they must reproduce the one-thread loop bit for bit, rebuilds included
     w = W.small_bed(n=3000)
     rng = np.random.default_rng(4)
     vel = rng.normal(size=(3000, 3)) * 50.0  # about 5 m/s: a rebuild every ~150 steps
     a = G.Engine(w["inp"], None, precision)
     b = G.Engine(w["inp"], None, precision)
-    a.set_kernel(G.Engine.KERNEL_COMPACT)
+    a.set_kernel(kernel)
     b.set_kernel(G.Engine.KERNEL_LOOP)
     for x in (a, b):
         x.set_particles(w["pos"], vel)

@@ -1,5 +1,5 @@
 """CPU tests of the engine's host side (no GPU calls): the library loads and
-exports every symbol include/dem_capi.h declares, the host derivation equals
+exports every symbol the headers in include/ declare, the host derivation equals
 the oracle's, and the feed / Tecplot / deck / restart-header interfaces
 behave as the reference's."""
 import ctypes as C
@@ -19,9 +19,14 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 def header_symbols():
-    txt = open(os.path.join(ROOT, "include", "dem_capi.h")).read()
-    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
-    return sorted(set(re.findall(r"\b(dem_[a-z0-9_]+)\s*\(", txt)))
+    names = set()
+    inc = os.path.join(ROOT, "include")
+    for h in sorted(os.listdir(inc)):
+        if h.endswith(".h"):
+            txt = open(os.path.join(inc, h)).read()
+            txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+            names |= set(re.findall(r"\b((?:dem|sur)_[a-z0-9_]+)\s*\(", txt))
+    return sorted(names)
 
 
 def test_library_exports_every_declared_symbol():
