@@ -69,6 +69,11 @@ class DemStats(C.Structure):
                                             "n_wall_slots", "n_wall_contact", "n_bridge")]
 
 
+class DemTuneResult(C.Structure):
+    _fields_ = [("cutGap", C.c_double), ("kernel", C.c_int), ("morton", C.c_int),
+                ("ms_per_step", C.c_double), ("ms_default", C.c_double), ("n_tried", C.c_int)]
+
+
 class DemError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__(f"{ERRORS.get(code, code)}: {msg}")
@@ -150,6 +155,9 @@ _SIGS = {
     "dem_batch_timing": (C.c_int, [P, D, C.POINTER(C.c_longlong), D, C.POINTER(C.c_longlong)]),
     "dem_batch_analyze": (C.c_int, [P, D, D, D]),
     "dem_analyze": (C.c_int, [P, D, D, D]),
+    # dem_tune.h
+    "dem_autotune": (C.c_int, [C.POINTER(DemInput), D, C.c_int, D, D, C.c_int, D, C.c_int, C.c_int,
+                               C.POINTER(DemTuneResult), D]),
     # dem_surrogate.h
     "sur_nparams": (C.c_int, []),
     "sur_create": (C.c_int, [C.c_ulonglong, C.POINTER(P)]),

@@ -25,6 +25,7 @@ SOURCES = [
     "DumpLoad.cpp",
     "dem_batch.cpp",
     "surrogate.hip",
+    "dem_tune.cpp",
 ]
 EXTRA_SOURCES = []  # appended by later modules (batched engine, surrogate)
 

@@ -21,6 +21,9 @@
 #include "dem_launch.h"
 
 static const int kBatchChunk = 64;
+// instances up to this size are rebuilt by the batched rebuild (one block
+// sorts an instance); larger ones by the multi-block single-instance path
+static const int kBatchRebuildMax = 1 << 16;
 
 struct dem_batch {
     std::vector<dem_ctx *> inst;
@@ -36,6 +39,9 @@ struct dem_batch {
     int ntask_wave = 0, ntask_loop = 0, task_cap = 0;
     std::vector<int> task_totals;                  // totals the task table was built for
     std::vector<int> task_kernel;
+    RbInst *d_rb = nullptr, *h_rb = nullptr;       // h_rb pinned
+    int2 *d_rtask = nullptr;                       // particle then cell tasks of a rebuild
+    long rtask_cap = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
     double step_ms = 0.0, rebuild_ms = 0.0;
     long long launches = 0, n_rebuild = 0;
@@ -50,6 +56,9 @@ static void batch_free(dem_batch *b) {
     (void)hipFree(b->d_out);
     (void)hipFree(b->d_stop);
     (void)hipFree(b->d_tasks);
+    (void)hipFree(b->d_rb);
+    (void)hipFree(b->d_rtask);
+    (void)hipHostFree(b->h_rb);
     (void)hipHostFree(b->h_tab);
     (void)hipHostFree(b->h_ana);
     (void)hipHostFree(b->h_out);
@@ -91,6 +100,8 @@ extern "C" int dem_batch_create(int n, const dem_input *in, const double *const 
         if (e == hipSuccess) e = hipHostMalloc((void **)&b->h_out, un * 3 * sizeof(double), 0);
         if (e == hipSuccess) e = hipHostMalloc((void **)&b->h_stop, un * sizeof(int), 0);
         if (e == hipSuccess) e = hipHostMalloc((void **)&b->h_ovf, un * sizeof(int), 0);
+        if (e == hipSuccess) e = hipMalloc((void **)&b->d_rb, un * sizeof(RbInst));
+        if (e == hipSuccess) e = hipHostMalloc((void **)&b->h_rb, un * sizeof(RbInst), 0);
         if (e == hipSuccess) e = hipEventCreate(&b->ev0);
         if (e == hipSuccess) e = hipEventCreate(&b->ev1);
         if (e == hipSuccess) e = hipEventCreate(&b->ev2);
@@ -169,24 +180,87 @@ static int build_tasks(dem_batch *b) {
     return DEM_OK;
 }
 
-// rebuilds every instance in need among those with steps left, then checks
+// table entry of a batched rebuild of c; performs the host side of the
+// rebuild (the buffer flips and bookkeeping of dem_do_rebuild)
+static void rebuild_entry(dem_ctx *c, int k, RbInst &R) {
+    dem_cnt_hstry_save(c);
+    c->old_total = c->saved ? c->old_total : 0;
+    memset(&R, 0, sizeof(R));
+    R.pa = c->P4[c->cur]; R.va = c->VW[c->cur]; R.lqa = c->lq[c->Lidx];
+    R.pb = c->P4[1 - c->cur]; R.vb = c->VW[1 - c->cur]; R.lqb = c->lq[1 - c->Lidx];
+    R.lvla = c->level[c->pcur]; R.ida = c->id[c->pcur];
+    R.lvlb = c->level[1 - c->pcur]; R.idb = c->id[1 - c->pcur];
+    R.keys = c->keys; R.vals = c->vals; R.keys2 = c->keys2; R.vals2 = c->vals2;
+    R.passes = (c->key_bits + 7) / 8 < 1 ? 1 : (c->key_bits + 7) / 8;
+    R.cstart = c->cstart; R.cend = c->cend; R.ncells = c->S.sysNCells;
+    const int o = 1 - c->hcur;
+    R.nbr = c->nbr[c->hcur]; R.nnb = c->nnb[c->hcur]; R.nbr_old = c->nbr[o]; R.nnb_old = c->nnb[o];
+    R.hp = c->hp[c->hcur]; R.hp_old = c->hp[o];
+    R.wl = c->wl[c->hcur]; R.nwl = c->nwl[c->hcur]; R.wl_old = c->wl[o]; R.nwl_old = c->nwl[o];
+    R.hw = c->hw[c->hcur]; R.hw_old = c->hw[o];
+    R.overflow = c->overflow;
+    R.cap = c->cap; R.total = c->total; R.old_total = c->saved ? c->old_total : 0;
+    R.inst = k;
+    R.Tw = c->Tw;
+    c->cur = 1 - c->cur;
+    c->Lidx = 1 - c->Lidx;
+    c->pcur = 1 - c->pcur;
+    c->saved = false;
+    c->has_lists = true;
+    c->old_total = c->total;
+    c->rebuild_pending = false;
+    c->n_rebuild++;
+    c->forces_valid = false;
+}
+
+// rebuilds every instance in need among those with steps left (the small
+// ones in one batched rebuild, the large ones one by one), then checks
 // their overflow flags with one synchronisation
 static int batch_rebuilds(dem_batch *b, const std::vector<int> &left) {
     const int n = (int)b->inst.size();
     std::vector<int> done;
+    std::vector<int2> pt, ct;
+    int nr = 0;
     HIPCHK(hipEventRecord(b->ev0, b->stream));
     for (int k = 0; k < n; ++k) {
         dem_ctx *c = b->inst[k];
         if (left[k] <= 0 || !c->rebuild_pending) continue;
-        int rc = dem_do_rebuild(c);
-        if (rc) return rc;
         done.push_back(k);
+        if (c->total > kBatchRebuildMax || c->total < 1) {
+            int rc = dem_do_rebuild(c);
+            if (rc) return rc;
+            continue;
+        }
+        rebuild_entry(c, k, b->h_rb[nr]);
+        for (int q = 0; q < (c->total + 255) / 256; ++q) pt.push_back(make_int2(nr, q));
+        for (int q = 0; q < (c->S.sysNCells + 255) / 256; ++q) ct.push_back(make_int2(nr, q));
+        ++nr;
     }
     if (done.empty()) return DEM_OK;
+    if (nr > 0) {
+        const long need = (long)(pt.size() + ct.size());
+        if (need > b->rtask_cap) {
+            (void)hipFree(b->d_rtask);
+            b->d_rtask = nullptr;
+            b->rtask_cap = 0;
+            HIPCHK(hipMalloc((void **)&b->d_rtask, (size_t)need * 2 * sizeof(int2)));
+            b->rtask_cap = need * 2;
+        }
+        std::vector<int2> all(pt);
+        all.insert(all.end(), ct.begin(), ct.end());
+        HIPCHK(hipMemcpyAsync(b->d_rb, b->h_rb, (size_t)nr * sizeof(RbInst), hipMemcpyHostToDevice,
+                              b->stream));
+        HIPCHK(hipMemcpyAsync(b->d_rtask, all.data(), all.size() * sizeof(int2), hipMemcpyHostToDevice,
+                              b->stream));
+        launch_rebuild_batch(b->d_inst, b->d_rb, nr, b->d_rtask, (int)pt.size(),
+                             b->d_rtask + pt.size(), (int)ct.size(), b->stream);
+        HIPCHK(hipGetLastError());
+    }
     HIPCHK(hipEventRecord(b->ev1, b->stream));
     for (size_t q = 0; q < done.size(); ++q)
         HIPCHK(hipMemcpyAsync(&b->h_ovf[q], b->inst[done[q]]->overflow, sizeof(int),
                               hipMemcpyDeviceToHost, b->stream));
+    // the host vectors copied above must outlive the copies: synchronise
     HIPCHK(hipStreamSynchronize(b->stream));
     float ms = 0.f;
     HIPCHK(hipEventElapsedTime(&ms, b->ev0, b->ev1));
@@ -209,12 +283,23 @@ extern "C" int dem_batch_step(dem_batch *b, int nsteps) {
     if ((rc = upload_insts(b))) return rc;
     std::vector<int> left(n, nsteps);
     const bool mixed = b->precision == DEM_PREC_MIXED;
+    // chunk length: an instance that requested a rebuild idles for the rest
+    // of its chunk, so the chunk follows the run lengths seen last time
+    // (the 3rd quartile over the running instances, between 8 and 64);
+    // chunking does not change any instance's sequence of steps
+    std::vector<int> last_run(n, kBatchChunk);
     for (;;) {
         bool any = false;
         for (int k = 0; k < n; ++k) any = any || left[k] > 0;
         if (!any) break;
         if ((rc = batch_rebuilds(b, left))) return rc;
         if ((rc = build_tasks(b))) return rc;
+        std::vector<int> runs;
+        for (int k = 0; k < n; ++k)
+            if (left[k] > 0) runs.push_back(last_run[k]);
+        std::sort(runs.begin(), runs.end());
+        int chunk = runs.empty() ? kBatchChunk : runs[(runs.size() * 3) / 4];
+        chunk = chunk < 8 ? 8 : (chunk > kBatchChunk ? kBatchChunk : chunk);
         int S = 0;
         for (int k = 0; k < n; ++k) {
             dem_ctx *c = b->inst[k];
@@ -229,7 +314,7 @@ extern "C" int dem_batch_step(dem_batch *b, int nsteps) {
             }
             T.cur = c->cur;
             T.Lidx = c->Lidx;
-            T.limit = left[k] < kBatchChunk ? (left[k] > 0 ? left[k] : 0) : kBatchChunk;
+            T.limit = left[k] < chunk ? (left[k] > 0 ? left[k] : 0) : chunk;
             T.Tw0 = c->Tw;
             T.dTw = c->S.topv * c->S.dt;
             S = T.limit > S ? T.limit : S;
@@ -259,6 +344,7 @@ extern "C" int dem_batch_step(dem_batch *b, int nsteps) {
             const int ex = dem_finish_chunk(c, lim, b->h_stop[k]);
             c->forces_valid = c->store_forces != 0 && ex > 0;
             left[k] -= ex;
+            last_run[k] = c->rebuild_pending ? ex + 1 : 2 * lim;
         }
     }
     return DEM_OK;

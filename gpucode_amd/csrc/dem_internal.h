@@ -111,3 +111,26 @@ struct AnaArgs {
     int cap, total;
     double Tw;
 };
+
+// One instance of a batched rebuild: the buffers before (a) and after (b)
+// the permutation into cell order, the sort arrays, the cell tables, and
+// the new and old lists and histories.
+struct RbInst {
+    const double4 *pa, *va;
+    const int *lqa, *lvla, *ida;
+    double4 *pb, *vb;
+    int *lqb, *lvlb, *idb;
+    unsigned *keys, *vals, *keys2, *vals2;
+    int *cstart, *cend;
+    int *nbr, *nnb;
+    const int *nbr_old, *nnb_old;
+    HistRec *hp;
+    const HistRec *hp_old;
+    int *wl, *nwl;
+    const int *wl_old, *nwl_old;
+    HistRec *hw;
+    const HistRec *hw_old;
+    int *overflow;
+    int cap, total, old_total, ncells, passes, inst;
+    double Tw;
+};

@@ -283,15 +283,20 @@ DEV void cell_of(const DevInst &P, const double4 q, int L, int &hash, unsigned &
                    : (unsigned)hash;
 }
 
-__global__ void k_calc_hash(const double4 *p, const int *level, const int *id, DevInst P,
-                            unsigned *keys, unsigned *vals, int total) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= total) return;
+DEV void calc_hash_one(const DevInst &P, const double4 *p, const int *level, const int *id,
+                       unsigned *keys, unsigned *vals, int i) {
     int h;
     unsigned key;
     cell_of(P, p[i], level[i], h, key);
     keys[id[i]] = key;
     vals[id[i]] = (unsigned)i;
+}
+
+__global__ void k_calc_hash(const double4 *p, const int *level, const int *id, DevInst P,
+                            unsigned *keys, unsigned *vals, int total) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    calc_hash_one(P, p, level, id, keys, vals, i);
 }
 
 void launch_calc_hash(const StateBuf &s, const int *level, const int *id, const DevInst &P,
@@ -304,10 +309,8 @@ void launch_calc_hash(const StateBuf &s, const int *level, const int *id, const 
 // ---------------------------------------------------------------------
 // permutation into cell order; zeroes the path length (main.cpp:492)
 // ---------------------------------------------------------------------
-__global__ void k_permute(const unsigned *perm, StateBuf a, StateBuf b, const int *lvl_a,
-                          int *lvl_b, const int *id_a, int *id_b, int total) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= total) return;
+DEV void permute_one(const unsigned *perm, const StateBuf &a, const StateBuf &b, const int *lvl_a,
+                     int *lvl_b, const int *id_a, int *id_b, int i) {
     int o = (int)perm[i];
     b.p[i] = a.p[o];
     b.v[2 * i] = a.v[2 * o];
@@ -317,6 +320,13 @@ __global__ void k_permute(const unsigned *perm, StateBuf a, StateBuf b, const in
     b.lqd[i] = a.lqd[o];
     lvl_b[i] = lvl_a[o];
     id_b[i] = id_a[o];
+}
+
+__global__ void k_permute(const unsigned *perm, StateBuf a, StateBuf b, const int *lvl_a,
+                          int *lvl_b, const int *id_a, int *id_b, int total) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    permute_one(perm, a, b, lvl_a, lvl_b, id_a, id_b, i);
 }
 
 void launch_permute(const unsigned *perm, const StateBuf &a, const StateBuf &b, const int *lvl_a,
@@ -331,10 +341,8 @@ void launch_permute(const unsigned *perm, const StateBuf &a, const StateBuf &b, 
 // ---------------------------------------------------------------------
 // CellStart/CellEnd indexed by the row-major cell hash; a cell's particles
 // are contiguous in storage in either order (equal sort keys = same cell)
-__global__ void k_cell_range(const unsigned *keys, const double4 *p, const int *level, DevInst P,
-                             int *cstart, int *cend, int total) {
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= total) return;
+DEV void cell_range_one(const unsigned *keys, const double4 *p, const int *level, const DevInst &P,
+                        int *cstart, int *cend, int total, int k) {
     const unsigned key = keys[k];
     const bool first = k == 0 || key != keys[k - 1];
     const bool last = k == total - 1 || key != keys[k + 1];
@@ -344,6 +352,13 @@ __global__ void k_cell_range(const unsigned *keys, const double4 *p, const int *
     cell_of(P, p[k], level[k], h, kk);
     if (first) cstart[h] = k;
     if (last) cend[h] = k + 1;
+}
+
+__global__ void k_cell_range(const unsigned *keys, const double4 *p, const int *level, DevInst P,
+                             int *cstart, int *cend, int total) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= total) return;
+    cell_range_one(keys, p, level, P, cstart, cend, total, k);
 }
 
 void launch_cell_range(const unsigned *keys, const StateBuf &s, const int *level, const DevInst &P,
@@ -361,12 +376,8 @@ void launch_cell_range(const unsigned *keys, const StateBuf &s, const int *level
 // The file is compiled with -ffp-contract=off, so the distance test rounds
 // like the host restatement.
 // ---------------------------------------------------------------------
-__global__ void __launch_bounds__(128) k_neighbor_find(const double4 *p, const int *level,
-                                                       DevInst P, const int *cstart,
-                                                       const int *cend, int *nbr, int *nnb,
-                                                       int *overflow, int cap, int total) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= total) return;
+DEV void neighbor_find_one(const double4 *p, const int *level, const DevInst &P, const int *cstart,
+                           const int *cend, int *nbr, int *nnb, int *overflow, int cap, int i) {
     const double4 o = p[i];
     const double ri = o.w;
     const int iL = level[i];
@@ -417,6 +428,15 @@ __global__ void __launch_bounds__(128) k_neighbor_find(const double4 *p, const i
     nnb[i] = num;
 }
 
+__global__ void __launch_bounds__(128) k_neighbor_find(const double4 *p, const int *level,
+                                                       DevInst P, const int *cstart,
+                                                       const int *cend, int *nbr, int *nnb,
+                                                       int *overflow, int cap, int total) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    neighbor_find_one(p, level, P, cstart, cend, nbr, nnb, overflow, cap, i);
+}
+
 void launch_neighbor_find(const StateBuf &s, const int *level, const DevInst &P, const int *cstart,
                           const int *cend, int *nbr, int *nnb, int *overflow, int cap, int total,
                           hipStream_t st) {
@@ -428,10 +448,8 @@ void launch_neighbor_find(const StateBuf &s, const int *level, const DevInst &P,
 // ---------------------------------------------------------------------
 // cylinder wall candidates (kernel.cu:1034-1073)
 // ---------------------------------------------------------------------
-__global__ void k_wall_list(const double4 *p, DevInst P, double Tw, int *wl, int *nwl, int cap,
-                            int total) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= total) return;
+DEV void wall_list_one(const double4 *p, const DevInst &P, double Tw, int *wl, int *nwl, int cap,
+                       int i) {
     const double4 q = p[i];
     const double r = q.w;
     int n = 0;
@@ -441,6 +459,13 @@ __global__ void k_wall_list(const double4 *p, DevInst P, double Tw, int *wl, int
     double dc = sqrt(ax * ax + ay * ay);
     if (dc >= P.R - (2.0 * r + P.searchGap)) wl[(long)(n++) * cap + i] = 2;
     nwl[i] = n;
+}
+
+__global__ void k_wall_list(const double4 *p, DevInst P, double Tw, int *wl, int *nwl, int cap,
+                            int total) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    wall_list_one(p, P, Tw, wl, nwl, cap, i);
 }
 
 void launch_wall_list(const StateBuf &s, const DevInst &P, double Tw, int *wl, int *nwl, int cap,
@@ -464,12 +489,9 @@ DEV HistRec hist_zero() {
     return r;
 }
 
-__global__ void k_hist_remap(const unsigned *perm, const int *id_new, const int *id_old,
-                             const int *nbr, const int *nnb, const int *nbr_old,
-                             const int *nnb_old, HistRec *h, const HistRec *h_old, int cap,
-                             int total, int old_total) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= total) return;
+DEV void hist_remap_one(const unsigned *perm, const int *id_new, const int *id_old, const int *nbr,
+                        const int *nnb, const int *nbr_old, const int *nnb_old, HistRec *h,
+                        const HistRec *h_old, int cap, int old_total, int i) {
     int p = (int)perm[i];
     int nold = p < old_total ? nnb_old[p] : 0;
     int n = nnb[i];
@@ -483,11 +505,18 @@ __global__ void k_hist_remap(const unsigned *perm, const int *id_new, const int 
     }
 }
 
-__global__ void k_hist_remap_pw(const unsigned *perm, const int *wl, const int *nwl,
-                                const int *wl_old, const int *nwl_old, HistRec *h,
-                                const HistRec *h_old, int cap, int total, int old_total) {
+__global__ void k_hist_remap(const unsigned *perm, const int *id_new, const int *id_old,
+                             const int *nbr, const int *nnb, const int *nbr_old,
+                             const int *nnb_old, HistRec *h, const HistRec *h_old, int cap,
+                             int total, int old_total) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= total) return;
+    hist_remap_one(perm, id_new, id_old, nbr, nnb, nbr_old, nnb_old, h, h_old, cap, old_total, i);
+}
+
+DEV void hist_remap_pw_one(const unsigned *perm, const int *wl, const int *nwl, const int *wl_old,
+                           const int *nwl_old, HistRec *h, const HistRec *h_old, int cap,
+                           int old_total, int i) {
     int p = (int)perm[i];
     int nold = p < old_total ? nwl_old[p] : 0;
     for (int a = 0; a < nwl[i]; ++a) {
@@ -497,6 +526,14 @@ __global__ void k_hist_remap_pw(const unsigned *perm, const int *wl, const int *
             if (wl_old[(long)b * cap + p] == w) hit = b;
         h[sa] = hit >= 0 ? h_old[(long)hit * cap + p] : hist_zero();
     }
+}
+
+__global__ void k_hist_remap_pw(const unsigned *perm, const int *wl, const int *nwl,
+                                const int *wl_old, const int *nwl_old, HistRec *h,
+                                const HistRec *h_old, int cap, int total, int old_total) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    hist_remap_pw_one(perm, wl, nwl, wl_old, nwl_old, h, h_old, cap, old_total, i);
 }
 
 void launch_hist_remap(const unsigned *perm, const int *id_new, const int *id_old, const int *nbr,
@@ -1643,6 +1680,160 @@ __global__ void __launch_bounds__(256) k_analyze(const DevInst *insts, const Ana
 
 void launch_analyze(const DevInst *insts, const AnaArgs *args, double *out, int n, hipStream_t s) {
     if (n > 0) hipLaunchKernelGGL(k_analyze, dim3(n), dim3(256), 0, s, insts, args, out);
+}
+
+// ---------------------------------------------------------------------
+// Batched rebuild: every stage of the rebuild (hash, sort, permutation,
+// cell ranges, neighbour and wall lists, history remap) runs as one launch
+// over all instances due for a rebuild.  Particle stages take tasks
+// (table entry, block of 256); the sort gives each instance one block.
+// Each stage calls the device function of the single-instance kernel, so
+// the lists and histories are identical to a separate rebuild.
+// ---------------------------------------------------------------------
+#define RB_BS 256
+#define BSORT_BS 1024
+
+__global__ void __launch_bounds__(RB_BS) kb_calc_hash(const DevInst *insts, const RbInst *tab,
+                                                      const int2 *tasks) {
+    const int2 tk = tasks[blockIdx.x];
+    const RbInst &R = tab[tk.x];
+    const int i = tk.y * RB_BS + threadIdx.x;
+    if (i < R.total) calc_hash_one(insts[R.inst], R.pa, R.lvla, R.ida, R.keys, R.vals, i);
+}
+
+// stable LSD radix sort of one instance's (key, value) pairs by one block:
+// per pass a digit histogram, its prefix sum, and a tile-by-tile stable
+// scatter (ranks within a tile from wave ballots, as k_rs_scatter)
+__global__ void __launch_bounds__(BSORT_BS) kb_sort(const RbInst *tab) {
+    const RbInst &R = tab[blockIdx.x];
+    const int n = R.total, tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    __shared__ unsigned base[256];
+    __shared__ unsigned wcnt[BSORT_BS / 64][256];
+    unsigned *ka = R.keys, *va = R.vals, *kb = R.keys2, *vb = R.vals2;
+    const unsigned long long lt = (1ull << lane) - 1ull;
+    if (n <= 1) return;
+    for (int pass = 0; pass < R.passes; ++pass) {
+        const int shift = 8 * pass;
+        if (tid < 256) base[tid] = 0;
+        __syncthreads();
+        for (int e = tid; e < n; e += BSORT_BS) atomicAdd(&base[(ka[e] >> shift) & 255u], 1u);
+        __syncthreads();
+        if (tid < 64) {
+            const unsigned c0 = base[4 * tid], c1 = base[4 * tid + 1], c2 = base[4 * tid + 2],
+                           c3 = base[4 * tid + 3];
+            const unsigned sum = c0 + c1 + c2 + c3;
+            const unsigned ex = wave_incl_scan(sum) - sum;
+            base[4 * tid] = ex;
+            base[4 * tid + 1] = ex + c0;
+            base[4 * tid + 2] = ex + c0 + c1;
+            base[4 * tid + 3] = ex + c0 + c1 + c2;
+        }
+        __syncthreads();
+        for (int t0 = 0; t0 < n; t0 += BSORT_BS) {
+            for (int q = tid; q < (BSORT_BS / 64) * 256; q += BSORT_BS) (&wcnt[0][0])[q] = 0;
+            __syncthreads();
+            const int e = t0 + tid;
+            const bool valid = e < n;
+            unsigned key = 0, val = 0, d = 256u;
+            if (valid) {
+                key = ka[e];
+                val = va[e];
+                d = (key >> shift) & 255u;
+            }
+            const unsigned long long peers = wave_match9(d);
+            const unsigned rank = (unsigned)__popcll(peers & lt);
+            if (valid && (peers & lt) == 0ull) wcnt[w][d] = (unsigned)__popcll(peers);
+            __syncthreads();
+            if (valid) {
+                unsigned pos = base[d] + rank;
+                for (int q = 0; q < w; ++q) pos += wcnt[q][d];
+                kb[pos] = key;
+                vb[pos] = val;
+            }
+            __syncthreads();
+            if (tid < 256) {
+                unsigned add = 0;
+                for (int q = 0; q < BSORT_BS / 64; ++q) add += wcnt[q][tid];
+                base[tid] += add;
+            }
+            __syncthreads();
+        }
+        unsigned *t = ka; ka = kb; kb = t;
+        t = va; va = vb; vb = t;
+    }
+    if (ka != R.keys) {
+        for (int e = tid; e < n; e += BSORT_BS) {
+            R.keys[e] = ka[e];
+            R.vals[e] = va[e];
+        }
+    }
+}
+
+__global__ void __launch_bounds__(RB_BS) kb_permute(const RbInst *tab, const int2 *tasks) {
+    const int2 tk = tasks[blockIdx.x];
+    const RbInst &R = tab[tk.x];
+    const int i = tk.y * RB_BS + threadIdx.x;
+    if (i >= R.total) return;
+    StateBuf a, b;
+    a.p = (double4 *)R.pa; a.v = (double4 *)R.va; a.lqd = (int *)R.lqa;
+    b.p = R.pb; b.v = R.vb; b.lqd = R.lqb;
+    permute_one(R.vals, a, b, R.lvla, R.lvlb, R.ida, R.idb, i);
+}
+
+// empty cells and a cleared overflow flag; tasks over blocks of cells
+__global__ void __launch_bounds__(RB_BS) kb_fill_cells(const RbInst *tab, const int2 *tasks) {
+    const int2 tk = tasks[blockIdx.x];
+    const RbInst &R = tab[tk.x];
+    const int c = tk.y * RB_BS + threadIdx.x;
+    if (c < R.ncells) {
+        R.cstart[c] = DEM_EMPTY;
+        R.cend[c] = DEM_EMPTY;
+    }
+    if (tk.y == 0 && threadIdx.x == 0) *R.overflow = 0;
+}
+
+__global__ void __launch_bounds__(RB_BS) kb_cell_range(const DevInst *insts, const RbInst *tab,
+                                                       const int2 *tasks) {
+    const int2 tk = tasks[blockIdx.x];
+    const RbInst &R = tab[tk.x];
+    const int k = tk.y * RB_BS + threadIdx.x;
+    if (k < R.total) cell_range_one(R.keys, R.pb, R.lvlb, insts[R.inst], R.cstart, R.cend, R.total, k);
+}
+
+__global__ void __launch_bounds__(RB_BS) kb_lists(const DevInst *insts, const RbInst *tab,
+                                                  const int2 *tasks) {
+    const int2 tk = tasks[blockIdx.x];
+    const RbInst &R = tab[tk.x];
+    const int i = tk.y * RB_BS + threadIdx.x;
+    if (i >= R.total) return;
+    const DevInst &P = insts[R.inst];
+    neighbor_find_one(R.pb, R.lvlb, P, R.cstart, R.cend, R.nbr, R.nnb, R.overflow, R.cap, i);
+    wall_list_one(R.pb, P, R.Tw, R.wl, R.nwl, R.cap, i);
+}
+
+__global__ void __launch_bounds__(RB_BS) kb_hist_remap(const RbInst *tab, const int2 *tasks) {
+    const int2 tk = tasks[blockIdx.x];
+    const RbInst &R = tab[tk.x];
+    const int i = tk.y * RB_BS + threadIdx.x;
+    if (i >= R.total) return;
+    hist_remap_one(R.vals, R.idb, R.ida, R.nbr, R.nnb, R.nbr_old, R.nnb_old, R.hp, R.hp_old, R.cap,
+                   R.old_total, i);
+    hist_remap_pw_one(R.vals, R.wl, R.nwl, R.wl_old, R.nwl_old, R.hw, R.hw_old, R.cap, R.old_total, i);
+}
+
+void launch_rebuild_batch(const DevInst *insts, const RbInst *tab, int ninst, const int2 *ptasks,
+                          int nptasks, const int2 *ctasks, int nctasks, hipStream_t s) {
+    if (ninst <= 0) return;
+    if (nptasks > 0)
+        hipLaunchKernelGGL(kb_calc_hash, dim3(nptasks), dim3(RB_BS), 0, s, insts, tab, ptasks);
+    hipLaunchKernelGGL(kb_sort, dim3(ninst), dim3(BSORT_BS), 0, s, tab);
+    if (nptasks > 0) hipLaunchKernelGGL(kb_permute, dim3(nptasks), dim3(RB_BS), 0, s, tab, ptasks);
+    if (nctasks > 0) hipLaunchKernelGGL(kb_fill_cells, dim3(nctasks), dim3(RB_BS), 0, s, tab, ctasks);
+    if (nptasks > 0) {
+        hipLaunchKernelGGL(kb_cell_range, dim3(nptasks), dim3(RB_BS), 0, s, insts, tab, ptasks);
+        hipLaunchKernelGGL(kb_lists, dim3(nptasks), dim3(RB_BS), 0, s, insts, tab, ptasks);
+        hipLaunchKernelGGL(kb_hist_remap, dim3(nptasks), dim3(RB_BS), 0, s, tab, ptasks);
+    }
 }
 
 // --- the reference's separate kernels (PpInterCalc, CirPWCalc,

@@ -35,6 +35,8 @@ void launch_step_fused(const DevInst &P, const StepArgs &A, bool wet, bool mixed
 void launch_step_compact(const DevInst &P, const StepArgs &A, int K, bool mixed, hipStream_t s);
 void launch_step_wave(const DevInst &P, const StepArgs &A, bool mixed, hipStream_t s);
 void launch_analyze(const DevInst *insts, const AnaArgs *args, double *out, int n, hipStream_t s);
+void launch_rebuild_batch(const DevInst *insts, const RbInst *tab, int ninst, const int2 *ptasks,
+                          int nptasks, const int2 *ctasks, int nctasks, hipStream_t s);
 void launch_step_batch(const DevInst *insts, const BatchInst *tab, const int2 *tasks, int ntasks,
                        bool wave, bool mixed, int step, hipStream_t s);
 void launch_pp_force(const DevInst &P, const StepArgs &A, bool wet, bool mixed, hipStream_t s);

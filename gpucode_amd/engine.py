@@ -54,6 +54,22 @@ def stream_copy(nbytes=1 << 30, iters=20):
     return g.value
 
 
+def autotune(inp, pos, dia=None, vel=None, steps=200, cut_gaps=None, precision=A.PREC_FP64):
+    """Times skins x step kernels x storage orders on this packing
+    (dem_autotune); returns (best as a dict, table [skin][LOOP/COMPACT/WAVE]
+    [row-major/Morton] of ms per step, -1 where skipped)."""
+    pos = np.ascontiguousarray(pos, np.float64)
+    v = None if vel is None else np.ascontiguousarray(vel, np.float64)
+    d = None if dia is None else np.ascontiguousarray(dia, np.float64)
+    cg = None if cut_gaps is None else np.ascontiguousarray(cut_gaps, np.float64)
+    ncut = 5 if cg is None else len(cg)
+    table = np.zeros((ncut, 3, 2))
+    r = A.DemTuneResult()
+    check(A.lib().dem_autotune(C.byref(inp), dptr(d), len(pos), dptr(pos), dptr(v), int(steps),
+                               dptr(cg), ncut, int(precision), C.byref(r), dptr(table)))
+    return {f: getattr(r, f) for f, _ in A.DemTuneResult._fields_}, table
+
+
 def load_para(path):
     v = [C.c_int(), C.c_int(), C.c_int()]
     check(A.lib().dem_load_para(str(path).encode(), *[C.byref(x) for x in v]))

@@ -191,3 +191,42 @@ def test_batch_analysis_equals_single():
     # size: after 100 steps nothing touches yet, so there is no bed top
     assert top[2] == -1.0e300 and phi[2] == 0.0
     b.close()
+
+
+@pytest.mark.gpu
+def test_autotune_times_candidates_and_keeps_the_physics():
+    w = W.small_bed(n=2000, seed=13)
+    rng = np.random.default_rng(2)
+    vel = rng.normal(size=(2000, 3)) * 20.0
+    best, table = G.autotune(w["inp"], w["pos"], vel=vel, steps=100, cut_gaps=[0.1, 0.3])
+    print(best, table)
+    assert best["n_tried"] == 12 and (table > 0).all()
+    k = {1: 0, 2: 1, 3: 2}[best["kernel"]]
+    assert table.min() == best["ms_per_step"] == table[[0.1, 0.3].index(best["cutGap"]), k,
+                                                       best["morton"]]
+    assert best["ms_default"] > 0
+    # the tuned configuration steps the same physics (to rounding: the cell
+    # size and list contents change the summation order)
+    runs = []
+    for cg, kern, mo in ((0.1, G.Engine.KERNEL_AUTO, True),
+                         (best["cutGap"], best["kernel"], bool(best["morton"]))):
+        w["inp"].cutGap = cg
+        e = G.Engine(w["inp"])
+        e.set_kernel(kern)
+        e.set_order(mo)
+        e.set_particles(w["pos"], vel)
+        e.step(100)
+        runs.append(e.particles()[0])
+    w["inp"].cutGap = 0.1
+    assert np.abs(runs[0] - runs[1]).max() < 1e-6
+
+
+@pytest.mark.gpu
+def test_autotune_skips_skins_shorter_than_capillary_bridges():
+    w = W.wet_bed(n=1500, liquid=0.05)
+    best, table = G.autotune(w["inp"], w["pos"], w["dia"], steps=50, cut_gaps=[0.01, 0.5])
+    # rupture distance (1 + 0) * 0.99 * (2 * 0.05 / 12)^0.33 ~ 0.2 d: the 0.01 skin is out
+    assert (table[0] == -1).all()
+    # wet packings run the one-thread loop only
+    assert (table[1, 0] > 0).all() and (table[1, 1:] == -1).all()
+    assert best["cutGap"] == 0.5 and best["kernel"] == G.Engine.KERNEL_LOOP
