@@ -98,7 +98,8 @@ def sweep_main(args):
         "speedup_vs_plain": surr_ph / plain_ph,
         "speedup_vs_plain_incl_training": surr_all_ph / plain_ph,
         "batched_speedup_vs_plain": batched_ph / plain_ph,
-        "phi_err": {"mean": r["phi_err_mean"], "p95": r["phi_err_p95"], "max": r["phi_err_max"],
+        "phi_err": {"mean": r["phi_err_mean"], "median": r["phi_err_median"],
+                    "p95": r["phi_err_p95"], "max": r["phi_err_max"],
                     "top": r["phi_err_top"], "random": r["phi_err_random"]},
         "z_err_mean": r["z_err_mean"],
         "times_s": {k: r[k] for k in ("t_gen", "t_train", "t_screen", "t_rerun", "t_plain_mean")},

@@ -161,7 +161,8 @@ static int alloc_all(dem_ctx *c) {
             (rc = dalloc(c, &c->lq[b], cap)) || (rc = dalloc(c, &c->level[b], cap)) ||
             (rc = dalloc(c, &c->id[b], cap)) || (rc = dalloc(c, &c->nbr[b], cap * K)) ||
             (rc = dalloc(c, &c->nnb[b], cap)) || (rc = dalloc(c, &c->wl[b], cap * DEM_WALLS)) ||
-            (rc = dalloc(c, &c->nwl[b], cap)) || (rc = dalloc(c, &c->hp[b], cap * K)) ||
+            (rc = dalloc(c, &c->nwl[b], cap)) || (rc = dalloc(c, &c->hp[b], 2 * cap * K)) ||
+            (rc = dalloc(c, &c->amask[b], cap)) || (rc = dalloc(c, &c->actoff[b], cap + 1)) ||
             (rc = dalloc(c, &c->hw[b], cap * DEM_WALLS)))
             return rc;
     }
@@ -186,6 +187,7 @@ static void free_all(dem_ctx *c) {
         (void)hipFree(c->id[b]); (void)hipFree(c->nbr[b]); (void)hipFree(c->nnb[b]);
         (void)hipFree(c->wl[b]); (void)hipFree(c->nwl[b]);
         (void)hipFree(c->hp[b]); (void)hipFree(c->hw[b]);
+        (void)hipFree(c->amask[b]); (void)hipFree(c->actoff[b]);
     }
     (void)hipFree(c->cstart); (void)hipFree(c->cend);
     (void)hipFree(c->keys); (void)hipFree(c->vals); (void)hipFree(c->keys2); (void)hipFree(c->vals2);

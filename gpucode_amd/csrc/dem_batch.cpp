@@ -196,6 +196,9 @@ static void rebuild_entry(dem_ctx *c, int k, RbInst &R) {
     const int o = 1 - c->hcur;
     R.nbr = c->nbr[c->hcur]; R.nnb = c->nnb[c->hcur]; R.nbr_old = c->nbr[o]; R.nnb_old = c->nnb[o];
     R.hp = c->hp[c->hcur]; R.hp_old = c->hp[o];
+    R.amask = c->amask[c->hcur]; R.amask_old = c->amask[o];
+    R.actoff = c->actoff[c->hcur]; R.actoff_old = c->actoff[o];
+    R.K = c->K;
     R.wl = c->wl[c->hcur]; R.nwl = c->nwl[c->hcur]; R.wl_old = c->wl[o]; R.nwl_old = c->nwl[o];
     R.hw = c->hw[c->hcur]; R.hw_old = c->hw[o];
     R.overflow = c->overflow;

@@ -30,6 +30,8 @@ StepArgs dem_step_args(dem_ctx *c, bool fused) {
     A.nbr = c->nbr[c->hcur];
     A.nnb = c->nnb[c->hcur];
     A.h = c->hp[c->hcur];
+    A.amask = c->amask[c->hcur];
+    A.actoff = c->actoff[c->hcur];
     A.wl = c->wl[c->hcur];
     A.nwl = c->nwl[c->hcur];
     A.hw = c->hw[c->hcur];
@@ -279,11 +281,29 @@ static int get_hist(dem_ctx *c, const HistRec *h, const int *dcnt, int slots, in
     return DEM_OK;
 }
 
+// a slot-major copy of the history (slot k of particle i at k*cap + i), for
+// export and import
+static int slot_view(dem_ctx *c, HistRec **out) {
+    *out = nullptr;
+    HIPCHK(hipMalloc((void **)out, (size_t)c->cap * c->K * sizeof(HistRec)));
+    HIPCHK(hipMemsetAsync(*out, 0, (size_t)c->cap * c->K * sizeof(HistRec), c->stream));
+    return DEM_OK;
+}
+
 extern "C" int dem_get_hist_pp(dem_ctx *c, int maxK, double *dispt, double *nDsp, double *nDsp_mx,
                                double *plstRad, double *plstDfm, int *stage, int *lqdBrg) {
     if (!c || maxK <= 0) return dem_fail(DEM_ERR_ARG, "bad argument");
-    return get_hist(c, c->hp[c->hcur], c->nnb[c->hcur], c->K, maxK, dispt, nDsp, nDsp_mx, plstRad,
-                    plstDfm, stage, lqdBrg);
+    HistRec *v = nullptr;
+    int rc = slot_view(c, &v);
+    if (rc == DEM_OK) {
+        launch_hist_slots(c->hp[c->hcur], c->nnb[c->hcur], c->amask[c->hcur], c->actoff[c->hcur], v,
+                          c->cap, (long)c->K * c->cap, c->total, true, c->stream);
+        rc = get_hist(c, v, c->nnb[c->hcur], c->K, maxK, dispt, nDsp, nDsp_mx, plstRad, plstDfm, stage,
+                      lqdBrg);
+    }
+    (void)hipStreamSynchronize(c->stream);
+    (void)hipFree(v);
+    return rc;
 }
 
 extern "C" int dem_get_walls(dem_ctx *c, int *num, int *list) {
@@ -377,8 +397,18 @@ extern "C" int dem_set_contacts(dem_ctx *c, int maxK, const int *num, const int 
     }
     HIPCHK(hipMemcpyAsync(c->nwl[c->hcur], wc.data(), n * sizeof(int), hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
-    if ((rc = put_hist(c, c->hp[c->hcur], maxK, n, dispt, nDsp, nDsp_mx, plstRad, plstDfm, stage, lqdBrg)))
-        return rc;
+    // restored history goes to the slot-major region (no compact records)
+    launch_fill_i32((int *)c->amask[c->hcur], 0, 2L * n, c->stream);
+    launch_fill_i32(c->actoff[c->hcur], 0, n, c->stream);
+    HistRec *v = nullptr;
+    if ((rc = slot_view(c, &v))) return rc;
+    rc = put_hist(c, v, maxK, n, dispt, nDsp, nDsp_mx, plstRad, plstDfm, stage, lqdBrg);
+    if (rc == DEM_OK)
+        launch_hist_slots(c->hp[c->hcur], c->nnb[c->hcur], c->amask[c->hcur], c->actoff[c->hcur], v,
+                          c->cap, (long)c->K * c->cap, n, false, c->stream);
+    (void)hipStreamSynchronize(c->stream);
+    (void)hipFree(v);
+    if (rc) return rc;
     if ((rc = put_hist(c, c->hw[c->hcur], DEM_WALLS, n, wdispt, wnDsp, wnDsp_mx, wplstRad, wplstDfm,
                        wstage, wlqdBrg)))
         return rc;
@@ -458,6 +488,8 @@ extern "C" int dem_bld_nebor_list(dem_ctx *c) {
     launch_fill_i32(c->overflow, 0, 1, s);
     launch_neighbor_find(c->buf(c->cur, c->Lidx), c->level[c->pcur], c->P, c->cstart, c->cend,
                          c->nbr[c->hcur], c->nnb[c->hcur], c->overflow, c->cap, n, s);
+    launch_active_slots(c->P4[c->cur], c->nbr[c->hcur], c->nnb[c->hcur], c->amask[c->hcur],
+                        c->actoff[c->hcur], c->keys2, c->sort_tmp, c->cap, n, s);
     HIPCHK(hipGetLastError());
     return DEM_OK;
 }
@@ -474,9 +506,10 @@ extern "C" int dem_hstry_copy(dem_ctx *c) {
     if (!c) return dem_fail(DEM_ERR_ARG, "null context");
     const int o = 1 - c->hcur;
     launch_hist_remap(c->vals, c->id[c->pcur], c->id[1 - c->pcur], c->nbr[c->hcur], c->nnb[c->hcur],
-                      c->nbr[o], c->nnb[o], c->hp[c->hcur], c->hp[o], c->wl[c->hcur],
+                      c->amask[c->hcur], c->actoff[c->hcur], c->nbr[o], c->nnb[o], c->amask[o],
+                      c->actoff[o], c->hp[c->hcur], c->hp[o], c->wl[c->hcur],
                       c->nwl[c->hcur], c->wl[o], c->nwl[o], c->hw[c->hcur], c->hw[o],
-                      c->cap, c->total, c->saved ? c->old_total : 0, c->stream);
+                      c->cap, (long)c->K * c->cap, c->total, c->saved ? c->old_total : 0, c->stream);
     HIPCHK(hipGetLastError());
     c->saved = false;
     c->has_lists = true;

@@ -43,7 +43,9 @@ struct dem_ctx {
     // lists and history, current = [hcur], old = [1-hcur]
     int *nbr[2] = {}, *nnb[2] = {};
     int *wl[2] = {}, *nwl[2] = {};
-    HistRec *hp[2] = {}, *hw[2] = {};
+    HistRec *hp[2] = {}, *hw[2] = {};   // hp: compact region K*cap, then slot-major K*cap
+    unsigned long long *amask[2] = {};  // compact-record slots of each particle
+    int *actoff[2] = {};                // start of each particle's compact records
     int hcur = 0;
     bool has_lists = false, saved = false;
     int old_total = 0;           // particles covered by the old lists

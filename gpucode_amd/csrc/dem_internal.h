@@ -81,7 +81,9 @@ struct StateBuf {
 struct StepArgs {
     StateBuf in, out;          // read A, write B (in == out for in-place)
     const int *nbr, *nnb;
-    HistRec *h;
+    HistRec *h;                // history (see hist_idx)
+    const unsigned long long *amask;  // per particle: slots whose records are in the compact region
+    const int *actoff;         // per particle: start of its compact records
     const int *wl, *nwl;
     HistRec *hw;
     double *Fx, *Fy, *Fz, *Mx, *My, *Mz, *Mr;  // optional force output
@@ -126,11 +128,15 @@ struct RbInst {
     const int *nbr_old, *nnb_old;
     HistRec *hp;
     const HistRec *hp_old;
+    unsigned long long *amask;
+    const unsigned long long *amask_old;
+    int *actoff;
+    const int *actoff_old;
     int *wl, *nwl;
     const int *wl_old, *nwl_old;
     HistRec *hw;
     const HistRec *hw_old;
     int *overflow;
-    int cap, total, old_total, ncells, passes, inst;
+    int cap, total, old_total, ncells, passes, inst, K, pad;
     double Tw;
 };

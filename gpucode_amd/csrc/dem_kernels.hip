@@ -489,11 +489,67 @@ DEV HistRec hist_zero() {
     return r;
 }
 
+// History layout.  The history array holds two regions of K*cap records:
+// a compact region, then a slot-major region.  At a rebuild the slots of
+// particle i that are in or near contact get their records in the compact region,
+// in slot order, from actoff[i] on (a prefix sum over the particles in
+// storage order), so that the contact records of a wave's particles are
+// contiguous; bit k of amask[i] marks those slots.  Every other slot k
+// keeps its record at K*cap + k*cap + i.  Where a record lives changes
+// nothing else: each directed slot has exactly one record, as in the
+// reference, and contacts are still summed in slot order.
+DEV long hist_idx(unsigned long long am, int off, int k, int i, long cap, long Kcap) {
+    if (k < 64 && ((am >> k) & 1ull)) return (long)off + __popcll(am & ((1ull << k) - 1ull));
+    return Kcap + (long)k * cap + i;
+}
+
+// slots of particle i (the first 64) in contact or within 1 % of the
+// diameter sum of it, which are the pairs likely to touch until the next
+// rebuild: amask[i] and their count.  (Only the layout depends on this.)
+#define DEM_ACTIVE_GAP 0.01
+DEV void active_slots_one(const double4 *p, const int *nbr, const int *nnb, unsigned long long *amask,
+                          unsigned *cnt, int cap, int i) {
+    const int n = nnb[i];
+    const double4 pi = p[i];
+    unsigned long long m = 0ull;
+    for (int a = 0; a < n && a < 64; ++a) {
+        const double4 q = p[nbr[(long)a * cap + i]];
+        const double dx = pi.x - q.x, dy = pi.y - q.y, dz = pi.z - q.z;
+        const double rs = pi.w + q.w;
+        if (sqrt(dx * dx + dy * dy + dz * dz) - rs < DEM_ACTIVE_GAP * 2.0 * rs) m |= 1ull << a;
+    }
+    amask[i] = m;
+    cnt[i] = (unsigned)__popcll(m);
+}
+
+__global__ void k_active_slots(const double4 *p, const int *nbr, const int *nnb,
+                               unsigned long long *amask, unsigned *cnt, int cap, int total) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < total) active_slots_one(p, nbr, nnb, amask, cnt, cap, i);
+}
+
+// amask and actoff of the new lists (cnt is scratch of total entries)
+void launch_active_slots(const double4 *p, const int *nbr, const int *nnb, unsigned long long *amask,
+                         int *actoff, unsigned *cnt, unsigned *scan_tmp, int cap, int total,
+                         hipStream_t s) {
+    if (total <= 0) return;
+    hipLaunchKernelGGL(k_active_slots, dim3(blocks_for(total, 256)), dim3(256), 0, s, p, nbr, nnb, amask,
+                       cnt, cap, total);
+    launch_excl_scan(cnt, (unsigned *)actoff, total, scan_tmp, s);
+}
+
 DEV void hist_remap_one(const unsigned *perm, const int *id_new, const int *id_old, const int *nbr,
-                        const int *nnb, const int *nbr_old, const int *nnb_old, HistRec *h,
-                        const HistRec *h_old, int cap, int old_total, int i) {
+                        const int *nnb, const unsigned long long *am_new, const int *off_new,
+                        const int *nbr_old, const int *nnb_old, const unsigned long long *am_old,
+                        const int *off_old, HistRec *h, const HistRec *h_old, int cap, long Kcap,
+                        int old_total, int i) {
     int p = (int)perm[i];
-    int nold = p < old_total ? nnb_old[p] : 0;
+    const bool had = p < old_total;
+    int nold = had ? nnb_old[p] : 0;
+    const unsigned long long amo = had ? am_old[p] : 0ull;
+    const int ofo = had ? off_old[p] : 0;
+    const unsigned long long amn = am_new[i];
+    const int ofn = off_new[i];
     int n = nnb[i];
     for (int a = 0; a < n; ++a) {
         long sa = (long)a * cap + i;
@@ -501,17 +557,51 @@ DEV void hist_remap_one(const unsigned *perm, const int *id_new, const int *id_o
         int hit = -1;
         for (int b = 0; b < nold; ++b)
             if (id_old[nbr_old[(long)b * cap + p]] == jid) hit = b;
-        h[sa] = hit >= 0 ? h_old[(long)hit * cap + p] : hist_zero();
+        h[hist_idx(amn, ofn, a, i, cap, Kcap)] =
+            hit >= 0 ? h_old[hist_idx(amo, ofo, hit, p, cap, Kcap)] : hist_zero();
     }
 }
 
 __global__ void k_hist_remap(const unsigned *perm, const int *id_new, const int *id_old,
-                             const int *nbr, const int *nnb, const int *nbr_old,
-                             const int *nnb_old, HistRec *h, const HistRec *h_old, int cap,
-                             int total, int old_total) {
+                             const int *nbr, const int *nnb, const unsigned long long *am_new,
+                             const int *off_new, const int *nbr_old, const int *nnb_old,
+                             const unsigned long long *am_old, const int *off_old, HistRec *h,
+                             const HistRec *h_old, int cap, long Kcap, int total, int old_total) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= total) return;
-    hist_remap_one(perm, id_new, id_old, nbr, nnb, nbr_old, nnb_old, h, h_old, cap, old_total, i);
+    hist_remap_one(perm, id_new, id_old, nbr, nnb, am_new, off_new, nbr_old, nnb_old, am_old, off_old, h,
+                   h_old, cap, Kcap, old_total, i);
+}
+
+// the directed slot-major view (record k*cap+i = slot k of particle i) of
+// the history, and back
+__global__ void k_hist_to_slots(const HistRec *h, const int *nnb, const unsigned long long *amask,
+                                const int *actoff, HistRec *out, int cap, long Kcap, int total) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    const unsigned long long am = amask[i];
+    const int off = actoff[i];
+    for (int a = 0; a < nnb[i]; ++a) out[(long)a * cap + i] = h[hist_idx(am, off, a, i, cap, Kcap)];
+}
+
+__global__ void k_hist_from_slots(HistRec *h, const int *nnb, const unsigned long long *amask,
+                                  const int *actoff, const HistRec *in, int cap, long Kcap, int total) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    const unsigned long long am = amask[i];
+    const int off = actoff[i];
+    for (int a = 0; a < nnb[i]; ++a) h[hist_idx(am, off, a, i, cap, Kcap)] = in[(long)a * cap + i];
+}
+
+void launch_hist_slots(HistRec *h, const int *nnb, const unsigned long long *amask, const int *actoff,
+                       HistRec *slots, int cap, long Kcap, int total, bool to_slots, hipStream_t s) {
+    if (total <= 0) return;
+    if (to_slots)
+        hipLaunchKernelGGL(k_hist_to_slots, dim3(blocks_for(total, 256)), dim3(256), 0, s, h, nnb, amask,
+                           actoff, slots, cap, Kcap, total);
+    else
+        hipLaunchKernelGGL(k_hist_from_slots, dim3(blocks_for(total, 256)), dim3(256), 0, s, h, nnb,
+                           amask, actoff, slots, cap, Kcap, total);
 }
 
 DEV void hist_remap_pw_one(const unsigned *perm, const int *wl, const int *nwl, const int *wl_old,
@@ -537,13 +627,16 @@ __global__ void k_hist_remap_pw(const unsigned *perm, const int *wl, const int *
 }
 
 void launch_hist_remap(const unsigned *perm, const int *id_new, const int *id_old, const int *nbr,
-                       const int *nnb, const int *nbr_old, const int *nnb_old, HistRec *h,
-                       const HistRec *h_old, const int *wl, const int *nwl, const int *wl_old,
-                       const int *nwl_old, HistRec *hw, const HistRec *hw_old, int cap, int total,
-                       int old_total, hipStream_t s) {
+                       const int *nnb, const unsigned long long *am_new, const int *off_new,
+                       const int *nbr_old, const int *nnb_old, const unsigned long long *am_old,
+                       const int *off_old, HistRec *h, const HistRec *h_old, const int *wl,
+                       const int *nwl, const int *wl_old, const int *nwl_old, HistRec *hw,
+                       const HistRec *hw_old, int cap, long Kcap, int total, int old_total,
+                       hipStream_t s) {
     if (total <= 0) return;
     hipLaunchKernelGGL(k_hist_remap, dim3(blocks_for(total, 128)), dim3(128), 0, s, perm, id_new,
-                       id_old, nbr, nnb, nbr_old, nnb_old, h, h_old, cap, total, old_total);
+                       id_old, nbr, nnb, am_new, off_new, nbr_old, nnb_old, am_old, off_old, h, h_old,
+                       cap, Kcap, total, old_total);
     hipLaunchKernelGGL(k_hist_remap_pw, dim3(blocks_for(total, 256)), dim3(256), 0, s, perm, wl,
                        nwl, wl_old, nwl_old, hw, hw_old, cap, total, old_total);
 }
@@ -856,6 +949,9 @@ DEV void pp_particle(const DevInst &P, const StepArgs &A, int i, const double4 p
     double di = 2.0 * ri;
     F.z = F.z - di * di * di;
     const int n = A.nnb[i];
+    const unsigned long long am = A.amask[i];
+    const int aoff = A.actoff[i];
+    const long Kcap = (long)A.K * A.cap;
     for (int k = 0; k < n; ++k) {
         const long sk = (long)k * A.cap + i;
         const int j = A.nbr[sk];
@@ -879,10 +975,11 @@ DEV void pp_particle(const DevInst &P, const StepArgs &A, int i, const double4 p
         const double gap = l - ri - rj;
         HistRec h;
         bool loaded = false;
+        const long hi = hist_idx(am, aoff, k, i, A.cap, Kcap);
         if (gap < 0.0) {
             const V3 jv = vel_of(s.v, j);
             const V3 jw = spin_of(s.v, j);
-            h = A.h[sk];
+            h = A.h[hi];
             loaded = true;
             double R = ri * rj / (ri + rj);
             V3 cpv = sub(sub(iv, jv), cross(add(mulv(iw, ri), mulv(jw, rj)), nv));
@@ -898,7 +995,7 @@ DEV void pp_particle(const DevInst &P, const StepArgs &A, int i, const double4 p
             double jVol = P.vol * (dj * dj * dj) / jnl;
             double V = iVol + jVol;
             double rup = rupture(P, V);
-            int fl = loaded ? h.flags : A.h[sk].flags;
+            int fl = loaded ? h.flags : A.h[hi].flags;
             int brg = (fl >> 8) & 1;
             if (gap < P.layer2 * (ri + rj)) brg = 1;
             else if (gap > rup) brg = 0;
@@ -906,7 +1003,7 @@ DEV void pp_particle(const DevInst &P, const StepArgs &A, int i, const double4 p
             if (loaded)
                 h.flags = nfl;
             else if (nfl != fl)
-                A.h[sk].flags = nfl;
+                A.h[hi].flags = nfl;
             double Sp = gap < 0 ? 0.0 : gap;
             if (brg == 1) {
                 double effR = 2 * ri * rj / (ri + rj);
@@ -916,7 +1013,7 @@ DEV void pp_particle(const DevInst &P, const StepArgs &A, int i, const double4 p
                 nbridge = nbridge + 1;
             }
         }
-        if (loaded) A.h[sk] = h;
+        if (loaded) A.h[hi] = h;
     }
 }
 
@@ -1165,7 +1262,8 @@ __global__ void __launch_bounds__(CP_BS) k_step_compact(DevInst P, StepArgs A) {
             pair_geometry<MIXED>(qi, qj, nv, gap);
             const V3 iv = vel_of(A.in.v, ii), iw = spin_of(A.in.v, ii);
             const V3 jv = vel_of(A.in.v, j), jw = spin_of(A.in.v, j);
-            HistRec h = A.h[sk];
+            const long hi = hist_idx(A.amask[ii], A.actoff[ii], k, ii, A.cap, (long)A.K * A.cap);
+            HistRec h = A.h[hi];
             const double ri = qi.w, rj = qj.w;
             const double R = ri * rj / (ri + rj);
             const V3 cpv = sub(sub(iv, jv), cross(add(mulv(iw, ri), mulv(jw, rj)), nv));
@@ -1175,7 +1273,7 @@ __global__ void __launch_bounds__(CP_BS) k_step_compact(DevInst P, StepArgs A) {
                 contact_mixed<true>(P.pp, P.rolling, P.dt, h, -gap, nv, R, ri, cpv, dF, dM, dMr);
             else
                 contact<true>(P.pp, P.rolling, P.dt, h, -gap, nv, R, ri, cpv, dF, dM, dMr);
-            A.h[sk] = h;
+            A.h[hi] = h;
             res[0 * CP_BS + tid] = dF.x; res[1 * CP_BS + tid] = dF.y; res[2 * CP_BS + tid] = dF.z;
             res[3 * CP_BS + tid] = dM.x; res[4 * CP_BS + tid] = dM.y; res[5 * CP_BS + tid] = dM.z;
             res[6 * CP_BS + tid] = dMr;
@@ -1279,10 +1377,11 @@ DEV int select_bit(unsigned long long m, unsigned r) {
 // one particle-particle entry of the compacted list: slot k of particle
 // ii, neighbour j
 template <bool MIXED>
-DEV void pair_entry(const DevInst &P, const StepArgs &A, int ii, int k, int j, const double4 qi,
-                    const double4 ai, const double4 bi, V3 &dF, V3 &dM, double &dMr) {
-    const long sk = (long)k * A.cap + ii;
-    HistRec h = A.h[sk];
+DEV void pair_entry(const DevInst &P, const StepArgs &A, int ii, int k, int j, unsigned long long am,
+                    int aoff, const double4 qi, const double4 ai, const double4 bi, V3 &dF, V3 &dM,
+                    double &dMr) {
+    const long hi = hist_idx(am, aoff, k, ii, A.cap, (long)A.K * A.cap);
+    HistRec h = A.h[hi];
     const double4 qj = A.in.p[j];
     const V3 iv = v3(ai.x, ai.y, ai.z), iw = v3(ai.w, bi.x, bi.y);
     const V3 jv = vel_of(A.in.v, j), jw = spin_of(A.in.v, j);
@@ -1296,7 +1395,7 @@ DEV void pair_entry(const DevInst &P, const StepArgs &A, int ii, int k, int j, c
         contact_mixed<true>(P.pp, P.rolling, P.dt, h, -gap, nv, R, ri, cpv, dF, dM, dMr);
     else
         contact<true>(P.pp, P.rolling, P.dt, h, -gap, nv, R, ri, cpv, dF, dM, dMr);
-    A.h[sk] = h;
+    A.h[hi] = h;
 }
 
 // one wall entry: slot kw of particle ii, wall w.  The wall velocity is the
@@ -1328,12 +1427,14 @@ DEV void wall_entry(const DevInst &P, const StepArgs &A, int ii, int kw, int w, 
 struct WvEntry {
     int ii, k;
     bool wall, valid;
+    unsigned long long am;  // the owner's compact-history mask and offset
+    int aoff;
 };
 
 // entry t of the wave's list; every lane of the wave must call this (it
 // reads the other lanes' offsets and masks through shuffles)
 DEV WvEntry wave_entry(unsigned t, unsigned T, unsigned off, unsigned mlo, unsigned mhi,
-                       unsigned wmask, int wbase) {
+                       unsigned wmask, int wbase, unsigned long long am, int aoff) {
     const unsigned tq = t < T ? t : T - 1;
     int o = 0;
 #pragma unroll
@@ -1345,6 +1446,9 @@ DEV WvEntry wave_entry(unsigned t, unsigned T, unsigned off, unsigned mlo, unsig
     const unsigned ohi = (unsigned)__shfl((int)mhi, o, 64);
     const unsigned ow = (unsigned)__shfl((int)wmask, o, 64);
     const unsigned ooff = (unsigned)__shfl((int)off, o, 64);
+    const unsigned alo = (unsigned)__shfl((int)(unsigned)am, o, 64);
+    const unsigned ahi = (unsigned)__shfl((int)(unsigned)(am >> 32), o, 64);
+    const int aof = __shfl(aoff, o, 64);
     const unsigned long long om = ((unsigned long long)ohi << 32) | olo;
     const unsigned r = tq - ooff, np = (unsigned)__popcll(om);
     WvEntry e;
@@ -1352,6 +1456,8 @@ DEV WvEntry wave_entry(unsigned t, unsigned T, unsigned off, unsigned mlo, unsig
     e.valid = t < T;
     e.wall = r >= np;
     e.k = e.wall ? select_bit(ow, r - np) : select_bit(om, r);
+    e.am = ((unsigned long long)ahi << 32) | alo;
+    e.aoff = aof;
     return e;
 }
 
@@ -1369,10 +1475,13 @@ DEV void step_wave_block(const DevInst &P, const StepArgs &A, unsigned blk, doub
     // first batch of neighbour indices is read together with the counts (in
     // bounds whatever the count is: the list holds K slots per particle);
     // entries past the count are masked after.
-    unsigned long long mask = 0ull;
+    unsigned long long mask = 0ull, am_own = 0ull;
     unsigned wmask = 0u;
+    int aoff_own = 0;
     double grav = 0.0;
     if (own) {
+        am_own = A.amask[i];
+        aoff_own = A.actoff[i];
         int jj[WV_NB];
         const int kb = A.K < WV_NB ? A.K : WV_NB;
 #pragma unroll
@@ -1428,7 +1537,7 @@ DEV void step_wave_block(const DevInst &P, const StepArgs &A, unsigned blk, doub
     WvEntry e;
     int j = 0;
     if (T > 0) {
-        e = wave_entry(lane, T, off, mlo, mhi, wmask, wbase);
+        e = wave_entry(lane, T, off, mlo, mhi, wmask, wbase, am_own, aoff_own);
         if (e.valid) j = (e.wall ? A.wl : A.nbr)[(long)e.k * cap + e.ii];
     }
     for (unsigned r0 = 0; r0 < T; r0 += 64) {
@@ -1436,7 +1545,7 @@ DEV void step_wave_block(const DevInst &P, const StepArgs &A, unsigned blk, doub
         const int jc = j;
         // next round's entry and its neighbour index
         if (r0 + 64 < T) {
-            e = wave_entry(r0 + 64 + lane, T, off, mlo, mhi, wmask, wbase);
+            e = wave_entry(r0 + 64 + lane, T, off, mlo, mhi, wmask, wbase, am_own, aoff_own);
             if (e.valid) j = (e.wall ? A.wl : A.nbr)[(long)e.k * cap + e.ii];
         } else {
             e.valid = false;
@@ -1447,7 +1556,7 @@ DEV void step_wave_block(const DevInst &P, const StepArgs &A, unsigned blk, doub
             const int o = ec.ii - wbase;
             const double4 qi = st->p[o], ai = st->va[o], bi = st->vb[o];
             if (!ec.wall)
-                pair_entry<MIXED>(P, A, ec.ii, ec.k, jc, qi, ai, bi, dF, dM, dMr);
+                pair_entry<MIXED>(P, A, ec.ii, ec.k, jc, ec.am, ec.aoff, qi, ai, bi, dF, dM, dMr);
             else
                 wall_entry<MIXED>(P, A, ec.ii, ec.k, jc, qi, ai, bi, dF, dM, dMr);
             res[0][lane] = dF.x; res[1][lane] = dF.y; res[2][lane] = dF.z;
@@ -1811,13 +1920,36 @@ __global__ void __launch_bounds__(RB_BS) kb_lists(const DevInst *insts, const Rb
     wall_list_one(R.pb, P, R.Tw, R.wl, R.nwl, R.cap, i);
 }
 
+__global__ void __launch_bounds__(RB_BS) kb_active(const RbInst *tab, const int2 *tasks) {
+    const int2 tk = tasks[blockIdx.x];
+    const RbInst &R = tab[tk.x];
+    const int i = tk.y * RB_BS + threadIdx.x;
+    if (i < R.total) active_slots_one(R.pb, R.nbr, R.nnb, R.amask, R.keys2, R.cap, i);
+}
+
+// actoff = exclusive prefix sum of the per-particle counts, one block per
+// instance, tile by tile
+__global__ void __launch_bounds__(SCAN_BS) kb_actoff(const RbInst *tab) {
+    const RbInst &R = tab[blockIdx.x];
+    __shared__ unsigned lds[4];
+    unsigned carry = 0;
+    for (int t0 = 0; t0 < R.total; t0 += SCAN_BS) {
+        const int e = t0 + threadIdx.x;
+        const unsigned v = e < R.total ? R.keys2[e] : 0u;
+        unsigned tot;
+        const unsigned ex = block_excl_scan(v, &tot, lds);
+        if (e < R.total) R.actoff[e] = (int)(carry + ex);
+        carry += tot;
+    }
+}
+
 __global__ void __launch_bounds__(RB_BS) kb_hist_remap(const RbInst *tab, const int2 *tasks) {
     const int2 tk = tasks[blockIdx.x];
     const RbInst &R = tab[tk.x];
     const int i = tk.y * RB_BS + threadIdx.x;
     if (i >= R.total) return;
-    hist_remap_one(R.vals, R.idb, R.ida, R.nbr, R.nnb, R.nbr_old, R.nnb_old, R.hp, R.hp_old, R.cap,
-                   R.old_total, i);
+    hist_remap_one(R.vals, R.idb, R.ida, R.nbr, R.nnb, R.amask, R.actoff, R.nbr_old, R.nnb_old,
+                   R.amask_old, R.actoff_old, R.hp, R.hp_old, R.cap, (long)R.K * R.cap, R.old_total, i);
     hist_remap_pw_one(R.vals, R.wl, R.nwl, R.wl_old, R.nwl_old, R.hw, R.hw_old, R.cap, R.old_total, i);
 }
 
@@ -1832,6 +1964,8 @@ void launch_rebuild_batch(const DevInst *insts, const RbInst *tab, int ninst, co
     if (nptasks > 0) {
         hipLaunchKernelGGL(kb_cell_range, dim3(nptasks), dim3(RB_BS), 0, s, insts, tab, ptasks);
         hipLaunchKernelGGL(kb_lists, dim3(nptasks), dim3(RB_BS), 0, s, insts, tab, ptasks);
+        hipLaunchKernelGGL(kb_active, dim3(nptasks), dim3(RB_BS), 0, s, tab, ptasks);
+        hipLaunchKernelGGL(kb_actoff, dim3(ninst), dim3(SCAN_BS), 0, s, tab);
         hipLaunchKernelGGL(kb_hist_remap, dim3(nptasks), dim3(RB_BS), 0, s, tab, ptasks);
     }
 }
@@ -2027,7 +2161,8 @@ __global__ void __launch_bounds__(256) k_stats(DevInst P, StepArgs A, unsigned l
             double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
             double gap = sqrt(dx * dx + dy * dy + dz * dz) - pi.w - pj.w;
             if (gap < 0.0) nc++;
-            if ((A.h[sk].flags >> 8) & 1) nbr++;
+            if ((A.h[hist_idx(A.amask[i], A.actoff[i], k, i, A.cap, (long)A.K * A.cap)].flags >> 8) & 1)
+                nbr++;
         }
         nw = A.nwl[i];
         for (int k = 0; k < (int)nw; ++k) {

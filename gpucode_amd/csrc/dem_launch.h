@@ -25,10 +25,17 @@ void launch_neighbor_find(const StateBuf &s, const int *level, const DevInst &P,
 void launch_wall_list(const StateBuf &s, const DevInst &P, double Tw, int *wl, int *nwl, int cap,
                       int total, hipStream_t st);
 void launch_hist_remap(const unsigned *perm, const int *id_new, const int *id_old, const int *nbr,
-                       const int *nnb, const int *nbr_old, const int *nnb_old, HistRec *h,
-                       const HistRec *h_old, const int *wl, const int *nwl, const int *wl_old,
-                       const int *nwl_old, HistRec *hw, const HistRec *hw_old, int cap, int total,
-                       int old_total, hipStream_t s);
+                       const int *nnb, const unsigned long long *am_new, const int *off_new,
+                       const int *nbr_old, const int *nnb_old, const unsigned long long *am_old,
+                       const int *off_old, HistRec *h, const HistRec *h_old, const int *wl,
+                       const int *nwl, const int *wl_old, const int *nwl_old, HistRec *hw,
+                       const HistRec *hw_old, int cap, long Kcap, int total, int old_total,
+                       hipStream_t s);
+void launch_active_slots(const double4 *p, const int *nbr, const int *nnb, unsigned long long *amask,
+                         int *actoff, unsigned *cnt, unsigned *scan_tmp, int cap, int total,
+                         hipStream_t s);
+void launch_hist_slots(HistRec *h, const int *nnb, const unsigned long long *amask, const int *actoff,
+                       HistRec *slots, int cap, long Kcap, int total, bool to_slots, hipStream_t s);
 
 void launch_step_fused(const DevInst &P, const StepArgs &A, bool wet, bool mixed, hipStream_t s);
 // dry step with contact compaction; needs K <= 64
