@@ -1333,7 +1333,7 @@ void launch_step_compact(const DevInst &P, const StepArgs &A, int K, bool mixed,
 //     the owner's masks (register shuffles only), evaluates the contact law
 //     and puts the force, moment and rolling term in the wave's LDS slice;
 //     each owner then adds its entries in list order.  The next round's
-//     neighbour indices load while a round computes.
+//     neighbour indices and history records load while a round computes.
 //  D: integration by the owner.
 // The owners' records stay in an LDS stash from A to D (not in registers,
 // which keeps the kernel at 4 waves per SIMD, and not re-read from memory).
@@ -1374,17 +1374,14 @@ DEV int select_bit(unsigned long long m, unsigned r) {
     return pos;
 }
 
-// one particle-particle entry of the compacted list: slot k of particle
-// ii, neighbour j
+// the contact law of one particle-particle entry of the compacted list
+// (history record h in, updated record out)
 template <bool MIXED>
-DEV void pair_entry(const DevInst &P, const StepArgs &A, int ii, int k, int j, unsigned long long am,
-                    int aoff, const double4 qi, const double4 ai, const double4 bi, V3 &dF, V3 &dM,
-                    double &dMr) {
-    const long hi = hist_idx(am, aoff, k, ii, A.cap, (long)A.K * A.cap);
-    HistRec h = A.h[hi];
-    const double4 qj = A.in.p[j];
+DEV void pair_compute(const DevInst &P, HistRec &h, const double4 qi, const double4 ai,
+                      const double4 bi, const double4 qj, const double4 aj, const double4 bj, V3 &dF,
+                      V3 &dM, double &dMr) {
     const V3 iv = v3(ai.x, ai.y, ai.z), iw = v3(ai.w, bi.x, bi.y);
-    const V3 jv = vel_of(A.in.v, j), jw = spin_of(A.in.v, j);
+    const V3 jv = v3(aj.x, aj.y, aj.z), jw = v3(aj.w, bj.x, bj.y);
     V3 nv;
     double gap;
     pair_geometry<MIXED>(qi, qj, nv, gap);
@@ -1395,7 +1392,6 @@ DEV void pair_entry(const DevInst &P, const StepArgs &A, int ii, int k, int j, u
         contact_mixed<true>(P.pp, P.rolling, P.dt, h, -gap, nv, R, ri, cpv, dF, dM, dMr);
     else
         contact<true>(P.pp, P.rolling, P.dt, h, -gap, nv, R, ri, cpv, dF, dM, dMr);
-    A.h[hi] = h;
 }
 
 // one wall entry: slot kw of particle ii, wall w.  The wall velocity is the
@@ -1531,22 +1527,45 @@ DEV void step_wave_block(const DevInst &P, const StepArgs &A, unsigned blk, doub
     Fz = Fz - grav;  // gravity first (kernel.cu:439)
     const unsigned mlo = (unsigned)mask, mhi = (unsigned)(mask >> 32);
 
-    // phase C, software-pipelined: the neighbour index (or wall id) of
-    // round r+1 is in flight while round r computes, so a round waits for
-    // one load (its history record and the two particles' state) only
+    // phase C, software-pipelined: a round first issues the loads of its
+    // neighbours' state, then those of the next round's neighbour indices
+    // (or wall ids) and history records, then computes; the history of
+    // round r+1 streams in from HBM while round r computes
+    const long Kcap = (long)A.K * cap;
     WvEntry e;
     int j = 0;
+    long hin = 0;
+    HistRec hn;
     if (T > 0) {
         e = wave_entry(lane, T, off, mlo, mhi, wmask, wbase, am_own, aoff_own);
-        if (e.valid) j = (e.wall ? A.wl : A.nbr)[(long)e.k * cap + e.ii];
+        if (e.valid) {
+            j = (e.wall ? A.wl : A.nbr)[(long)e.k * cap + e.ii];
+            if (!e.wall) {
+                hin = hist_idx(e.am, e.aoff, e.k, e.ii, cap, Kcap);
+                hn = A.h[hin];
+            }
+        }
     }
     for (unsigned r0 = 0; r0 < T; r0 += 64) {
         const WvEntry ec = e;
         const int jc = j;
-        // next round's entry and its neighbour index
+        const long hic = hin;
+        HistRec hc = hn;
+        double4 qj, aj, bj;
+        if (ec.valid && !ec.wall) {
+            qj = A.in.p[jc];
+            aj = A.in.v[2 * jc];
+            bj = A.in.v[2 * jc + 1];
+        }
         if (r0 + 64 < T) {
             e = wave_entry(r0 + 64 + lane, T, off, mlo, mhi, wmask, wbase, am_own, aoff_own);
-            if (e.valid) j = (e.wall ? A.wl : A.nbr)[(long)e.k * cap + e.ii];
+            if (e.valid) {
+                j = (e.wall ? A.wl : A.nbr)[(long)e.k * cap + e.ii];
+                if (!e.wall) {
+                    hin = hist_idx(e.am, e.aoff, e.k, e.ii, cap, Kcap);
+                    hn = A.h[hin];
+                }
+            }
         } else {
             e.valid = false;
         }
@@ -1555,10 +1574,12 @@ DEV void step_wave_block(const DevInst &P, const StepArgs &A, unsigned blk, doub
             double dMr = 0.0;
             const int o = ec.ii - wbase;
             const double4 qi = st->p[o], ai = st->va[o], bi = st->vb[o];
-            if (!ec.wall)
-                pair_entry<MIXED>(P, A, ec.ii, ec.k, jc, ec.am, ec.aoff, qi, ai, bi, dF, dM, dMr);
-            else
+            if (!ec.wall) {
+                pair_compute<MIXED>(P, hc, qi, ai, bi, qj, aj, bj, dF, dM, dMr);
+                A.h[hic] = hc;
+            } else {
                 wall_entry<MIXED>(P, A, ec.ii, ec.k, jc, qi, ai, bi, dF, dM, dMr);
+            }
             res[0][lane] = dF.x; res[1][lane] = dF.y; res[2][lane] = dF.z;
             res[3][lane] = dM.x; res[4][lane] = dM.y; res[5][lane] = dM.z;
             res[6][lane] = dMr;
