@@ -1,12 +1,18 @@
 """Throughput of the DEM time step on one MI355X.
 
-Workload (the 2M-sphere throughput case): 2,000,000 monodisperse dry
-spheres, d = 0.3 mm, in a cylinder of radius 30 mm (100 d) with a floor,
-neighbour-list skin 0.1 d, mu = 0.5, rho 2500, E 70 GPa, nu 0.25, e 0.7,
-yield at 1 % d, FP64 state.  The bed is a seeded layer-by-layer pocket
-deposition (about 54 d tall); W warmup steps settle it, then K steps are
-timed, rebuilds included.  A "step" is one pass of the time step over all
-particles (forces, walls, integration; a neighbour rebuild when due).
+Default workload (the 2M-sphere throughput case): 2,000,000 monodisperse
+dry spheres, d = 0.3 mm, in a cylinder of radius 30 mm (100 d) with a
+floor, neighbour-list skin 0.1 d, mu = 0.5, rho 2500, E 70 GPa, nu 0.25,
+e 0.7, yield at 1 % d, FP64 state.  The bed is a seeded layer-by-layer
+pocket deposition (about 54 d tall); W warmup steps settle it, then K steps
+are timed, rebuilds included.  A "step" is one pass of the time step over
+all particles (forces, walls, integration; a neighbour rebuild when due).
+
+--workload wet_poly: 200,000 wet polydisperse spheres (lognormal d, median
+0.5 mm, sigma 0.25, clipped 0.25-1.0 mm, three size levels; capillary
+bridges with liquid 1 % of the mean particle volume, theta 20 deg) poured
+from a seeded random feed into a cylinder of radius 25 mm; by default the
+whole 50,000-step pour is timed, from the fed state, with no warmup.
 
 Prints ONE JSON line on stdout.  Progress goes to stderr.
 """
@@ -39,30 +45,57 @@ def algorithmic_bytes(st, n):
     return 176.0 + 4.0 * n_nb + 128.0 * (n_c + n_w), n_nb, n_c, n_w
 
 
-def load_traffic(kernel):
-    path = os.path.join(ROOT, "profiles", f"traffic_{kernel}_packed_bed_2M.json")
+def load_traffic(kernel, tag):
+    path = os.path.join(ROOT, "profiles", f"traffic_{kernel}_{tag}.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
         return json.load(f)
 
 
-def cpu_baseline(engine, inp, threads, budget_s=12.0):
+def cpu_baseline(engine, inp, threads, budget_s=12.0, dia=None, start=None):
+    """The FP64 C oracle timed on the host.  Without start: force + wall +
+    integrate steps of the engine's current bed after one untimed rebuild.
+    With start (positions, velocities, spins at the beginning of a pour):
+    whole steps, rebuilds included, from that state and from the engine's
+    current (end) state, half the budget each; the rate is the particle-steps
+    of both samples over their summed time."""
     from oracle import pyoracle as O
 
     O.threads(threads)
     pos, vel, angv = engine.particles()
-    o = O.Oracle(inp)
-    o.set_particles(pos, vel, angv)
-    o.rebuild()
-    t1 = o.time_force_steps(1)
-    steps = int(max(1, min(20, budget_s / max(t1, 1e-6))))
-    t = o.time_force_steps(steps)
     n = len(pos)
-    o.close()
-    return dict(value=n * steps / t, unit="particle-steps/s", cores=threads, kind="port",
-                sample=f"FP64 C oracle (OpenMP, gather form): {steps} force+wall+integrate steps "
-                       f"of the same {n}-sphere bed after one untimed rebuild")
+    if start is None:
+        o = O.Oracle(inp, dia)
+        o.set_particles(pos, vel, angv)
+        o.rebuild()
+        t1 = o.time_force_steps(1)
+        steps = int(max(1, min(20, budget_s / max(t1, 1e-6))))
+        t = o.time_force_steps(steps)
+        o.close()
+        return dict(value=n * steps / t, unit="particle-steps/s", cores=threads, kind="port",
+                    sample=f"FP64 C oracle (OpenMP, gather form): {steps} force+wall+integrate "
+                           f"steps of the same {n}-sphere bed after one untimed rebuild")
+    total_steps, total_t, parts = 0, 0.0, []
+    for label, state in (("start", start), ("end", (pos, vel, angv))):
+        o = O.Oracle(inp, dia)
+        o.set_particles(*state)
+        o.rebuild()
+        t0 = time.perf_counter()
+        o.step(10)
+        t10 = time.perf_counter() - t0
+        steps = int(max(10, min(5000, 0.5 * budget_s / max(t10 / 10, 1e-6))))
+        t0 = time.perf_counter()
+        o.step(steps)
+        t = time.perf_counter() - t0
+        o.close()
+        total_steps += steps
+        total_t += t
+        parts.append(f"{steps} at the {label}")
+    return dict(value=n * total_steps / total_t, unit="particle-steps/s", cores=threads,
+                kind="port",
+                sample=f"FP64 C oracle (OpenMP, gather form): whole steps, rebuilds included, of "
+                       f"the same {n}-sphere pour: {' and '.join(parts)} of the timed run")
 
 
 def sweep_main(args):
@@ -111,9 +144,16 @@ def sweep_main(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5000)
-    ap.add_argument("--warmup", type=int, default=300)
-    ap.add_argument("--n", type=int, default=2_000_000)
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed steps (default 5,000 for packed_bed; 50,000, the whole pour, for "
+                         "wet_poly)")
+    ap.add_argument("--warmup", type=int, default=None,
+                    help="untimed steps first (default 300 for packed_bed, 0 for wet_poly)")
+    ap.add_argument("--n", type=int, default=None,
+                    help="particles (default 2,000,000 for packed_bed, 200,000 for wet_poly)")
+    ap.add_argument("--workload", choices=["packed_bed", "wet_poly"], default="packed_bed",
+                    help="packed_bed: the 2M-sphere dry bed (the headline); wet_poly: the 200k "
+                         "polydisperse wet pour (capillary bridges, three size levels)")
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
     ap.add_argument("--kernel", choices=["auto", "wave", "compact", "loop"], default="auto",
                     help="fused step kernel: per-wave contact compaction (auto, wave), "
@@ -141,38 +181,81 @@ def main():
         return sweep_main(args)
 
     t0 = time.time()
-    w = W.packed_bed(n=args.n)
-    log(f"bed of {args.n} spheres built in {time.time() - t0:.1f}s")
+    if args.workload == "packed_bed":
+        n_req = args.n or 2_000_000
+        w = W.packed_bed(n=n_req)
+        desc = f"packed_bed_{n_req // 1000000}M (dry, monodisperse d=0.3mm, R=30mm, skin 0.1d, mu 0.5)"
+        metric = "particle-steps/s at 2M spheres"
+        tag = f"packed_bed_{n_req // 1000000}M"
+        steps = 5000 if args.steps is None else args.steps
+        warmup = 300 if args.warmup is None else args.warmup
+        nseg = 1
+    else:
+        n_req = args.n or 200_000
+        w = W.wet_poly(n=n_req)
+        steps = w["steps"] if args.steps is None else args.steps
+        warmup = 0 if args.warmup is None else args.warmup
+        desc = (f"wet_poly_{n_req // 1000}k (lognormal d 0.25-1.0mm, 3 size levels, liquid 1%, "
+                f"theta 20deg, R=25mm; the {steps}-step pour from the seeded feed)")
+        metric = f"particle-steps/s at {n_req // 1000}k wet polydisperse spheres"
+        tag = f"wet_poly_{n_req // 1000}k"
+        nseg = 10
+    args.steps, args.warmup = steps, warmup
+    log(f"{args.workload} of {n_req} spheres built in {time.time() - t0:.1f}s")
     prec = G.PREC_FP64 if args.precision == "fp64" else G.PREC_MIXED
-    e = G.Engine(w["inp"], None, prec)
+    e = G.Engine(w["inp"], w["dia"], prec)
     e.set_kernel({"auto": G.Engine.KERNEL_AUTO, "wave": G.Engine.KERNEL_WAVE,
                   "compact": G.Engine.KERNEL_COMPACT, "loop": G.Engine.KERNEL_LOOP}[args.kernel])
     e.set_particles(w["pos"])
+    n = e.total
     log(f"engine up: {e.device_bytes() / 2**30:.2f} GiB on device")
     t0 = time.time()
-    e.step(args.warmup)
+    e.step(warmup)
     e.sync()
-    log(f"warmup {args.warmup} steps in {time.time() - t0:.1f}s, rebuilds {e.n_rebuild}")
-    st0 = e.stats()
-    tm0 = e.timing()
+    log(f"warmup {warmup} steps in {time.time() - t0:.1f}s, rebuilds {e.n_rebuild}")
+    start = e.particles() if nseg > 1 else None
     rb0 = e.n_rebuild
-    t0 = time.perf_counter()
-    e.step(args.steps)
-    e.sync()
-    el = time.perf_counter() - t0
-    tm1 = e.timing()
-    st1 = e.stats()
-    n = e.total
-    log(f"timed {args.steps} steps in {el:.3f}s, rebuilds {e.n_rebuild - rb0}")
+    # The timed steps run in nseg segments; between segments (outside the
+    # timed region) the bed's neighbour and contact counts are sampled, so
+    # the algorithmic bytes follow a bed whose contacts change as it pours.
+    seg = [steps // nseg + (1 if i < steps % nseg else 0) for i in range(nseg)]
+    st_prev, tm_prev = e.stats(), e.timing()
+    st_first = st_prev
+    el = 0.0
+    bytes_alg = fused_ms = 0.0
+    launches = executed = 0
+    for i, k in enumerate(seg):
+        t0 = time.perf_counter()
+        e.step(k)
+        e.sync()
+        el += time.perf_counter() - t0
+        st, tm = e.stats(), e.timing()
+        stm = {key: 0.5 * (st_prev[key] + st[key]) for key in st}
+        B_seg, _, _, _ = algorithmic_bytes(stm, n)
+        if w["inp"].forceCap == 1:
+            # lqdCnt old/new, and lqdBrg r/w of the bridged slots
+            B_seg += 8.0 + 8.0 * stm["n_bridge"] / n
+        ex = tm["fused_steps"] - tm_prev["fused_steps"]
+        bytes_alg += B_seg * n * ex
+        executed += ex
+        launches += tm["fused_launches"] - tm_prev["fused_launches"]
+        fused_ms += tm["fused_ms"] - tm_prev["fused_ms"]
+        if nseg > 1:
+            log(f"segment {i + 1}/{nseg}: {k} steps, {el:.2f}s so far, rebuilds "
+                f"{e.n_rebuild - rb0}, contacts/particle {st['n_contact'] / n:.2f}")
+        st_prev, tm_prev = st, tm
+    log(f"timed {steps} steps in {el:.3f}s, rebuilds {e.n_rebuild - rb0}")
 
-    st = {k: 0.5 * (st0[k] + st1[k]) for k in st0}
-    B, n_nb, n_c, n_w = algorithmic_bytes(st, n)
-    launches = tm1["fused_launches"] - tm0["fused_launches"]
-    kern_ms = (tm1["fused_ms"] - tm0["fused_ms"]) / max(launches, 1)
-    achieved = B * n / (kern_ms * 1e-3) / 1e9
+    st = {k: 0.5 * (st_first[k] + st_prev[k]) for k in st_first}
+    _, n_nb, n_c, n_w = algorithmic_bytes(st, n)
+    B = bytes_alg / max(executed, 1) / n
+    kern_ms = fused_ms / max(launches, 1)
+    # algorithmic bytes of the steps that ran over the time of all launches
+    # (a launch after a rebuild became due returns at once and moves nothing)
+    achieved = bytes_alg / (fused_ms * 1e-3) / 1e9
     kname = e.step_kernel_name()
     traffic = None
-    tr = load_traffic(kname)
+    tr = load_traffic(kname, tag)
     if tr:
         traffic = tr.get("bytes_per_launch")
 
@@ -184,13 +267,13 @@ def main():
     if not args.no_cpu_baseline:
         thr = min(16, os.cpu_count() or 1)
         t0 = time.time()
-        base = cpu_baseline(e, w["inp"], thr)
+        base = cpu_baseline(e, w["inp"], thr, dia=w["dia"], start=start)
         log(f"cpu baseline {base['value']:.3e} particle-steps/s on {thr} threads "
             f"({time.time() - t0:.1f}s)")
 
     value = n * args.steps / el
     out = {
-        "metric": "particle-steps/s at 2M spheres",
+        "metric": metric,
         "value": value,
         "unit": "particle-steps/s",
         "n_gpus": 1,
@@ -203,9 +286,11 @@ def main():
         "dtype": "f64" if prec == G.PREC_FP64 else "f32-pair/f64-state",
         "data": "synthetic",
         "config": {
-            "workload": "packed_bed_2M (dry, monodisperse d=0.3mm, R=30mm, skin 0.1d, mu 0.5)",
-            "n_particles": n, "seed": 0xC3, "parallelism": "single-gpu",
+            "workload": desc,
+            "n_particles": n, "seed": 0xC3 if args.workload == "packed_bed" else 0xC2,
+            "parallelism": "single-gpu",
             "rebuilds_timed": e.n_rebuild - rb0,
+            "launches_timed": launches, "steps_executed": executed,
             "n_nb": round(n_nb, 3), "n_contact": round(n_c, 3), "n_wall_contact": round(n_w, 4),
         },
         "roofline": {

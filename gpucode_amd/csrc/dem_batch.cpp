@@ -370,12 +370,24 @@ static int analyze(const std::vector<dem_ctx *> &cs, hipStream_t st, DevInst *d_
                    AnaArgs *h_ana, double *d_out, double *h_out, double *phi, double *Z, double *top) {
     const int n = (int)cs.size();
     std::vector<DevInst> hi(n);
+    std::vector<int> rebuilt;
     for (int k = 0; k < n; ++k) {
         dem_ctx *c = cs[k];
         if (c->rebuild_pending || !c->has_lists) {
             int rc = dem_do_rebuild(c);  // lists of the current positions
             if (rc) return rc;
+            rebuilt.push_back(k);
         }
+    }
+    for (int k : rebuilt) {  // a truncated list would undercount Z
+        int ov = 0;
+        HIPCHK(hipMemcpyAsync(&ov, cs[k]->overflow, sizeof(int), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        if (ov > 0)
+            return dem_fail(DEM_ERR_OVERFLOW, "instance %d: a neighbour list needed %d slots", k, ov);
+    }
+    for (int k = 0; k < n; ++k) {
+        dem_ctx *c = cs[k];
         hi[k] = c->P;
         AnaArgs &G = h_ana[k];
         G.p = c->P4[c->cur];

@@ -61,9 +61,17 @@ def make_input(NP, d_max, R, H, mu, *, E=70e9, nu=0.25, rho=2500.0, e=0.7, yield
 
 
 def feed_positions(inp, dia, n, seed):
+    """Initial positions from the engine's ParticleFeed, called until n
+    particles are placed (one call places at most 60,000, as in the
+    reference) or a call places none."""
     S, rad, _, _ = derive(inp, dia)
     pos = np.zeros((inp.NP, 3))
-    count, _ = particle_feed(S, rad, pos, 0, seed)
+    count = 0
+    while count < n:
+        new, seed = particle_feed(S, rad, pos, count, seed)
+        if new <= count:
+            break
+        count = new
     if count < n:
         raise RuntimeError(f"feed placed only {count} of {n} particles")
     return pos[:n].copy()
