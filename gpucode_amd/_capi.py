@@ -128,6 +128,8 @@ _SIGS = {
     "dem_sync": (C.c_int, [P]),
     "dem_set_kernel": (C.c_int, [P, C.c_int]),
     "dem_set_order": (C.c_int, [P, C.c_int]),
+    "dem_set_relayout": (C.c_int, [P, C.c_int]),
+    "dem_n_relayout": (C.c_int, [P]),
     "dem_iter": (C.c_int, [P]),
     "dem_ctime": (C.c_double, [P]),
     "dem_n_rebuild": (C.c_int, [P]),

@@ -169,7 +169,7 @@ static int alloc_all(dem_ctx *c) {
     if ((rc = dalloc(c, &c->cstart, (size_t)c->S.sysNCells)) ||
         (rc = dalloc(c, &c->cend, (size_t)c->S.sysNCells)) || (rc = dalloc(c, &c->keys, cap)) ||
         (rc = dalloc(c, &c->vals, cap)) || (rc = dalloc(c, &c->keys2, cap)) ||
-        (rc = dalloc(c, &c->vals2, cap)) ||
+        (rc = dalloc(c, &c->vals2, cap)) || (rc = dalloc(c, &c->nmask, cap)) ||
         (rc = dalloc(c, &c->sort_tmp, (size_t)radix_tmp_elems((long)cap))))
         return rc;
     for (int q = 0; q < 7; ++q)
@@ -191,7 +191,7 @@ static void free_all(dem_ctx *c) {
     }
     (void)hipFree(c->cstart); (void)hipFree(c->cend);
     (void)hipFree(c->keys); (void)hipFree(c->vals); (void)hipFree(c->keys2); (void)hipFree(c->vals2);
-    (void)hipFree(c->sort_tmp);
+    (void)hipFree(c->sort_tmp); (void)hipFree(c->nmask);
     for (int q = 0; q < 7; ++q) (void)hipFree(c->F[q]);
     (void)hipFree(c->stop_at); (void)hipFree(c->overflow); (void)hipFree(c->upd_count);
     (void)hipFree(c->stats);

@@ -195,9 +195,11 @@ static void rebuild_entry(dem_ctx *c, int k, RbInst &R) {
     R.cstart = c->cstart; R.cend = c->cend; R.ncells = c->S.sysNCells;
     const int o = 1 - c->hcur;
     R.nbr = c->nbr[c->hcur]; R.nnb = c->nnb[c->hcur]; R.nbr_old = c->nbr[o]; R.nnb_old = c->nnb[o];
-    R.hp = c->hp[c->hcur]; R.hp_old = c->hp[o];
-    R.amask = c->amask[c->hcur]; R.amask_old = c->amask[o];
-    R.actoff = c->actoff[c->hcur]; R.actoff_old = c->actoff[o];
+    const int ol = 1 - c->hl;
+    R.hp = c->hp[c->hl]; R.hp_old = c->hp[ol];
+    R.amask = c->amask[c->hl]; R.amask_old = c->amask[ol];
+    R.actoff = c->actoff[c->hl]; R.actoff_old = c->actoff[ol];
+    R.nmask = c->nmask;
     R.K = c->K;
     R.wl = c->wl[c->hcur]; R.nwl = c->nwl[c->hcur]; R.wl_old = c->wl[o]; R.nwl_old = c->nwl[o];
     R.hw = c->hw[c->hcur]; R.hw_old = c->hw[o];
@@ -214,6 +216,7 @@ static void rebuild_entry(dem_ctx *c, int k, RbInst &R) {
     c->rebuild_pending = false;
     c->n_rebuild++;
     c->forces_valid = false;
+    c->layout_age = 0;
 }
 
 // rebuilds every instance in need among those with steps left (the small

@@ -29,9 +29,10 @@ StepArgs dem_step_args(dem_ctx *c, bool fused) {
     }
     A.nbr = c->nbr[c->hcur];
     A.nnb = c->nnb[c->hcur];
-    A.h = c->hp[c->hcur];
-    A.amask = c->amask[c->hcur];
-    A.actoff = c->actoff[c->hcur];
+    A.h = c->hp[c->hl];
+    A.amask = c->amask[c->hl];
+    A.actoff = c->actoff[c->hl];
+    A.nmask = c->nmask;
     A.wl = c->wl[c->hcur];
     A.nwl = c->nwl[c->hcur];
     A.hw = c->hw[c->hcur];
@@ -296,7 +297,7 @@ extern "C" int dem_get_hist_pp(dem_ctx *c, int maxK, double *dispt, double *nDsp
     HistRec *v = nullptr;
     int rc = slot_view(c, &v);
     if (rc == DEM_OK) {
-        launch_hist_slots(c->hp[c->hcur], c->nnb[c->hcur], c->amask[c->hcur], c->actoff[c->hcur], v,
+        launch_hist_slots(c->hp[c->hl], c->nnb[c->hcur], c->amask[c->hl], c->actoff[c->hl], v,
                           c->cap, (long)c->K * c->cap, c->total, true, c->stream);
         rc = get_hist(c, v, c->nnb[c->hcur], c->K, maxK, dispt, nDsp, nDsp_mx, plstRad, plstDfm, stage,
                       lqdBrg);
@@ -398,13 +399,16 @@ extern "C" int dem_set_contacts(dem_ctx *c, int maxK, const int *num, const int 
     HIPCHK(hipMemcpyAsync(c->nwl[c->hcur], wc.data(), n * sizeof(int), hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     // restored history goes to the slot-major region (no compact records)
-    launch_fill_i32((int *)c->amask[c->hcur], 0, 2L * n, c->stream);
-    launch_fill_i32(c->actoff[c->hcur], 0, n, c->stream);
+    launch_fill_i32((int *)c->amask[c->hl], 0, 2L * n, c->stream);
+    launch_fill_i32(c->actoff[c->hl], 0, n, c->stream);
+    // no bound on how far the restored lists' particles have moved: every
+    // slot is tested
+    launch_fill_i32((int *)c->nmask, -1, 2L * n, c->stream);
     HistRec *v = nullptr;
     if ((rc = slot_view(c, &v))) return rc;
     rc = put_hist(c, v, maxK, n, dispt, nDsp, nDsp_mx, plstRad, plstDfm, stage, lqdBrg);
     if (rc == DEM_OK)
-        launch_hist_slots(c->hp[c->hcur], c->nnb[c->hcur], c->amask[c->hcur], c->actoff[c->hcur], v,
+        launch_hist_slots(c->hp[c->hl], c->nnb[c->hcur], c->amask[c->hl], c->actoff[c->hl], v,
                           c->cap, (long)c->K * c->cap, n, false, c->stream);
     (void)hipStreamSynchronize(c->stream);
     (void)hipFree(v);
@@ -465,6 +469,7 @@ extern "C" int dem_cnt_hstry_save(dem_ctx *c) {
     if (!c) return dem_fail(DEM_ERR_ARG, "null context");
     if (c->has_lists) {
         c->hcur = 1 - c->hcur;
+        c->hl = 1 - c->hl;
         c->saved = true;
     }
     return DEM_OK;
@@ -488,8 +493,9 @@ extern "C" int dem_bld_nebor_list(dem_ctx *c) {
     launch_fill_i32(c->overflow, 0, 1, s);
     launch_neighbor_find(c->buf(c->cur, c->Lidx), c->level[c->pcur], c->P, c->cstart, c->cend,
                          c->nbr[c->hcur], c->nnb[c->hcur], c->overflow, c->cap, n, s);
-    launch_active_slots(c->P4[c->cur], c->nbr[c->hcur], c->nnb[c->hcur], c->amask[c->hcur],
-                        c->actoff[c->hcur], c->keys2, c->sort_tmp, c->cap, n, s);
+    launch_active_slots(c->P4[c->cur], c->nbr[c->hcur], c->nnb[c->hcur], c->amask[c->hl],
+                        c->actoff[c->hl], c->nmask, near_gap_of(c->P), c->keys2, c->sort_tmp,
+                        c->cap, n, s);
     HIPCHK(hipGetLastError());
     return DEM_OK;
 }
@@ -504,10 +510,10 @@ extern "C" int dem_bld_cir_wall(dem_ctx *c) {
 
 extern "C" int dem_hstry_copy(dem_ctx *c) {
     if (!c) return dem_fail(DEM_ERR_ARG, "null context");
-    const int o = 1 - c->hcur;
+    const int o = 1 - c->hcur, ol = 1 - c->hl;
     launch_hist_remap(c->vals, c->id[c->pcur], c->id[1 - c->pcur], c->nbr[c->hcur], c->nnb[c->hcur],
-                      c->amask[c->hcur], c->actoff[c->hcur], c->nbr[o], c->nnb[o], c->amask[o],
-                      c->actoff[o], c->hp[c->hcur], c->hp[o], c->wl[c->hcur],
+                      c->amask[c->hl], c->actoff[c->hl], c->nbr[o], c->nnb[o], c->amask[ol],
+                      c->actoff[ol], c->hp[c->hl], c->hp[ol], c->wl[c->hcur],
                       c->nwl[c->hcur], c->wl[o], c->nwl[o], c->hw[c->hcur], c->hw[o],
                       c->cap, (long)c->K * c->cap, c->total, c->saved ? c->old_total : 0, c->stream);
     HIPCHK(hipGetLastError());
@@ -525,6 +531,27 @@ int dem_do_rebuild(dem_ctx *c) {
     c->rebuild_pending = false;
     c->n_rebuild++;
     c->forces_valid = false;
+    c->layout_age = 0;
+    return DEM_OK;
+}
+
+// Layout refresh: the compact region is rebuilt from the contacts of now,
+// with the lists unchanged.  Contacts that formed since the layout was made
+// move out of the slot-major region; records only move, so results are
+// unchanged.
+int dem_relayout(dem_ctx *c) {
+    if (!c->has_lists || c->total <= 0) return DEM_OK;
+    const int o = c->hl, n = c->total;
+    c->hl = 1 - c->hl;
+    launch_active_slots(c->P4[c->cur], c->nbr[c->hcur], c->nnb[c->hcur], c->amask[c->hl],
+                        c->actoff[c->hl], nullptr, 0.0, c->keys2, c->sort_tmp, c->cap, n,
+                        c->stream);
+    launch_hist_relayout(c->nnb[c->hcur], c->amask[o], c->actoff[o], c->amask[c->hl],
+                         c->actoff[c->hl], c->hp[o], c->hp[c->hl], c->cap, (long)c->K * c->cap, n,
+                         c->stream);
+    HIPCHK(hipGetLastError());
+    c->layout_age = 0;
+    c->n_relayout++;
     return DEM_OK;
 }
 
@@ -601,6 +628,14 @@ int dem_step_kernel(const dem_ctx *c) {
     return c->variant == DEM_KERNEL_AUTO ? DEM_KERNEL_WAVE : c->variant;
 }
 
+extern "C" int dem_set_relayout(dem_ctx *c, int every) {
+    if (!c || every < 0) return dem_fail(DEM_ERR_ARG, "bad argument");
+    c->relayout_every = every;
+    return DEM_OK;
+}
+
+extern "C" int dem_n_relayout(dem_ctx *c) { return c ? c->n_relayout : 0; }
+
 extern "C" int dem_set_kernel(dem_ctx *c, int variant) {
     if (!c) return dem_fail(DEM_ERR_ARG, "null context");
     if (variant < DEM_KERNEL_AUTO || variant > DEM_KERNEL_WAVE)
@@ -656,6 +691,7 @@ int dem_finish_chunk(dem_ctx *c, int nsteps, int stop) {
     for (int s = 0; s < done; ++s) c->Tw = c->Tw + c->S.topv * c->S.dt;
     for (int s = 0; s < done; ++s) c->ctime = c->ctime + c->S.dt;
     c->iter += done;
+    c->layout_age += done;
     if (stop <= nsteps) c->rebuild_pending = true;
     return done;
 }
@@ -672,6 +708,8 @@ extern "C" int dem_step(dem_ctx *c, int nsteps) {
             float ms = 0.f;
             HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
             c->rebuild_ms += ms;
+        } else if (c->relayout_every > 0 && c->layout_age >= c->relayout_every) {
+            if ((rc = dem_relayout(c))) return rc;
         }
         int chunk = nsteps - done < kChunk ? nsteps - done : kChunk;
         int ex = 0;

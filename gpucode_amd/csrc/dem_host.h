@@ -40,13 +40,19 @@ struct dem_ctx {
     int *level[2] = {}, *id[2] = {};
     int pcur = 0;
 
-    // lists and history, current = [hcur], old = [1-hcur]
+    // lists and wall history, current = [hcur], old = [1-hcur]; the
+    // particle-pair history and its layout, current = [hl], old = [1-hl]
+    // (both flip at a rebuild, hl alone at a layout refresh)
     int *nbr[2] = {}, *nnb[2] = {};
     int *wl[2] = {}, *nwl[2] = {};
     HistRec *hp[2] = {}, *hw[2] = {};   // hp: compact region K*cap, then slot-major K*cap
     unsigned long long *amask[2] = {};  // compact-record slots of each particle
     int *actoff[2] = {};                // start of each particle's compact records
-    int hcur = 0;
+    unsigned long long *nmask = nullptr;  // slots that can touch before the next rebuild
+    int hcur = 0, hl = 0;
+    int relayout_every = 512;    // steps between layout refreshes (0: never)
+    long long layout_age = 0;    // steps since the layout was made
+    int n_relayout = 0;
     bool has_lists = false, saved = false;
     int old_total = 0;           // particles covered by the old lists
 

@@ -7,7 +7,12 @@
 //   lqd [cap] int        bridge count, two buffers (lqdCnt / lqdCnt_Old)
 //   nbr [K*cap] int      slot-major: nbr[k*cap + i] = sorted index of the
 //                        k-th neighbour of i; nnb[cap] counts
-//   H  [K*cap] HistRec   64-byte contact history per directed slot, slot-major
+//   H  [2*K*cap] HistRec 64-byte contact history, one record per directed
+//                        slot: a compact region (the slots in contact when
+//                        the layout was made, per particle from actoff[i],
+//                        bit k of amask[i] marking them), then a slot-major
+//                        region for every other slot (see hist_idx)
+//   nmask [cap] u64      slots that can touch before the next rebuild
 //   wl [3*cap] int, nwl [cap], HW [3*cap] HistRec  wall slots
 // Buffers A (read) and B (written) of P/VW swap every fused step.  One
 // record per particle or slot means a gather touches one 32/64-byte sector
@@ -62,6 +67,12 @@ struct DevInst {
     unsigned mortonOff[DEM_MAXL];  // key offset of each level's Morton range
 };
 
+// Gap at a rebuild above which a pair cannot touch before the next rebuild:
+// until then each centre stays within rebuildThr of where it was, so a gap
+// shrinks by at most 2 rebuildThr; the factor 2.2 adds a margin far above
+// rounding.
+__host__ __device__ inline double near_gap_of(const DevInst &P) { return 2.2 * P.rebuildThr; }
+
 // Seven history fields of one directed contact (heydem.h:153-190) plus the
 // flags word: stage in bits 0-7, lqdBrg in bit 8.
 struct __attribute__((aligned(64))) HistRec {
@@ -84,6 +95,7 @@ struct StepArgs {
     HistRec *h;                // history (see hist_idx)
     const unsigned long long *amask;  // per particle: slots whose records are in the compact region
     const int *actoff;         // per particle: start of its compact records
+    const unsigned long long *nmask;  // per particle: slots that can touch before the next rebuild
     const int *wl, *nwl;
     HistRec *hw;
     double *Fx, *Fy, *Fz, *Mx, *My, *Mz, *Mr;  // optional force output
@@ -132,6 +144,7 @@ struct RbInst {
     const unsigned long long *amask_old;
     int *actoff;
     const int *actoff_old;
+    unsigned long long *nmask;
     int *wl, *nwl;
     const int *wl_old, *nwl_old;
     HistRec *hw;

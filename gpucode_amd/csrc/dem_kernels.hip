@@ -503,39 +503,79 @@ DEV long hist_idx(unsigned long long am, int off, int k, int i, long cap, long K
     return Kcap + (long)k * cap + i;
 }
 
-// slots of particle i (the first 64) in contact or within 1 % of the
-// diameter sum of it, which are the pairs likely to touch until the next
-// rebuild: amask[i] and their count.  (Only the layout depends on this.)
-#define DEM_ACTIVE_GAP 0.01
+// slots of particle i (the first 64) in contact now: amask[i] and their
+// count.  Only the layout depends on this.  Contacts that form later sit in
+// the slot-major region until the next rebuild or layout refresh; keeping
+// near-contacts in the compact region as well leaves holes between the
+// records that are read, and costs more than it saves.
+//
+// At a rebuild (nmask given) also the slots that can touch before the next
+// one: a rebuild comes as soon as a particle's path length passes
+// rebuildThr, so until then no centre moves farther than that from where
+// it was at the rebuild, and a pair's gap shrinks by at most 2 rebuildThr.
+// Slots with a larger gap now (near_gap_of, with a 10 % margin) cannot
+// touch, and the wave step kernel does not test them.  Bits cover the
+// first 64 slots; the kernels that use nmask need K <= 64.
 DEV void active_slots_one(const double4 *p, const int *nbr, const int *nnb, unsigned long long *amask,
-                          unsigned *cnt, int cap, int i) {
+                          unsigned *cnt, unsigned long long *nmask, double near_gap, int cap,
+                          int i) {
     const int n = nnb[i];
     const double4 pi = p[i];
-    unsigned long long m = 0ull;
+    unsigned long long m = 0ull, nm = 0ull;
     for (int a = 0; a < n && a < 64; ++a) {
         const double4 q = p[nbr[(long)a * cap + i]];
         const double dx = pi.x - q.x, dy = pi.y - q.y, dz = pi.z - q.z;
         const double rs = pi.w + q.w;
-        if (sqrt(dx * dx + dy * dy + dz * dz) - rs < DEM_ACTIVE_GAP * 2.0 * rs) m |= 1ull << a;
+        const double gap = sqrt(dx * dx + dy * dy + dz * dz) - rs;
+        if (gap < 0.0) m |= 1ull << a;
+        if (gap < near_gap) nm |= 1ull << a;
     }
     amask[i] = m;
     cnt[i] = (unsigned)__popcll(m);
+    if (nmask) nmask[i] = nm;
 }
 
 __global__ void k_active_slots(const double4 *p, const int *nbr, const int *nnb,
-                               unsigned long long *amask, unsigned *cnt, int cap, int total) {
+                               unsigned long long *amask, unsigned *cnt,
+                               unsigned long long *nmask, double near_gap, int cap, int total) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < total) active_slots_one(p, nbr, nnb, amask, cnt, cap, i);
+    if (i < total) active_slots_one(p, nbr, nnb, amask, cnt, nmask, near_gap, cap, i);
 }
 
-// amask and actoff of the new lists (cnt is scratch of total entries)
+// amask and actoff of the lists (cnt is scratch of total entries), and at a
+// rebuild nmask (null at a layout refresh)
 void launch_active_slots(const double4 *p, const int *nbr, const int *nnb, unsigned long long *amask,
-                         int *actoff, unsigned *cnt, unsigned *scan_tmp, int cap, int total,
-                         hipStream_t s) {
+                         int *actoff, unsigned long long *nmask, double near_gap, unsigned *cnt,
+                         unsigned *scan_tmp, int cap, int total, hipStream_t s) {
     if (total <= 0) return;
     hipLaunchKernelGGL(k_active_slots, dim3(blocks_for(total, 256)), dim3(256), 0, s, p, nbr, nnb, amask,
-                       cnt, cap, total);
+                       cnt, nmask, near_gap, cap, total);
     launch_excl_scan(cnt, (unsigned *)actoff, total, scan_tmp, s);
+}
+
+// layout refresh: the same lists, records moved from one layout (am_old,
+// off_old in h_old) to another (am_new, off_new in h); every listed slot's
+// record is copied unchanged
+__global__ void k_hist_relayout(const int *nnb, const unsigned long long *am_old,
+                                const int *off_old, const unsigned long long *am_new,
+                                const int *off_new, const HistRec *h_old, HistRec *h, int cap,
+                                long Kcap, int total) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    const unsigned long long amo = am_old[i], amn = am_new[i];
+    const int ofo = off_old[i], ofn = off_new[i];
+    const int n = nnb[i];
+    for (int k = 0; k < n; ++k)
+        h[hist_idx(amn, ofn, k, i, cap, Kcap)] = h_old[hist_idx(amo, ofo, k, i, cap, Kcap)];
+}
+
+void launch_hist_relayout(const int *nnb, const unsigned long long *am_old, const int *off_old,
+                          const unsigned long long *am_new, const int *off_new,
+                          const HistRec *h_old, HistRec *h, int cap, long Kcap, int total,
+                          hipStream_t s) {
+    if (total <= 0) return;
+    hipLaunchKernelGGL(k_hist_relayout, dim3(blocks_for(total, 256)), dim3(256), 0, s, nnb, am_old,
+                       off_old, am_new, off_new, h_old, h, cap, Kcap, total);
 }
 
 DEV void hist_remap_one(const unsigned *perm, const int *id_new, const int *id_old, const int *nbr,
@@ -1467,10 +1507,9 @@ DEV void step_wave_block(const DevInst &P, const StepArgs &A, unsigned blk, doub
     const bool own = i < A.total;
     const long cap = A.cap;
 
-    // phase A: contact masks of the pair slots and the wall slots.  The
-    // first batch of neighbour indices is read together with the counts (in
-    // bounds whatever the count is: the list holds K slots per particle);
-    // entries past the count are masked after.
+    // phase A: contact masks of the pair slots and the wall slots.  Only
+    // the slots of nmask are tested (the others cannot touch before the
+    // next rebuild), WV_NB at a time.
     unsigned long long mask = 0ull, am_own = 0ull;
     unsigned wmask = 0u;
     int aoff_own = 0;
@@ -1478,24 +1517,22 @@ DEV void step_wave_block(const DevInst &P, const StepArgs &A, unsigned blk, doub
     if (own) {
         am_own = A.amask[i];
         aoff_own = A.actoff[i];
-        int jj[WV_NB];
-        const int kb = A.K < WV_NB ? A.K : WV_NB;
-#pragma unroll
-        for (int u = 0; u < WV_NB; ++u) jj[u] = u < kb ? A.nbr[(long)u * cap + i] : i;
+        unsigned long long nm = A.nmask[i];
         const double4 pi = A.in.p[i];
         const int n = A.nnb[i], nw = A.nwl[i];
         st->p[lane] = pi;
         st->va[lane] = A.in.v[2 * i];
         st->vb[lane] = A.in.v[2 * i + 1];
-        for (int k0 = 0; k0 < n; k0 += WV_NB) {
-            if (k0 > 0) {
+        if (n < 64) nm &= (1ull << n) - 1ull;
+        while (nm) {
+            int ks[WV_NB], jj[WV_NB];
 #pragma unroll
-                for (int u = 0; u < WV_NB; ++u)
-                    jj[u] = k0 + u < n ? A.nbr[(long)(k0 + u) * cap + i] : i;
-            } else {
-#pragma unroll
-                for (int u = 0; u < WV_NB; ++u) jj[u] = u < n ? jj[u] : i;
+            for (int u = 0; u < WV_NB; ++u) {
+                ks[u] = nm ? __ffsll((long long)nm) - 1 : -1;
+                nm &= nm - 1ull;
             }
+#pragma unroll
+            for (int u = 0; u < WV_NB; ++u) jj[u] = ks[u] >= 0 ? A.nbr[(long)ks[u] * cap + i] : i;
             double4 q[WV_NB];
 #pragma unroll
             for (int u = 0; u < WV_NB; ++u) q[u] = A.in.p[jj[u]];
@@ -1503,7 +1540,7 @@ DEV void step_wave_block(const DevInst &P, const StepArgs &A, unsigned blk, doub
             for (int u = 0; u < WV_NB; ++u) {
                 V3 nv;
                 double gap;
-                if (k0 + u < n && pair_geometry<MIXED>(pi, q[u], nv, gap)) mask |= 1ull << (k0 + u);
+                if (ks[u] >= 0 && pair_geometry<MIXED>(pi, q[u], nv, gap)) mask |= 1ull << ks[u];
             }
         }
         const V3 ip = v3(pi.x, pi.y, pi.z);
@@ -1941,11 +1978,14 @@ __global__ void __launch_bounds__(RB_BS) kb_lists(const DevInst *insts, const Rb
     wall_list_one(R.pb, P, R.Tw, R.wl, R.nwl, R.cap, i);
 }
 
-__global__ void __launch_bounds__(RB_BS) kb_active(const RbInst *tab, const int2 *tasks) {
+__global__ void __launch_bounds__(RB_BS) kb_active(const DevInst *insts, const RbInst *tab,
+                                                  const int2 *tasks) {
     const int2 tk = tasks[blockIdx.x];
     const RbInst &R = tab[tk.x];
     const int i = tk.y * RB_BS + threadIdx.x;
-    if (i < R.total) active_slots_one(R.pb, R.nbr, R.nnb, R.amask, R.keys2, R.cap, i);
+    if (i < R.total)
+        active_slots_one(R.pb, R.nbr, R.nnb, R.amask, R.keys2, R.nmask,
+                         near_gap_of(insts[R.inst]), R.cap, i);
 }
 
 // actoff = exclusive prefix sum of the per-particle counts, one block per
@@ -1985,7 +2025,7 @@ void launch_rebuild_batch(const DevInst *insts, const RbInst *tab, int ninst, co
     if (nptasks > 0) {
         hipLaunchKernelGGL(kb_cell_range, dim3(nptasks), dim3(RB_BS), 0, s, insts, tab, ptasks);
         hipLaunchKernelGGL(kb_lists, dim3(nptasks), dim3(RB_BS), 0, s, insts, tab, ptasks);
-        hipLaunchKernelGGL(kb_active, dim3(nptasks), dim3(RB_BS), 0, s, tab, ptasks);
+        hipLaunchKernelGGL(kb_active, dim3(nptasks), dim3(RB_BS), 0, s, insts, tab, ptasks);
         hipLaunchKernelGGL(kb_actoff, dim3(ninst), dim3(SCAN_BS), 0, s, tab);
         hipLaunchKernelGGL(kb_hist_remap, dim3(nptasks), dim3(RB_BS), 0, s, tab, ptasks);
     }

@@ -32,8 +32,12 @@ void launch_hist_remap(const unsigned *perm, const int *id_new, const int *id_ol
                        const HistRec *hw_old, int cap, long Kcap, int total, int old_total,
                        hipStream_t s);
 void launch_active_slots(const double4 *p, const int *nbr, const int *nnb, unsigned long long *amask,
-                         int *actoff, unsigned *cnt, unsigned *scan_tmp, int cap, int total,
-                         hipStream_t s);
+                         int *actoff, unsigned long long *nmask, double near_gap, unsigned *cnt,
+                         unsigned *scan_tmp, int cap, int total, hipStream_t s);
+void launch_hist_relayout(const int *nnb, const unsigned long long *am_old, const int *off_old,
+                          const unsigned long long *am_new, const int *off_new,
+                          const HistRec *h_old, HistRec *h, int cap, long Kcap, int total,
+                          hipStream_t s);
 void launch_hist_slots(HistRec *h, const int *nnb, const unsigned long long *amask, const int *actoff,
                        HistRec *slots, int cap, long Kcap, int total, bool to_slots, hipStream_t s);
 

@@ -241,6 +241,14 @@ class Engine:
         """Storage order between rebuilds: Morton (True) or row-major."""
         check(A.lib().dem_set_order(self._ctx, int(bool(morton))))
 
+    def set_relayout(self, every):
+        """Steps between history layout refreshes in step() (0: never)."""
+        check(A.lib().dem_set_relayout(self._ctx, int(every)))
+
+    @property
+    def n_relayout(self):
+        return A.lib().dem_n_relayout(self._ctx)
+
     @property
     def iter(self):
         return A.lib().dem_iter(self._ctx)

@@ -235,6 +235,14 @@ int dem_set_kernel(dem_ctx *ctx, int variant);
  * order of the cells, 0 = the reference's row-major cell hash.  Lists,
  * forces and the exported hash/cell tables are identical either way. */
 int dem_set_order(dem_ctx *ctx, int morton);
+/* History layout refresh in dem_step: every `every` steps (default 1024;
+ * 0 turns it off) without a rebuild, the contact records are regathered
+ * into the compact per-wave region from the contacts of that moment.
+ * Records only move, so results are identical with any setting.  No
+ * reference counterpart (the reference keeps one slot-major layout). */
+int dem_set_relayout(dem_ctx *ctx, int every);
+/* Layout refreshes done so far. */
+int dem_n_relayout(dem_ctx *ctx);
 int dem_iter(dem_ctx *ctx);
 double dem_ctime(dem_ctx *ctx);
 int dem_n_rebuild(dem_ctx *ctx);

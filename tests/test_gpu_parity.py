@@ -211,6 +211,58 @@ def test_compacting_kernel_equals_loop_kernel(precision, kernel):
         assert np.array_equal(ha[f], hb[f]), f
 
 
+@pytest.mark.parametrize("kernel", [G.Engine.KERNEL_LOOP, G.Engine.KERNEL_COMPACT,
+                                    G.Engine.KERNEL_WAVE])
+def test_layout_refresh_changes_nothing(kernel):
+    # a layout refresh only moves history records: with refreshes every 64
+    # steps between rebuilds, the run is bit-identical to one without, and
+    # both match the oracle
+    w = W.small_bed(n=3000)
+    rng = np.random.default_rng(5)
+    vel = rng.normal(size=(3000, 3)) * 20.0
+    a = G.Engine(w["inp"])
+    b = G.Engine(w["inp"])
+    o = O.Oracle(w["inp"])
+    for x in (a, b):
+        x.set_kernel(kernel)
+        x.set_particles(w["pos"], vel)
+        x.store_forces(True)
+    o.set_particles(w["pos"], vel)
+    a.set_relayout(64)
+    b.set_relayout(0)
+    a.step(640)
+    b.step(640)
+    o.step(640)
+    assert a.n_relayout >= 3 and b.n_relayout == 0
+    assert a.n_rebuild == b.n_rebuild == o.n_rebuild
+    for u, v in zip(a.particles(), b.particles()):
+        assert np.array_equal(u, v)
+    for u, v in zip(a.forces(), b.forces()):
+        assert np.array_equal(u, v)
+    ha, hb = a.hist_pp(), b.hist_pp()
+    for f in ha:
+        assert np.array_equal(ha[f], hb[f]), f
+    compare_state(a, o)
+
+
+def test_layout_refresh_wet_polydisperse():
+    w = W.wet_bed(n=3000, poly=True, reference_law=False, theta_deg=20.0, liquid=0.2)
+    a = G.Engine(w["inp"], w["dia"])
+    b = G.Engine(w["inp"], w["dia"])
+    for x in (a, b):
+        x.set_particles(w["pos"])
+    a.set_relayout(64)
+    b.set_relayout(0)
+    a.step(400)
+    b.step(400)
+    assert a.n_relayout >= 3
+    for u, v in zip(a.particles(), b.particles()):
+        assert np.array_equal(u, v)
+    ha, hb = a.hist_pp(), b.hist_pp()
+    for f in ha:
+        assert np.array_equal(ha[f], hb[f]), f
+
+
 def test_morton_storage_equals_row_major_storage():
     # storage order changes memory locality only: lists (as original ids),
     # the reference's hash/cell tables and the trajectory are identical
