@@ -130,6 +130,8 @@ _SIGS = {
     "dem_set_order": (C.c_int, [P, C.c_int]),
     "dem_set_relayout": (C.c_int, [P, C.c_int]),
     "dem_n_relayout": (C.c_int, [P]),
+    "dem_set_pair_records": (C.c_int, [P, C.c_int]),
+    "dem_pair_records_used": (C.c_int, [P]),
     "dem_iter": (C.c_int, [P]),
     "dem_ctime": (C.c_double, [P]),
     "dem_n_rebuild": (C.c_int, [P]),

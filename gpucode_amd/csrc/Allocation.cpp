@@ -181,6 +181,7 @@ static int alloc_all(dem_ctx *c) {
 }
 
 static void free_all(dem_ctx *c) {
+    dem_pair_free(c);
     for (int b = 0; b < 2; ++b) {
         (void)hipFree(c->P4[b]); (void)hipFree(c->VW[b]);
         (void)hipFree(c->lq[b]); (void)hipFree(c->level[b]);

@@ -283,9 +283,12 @@ static int batch_rebuilds(dem_batch *b, const std::vector<int> &left) {
 extern "C" int dem_batch_step(dem_batch *b, int nsteps) {
     if (!b || nsteps < 0) return dem_fail(DEM_ERR_ARG, "bad argument");
     const int n = (int)b->inst.size();
-    for (dem_ctx *c : b->inst)
-        if (c->precision != b->precision) return dem_fail(DEM_ERR_STATE, "mixed precisions in batch");
     int rc;
+    for (dem_ctx *c : b->inst) {
+        if (c->precision != b->precision) return dem_fail(DEM_ERR_STATE, "mixed precisions in batch");
+        // the batched kernels step the directed records
+        if ((rc = dem_pair_unpack(c))) return rc;
+    }
     if ((rc = upload_insts(b))) return rc;
     std::vector<int> left(n, nsteps);
     const bool mixed = b->precision == DEM_PREC_MIXED;

@@ -62,6 +62,8 @@ static int check_overflow(dem_ctx *c) {
 // state import / export
 // ---------------------------------------------------------------------
 static void reset_lists(dem_ctx *c) {
+    c->pair_packed = false;
+    c->pair_bad = false;
     c->has_lists = false;
     c->saved = false;
     c->old_total = 0;
@@ -126,7 +128,9 @@ extern "C" int dem_set_particles(dem_ctx *c, int total, const double *pos, const
 extern "C" int dem_feed(dem_ctx *c, int n, const double *pos) {
     if (!c || n < 0 || !pos || c->total + n > c->cap) return dem_fail(DEM_ERR_ARG, "feed exceeds NP");
     int off = c->total;
-    int rc = upload_particles(c, off, n, pos, nullptr, nullptr);
+    int rc = dem_pair_unpack(c);
+    if (rc) return rc;
+    rc = upload_particles(c, off, n, pos, nullptr, nullptr);
     if (rc) return rc;
     c->total += n;
     c->rebuild_pending = true;  // FeedFlag (main.cpp:442, 473)
@@ -295,7 +299,9 @@ extern "C" int dem_get_hist_pp(dem_ctx *c, int maxK, double *dispt, double *nDsp
                                double *plstRad, double *plstDfm, int *stage, int *lqdBrg) {
     if (!c || maxK <= 0) return dem_fail(DEM_ERR_ARG, "bad argument");
     HistRec *v = nullptr;
-    int rc = slot_view(c, &v);
+    int rc = dem_pair_unpack(c);
+    if (rc) return rc;
+    rc = slot_view(c, &v);
     if (rc == DEM_OK) {
         launch_hist_slots(c->hp[c->hl], c->nnb[c->hcur], c->amask[c->hl], c->actoff[c->hl], v,
                           c->cap, (long)c->K * c->cap, c->total, true, c->stream);
@@ -367,6 +373,8 @@ extern "C" int dem_set_contacts(dem_ctx *c, int maxK, const int *num, const int 
                                 const int *wlqdBrg) {
     if (!c || !num || !list || !wnum || !wlist || maxK <= 0 || maxK > c->K)
         return dem_fail(DEM_ERR_ARG, "bad argument");
+    c->pair_packed = false;  // the records are replaced
+    c->pair_bad = false;
     const int n = c->total;
     // the restored lists must be in the current (identity) order: ids 0..n-1
     std::vector<int> ids;
@@ -463,10 +471,114 @@ extern "C" int dem_get_cells(dem_ctx *c, int *cellStart, int *cellEnd) {
 }
 
 // ---------------------------------------------------------------------
+// shared pair records of the wave kernel (PAIR_MIRROR in dem_internal.h)
+// ---------------------------------------------------------------------
+void dem_pair_free(dem_ctx *c) {
+    for (int b = 0; b < 2; ++b) {
+        (void)hipFree(c->hq[b]);
+        c->hq[b] = nullptr;
+    }
+    (void)hipFree(c->nrec); (void)hipFree(c->pcnt); (void)hipFree(c->poff);
+    (void)hipFree(c->pair_err);
+    c->nrec = nullptr; c->pcnt = nullptr; c->poff = nullptr; c->pair_err = nullptr;
+    c->hq_cap = c->nrec_cap = 0;
+    c->pair_packed = false;
+}
+
+static int grow(dem_ctx *c, void **p, long *have, long need, size_t elem) {
+    if (need <= *have) return DEM_OK;
+    long n = need + need / 4 + 1024;
+    (void)hipFree(*p);
+    *p = nullptr;
+    *have = 0;
+    HIPCHK(hipMalloc(p, (size_t)n * elem));
+    *have = n;
+    return DEM_OK;
+}
+
+// The directed records of the near slots become pair records (both
+// buffers), with the lists unchanged.  When the lists or the records are
+// not symmetric (k_pair_fill_mirror), the context keeps to the directed
+// records from then on.
+int dem_pair_pack(dem_ctx *c) {
+    if (c->pair_packed || c->pair_bad || !c->has_lists || c->total <= 0) return DEM_OK;
+    const int n = c->total;
+    const long cap = c->cap, Kcap = (long)c->K * c->cap;
+    hipStream_t s = c->stream;
+    if (!c->pcnt) {
+        HIPCHK(hipMalloc((void **)&c->pcnt, 3 * (size_t)cap * sizeof(unsigned)));
+        HIPCHK(hipMalloc((void **)&c->poff, 3 * (size_t)cap * sizeof(unsigned)));
+        HIPCHK(hipMalloc((void **)&c->pair_err, sizeof(int)));
+    }
+    launch_pair_count(c->P4[c->cur], c->nbr[c->hcur], c->nnb[c->hcur], c->nmask, c->pcnt, c->cap, n, s);
+    for (int q = 0; q < 3; ++q)
+        launch_excl_scan(c->pcnt + q * cap, c->poff + q * cap, n, c->sort_tmp, s);
+    unsigned last[6];
+    for (int q = 0; q < 3; ++q) {
+        HIPCHK(hipMemcpyAsync(&last[2 * q], c->pcnt + q * cap + (n - 1), sizeof(unsigned),
+                              hipMemcpyDeviceToHost, s));
+        HIPCHK(hipMemcpyAsync(&last[2 * q + 1], c->poff + q * cap + (n - 1), sizeof(unsigned),
+                              hipMemcpyDeviceToHost, s));
+    }
+    HIPCHK(hipStreamSynchronize(s));
+    const long t1 = (long)last[0] + last[1], t2 = (long)last[2] + last[3], tn = (long)last[4] + last[5];
+    if (t1 + t2 > 0x7fffffffL || tn > 0x7fffffffL)
+        return dem_fail(DEM_ERR_NOMEM, "too many near slots for 32-bit pair record indices");
+    long h0 = c->hq_cap, h1 = c->hq_cap;
+    int rc;
+    if ((rc = grow(c, (void **)&c->hq[0], &h0, t1 + t2, sizeof(HistRec))) ||
+        (rc = grow(c, (void **)&c->hq[1], &h1, t1 + t2, sizeof(HistRec))) ||
+        (rc = grow(c, (void **)&c->nrec, &c->nrec_cap, tn, sizeof(int))))
+        return rc;
+    c->hq_cap = h0 < h1 ? h0 : h1;
+    launch_fill_i32(c->pair_err, 0, 1, s);
+    launch_pair_fill(c->P4[c->cur], c->VW[c->cur], c->nbr[c->hcur], c->nnb[c->hcur], c->nmask, c->poff,
+                     (unsigned)t1, c->hp[c->hl], c->amask[c->hl], c->actoff[c->hl], c->nrec, c->hq[0],
+                     c->hq[1], c->pair_err, c->cap, Kcap, n, s);
+    HIPCHK(hipGetLastError());
+    int bad = 0;
+    HIPCHK(hipMemcpyAsync(&bad, c->pair_err, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (bad) {
+        c->pair_bad = true;
+        return DEM_OK;
+    }
+    c->pair_packed = true;
+    c->hqc = 0;
+    c->layout_age = 0;
+    c->n_pack++;
+    return DEM_OK;
+}
+
+int dem_pair_unpack(dem_ctx *c) {
+    if (!c->pair_packed) return DEM_OK;
+    c->pair_packed = false;
+    launch_pair_unpack(c->nnb[c->hcur], c->nmask, c->poff + 2L * c->cap, c->nrec, c->hq[c->hqc],
+                       c->hp[c->hl], c->amask[c->hl], c->actoff[c->hl], c->cap,
+                       (long)c->K * c->cap, c->total, c->stream);
+    HIPCHK(hipGetLastError());
+    return DEM_OK;
+}
+
+extern "C" int dem_set_pair_records(dem_ctx *c, int on) {
+    if (!c) return dem_fail(DEM_ERR_ARG, "null context");
+    int rc;
+    if (!on && (rc = dem_pair_unpack(c))) return rc;
+    c->pair_on = on != 0;
+    return DEM_OK;
+}
+
+extern "C" int dem_pair_records_used(dem_ctx *c) {
+    return c && c->pair_on && !c->pair_bad && dem_step_kernel(c) == DEM_KERNEL_WAVE;
+}
+
+// ---------------------------------------------------------------------
 // rebuild pieces (main.cpp:473-527)
 // ---------------------------------------------------------------------
 extern "C" int dem_cnt_hstry_save(dem_ctx *c) {
     if (!c) return dem_fail(DEM_ERR_ARG, "null context");
+    int rc = dem_pair_unpack(c);
+    if (rc) return rc;
     if (c->has_lists) {
         c->hcur = 1 - c->hcur;
         c->hl = 1 - c->hl;
@@ -541,6 +653,8 @@ int dem_do_rebuild(dem_ctx *c) {
 // unchanged.
 int dem_relayout(dem_ctx *c) {
     if (!c->has_lists || c->total <= 0) return DEM_OK;
+    int rc = dem_pair_unpack(c);
+    if (rc) return rc;
     const int o = c->hl, n = c->total;
     c->hl = 1 - c->hl;
     launch_active_slots(c->P4[c->cur], c->nbr[c->hcur], c->nnb[c->hcur], c->amask[c->hl],
@@ -576,6 +690,8 @@ extern "C" int dem_var_init(dem_ctx *c) {
 
 extern "C" int dem_pp_inter_calc(dem_ctx *c) {
     if (!c) return dem_fail(DEM_ERR_ARG, "null context");
+    int rc = dem_pair_unpack(c);
+    if (rc) return rc;
     StepArgs A = dem_step_args(c, false);
     launch_pp_force(c->P, A, c->S.forceCap == 1, c->precision == DEM_PREC_MIXED, c->stream);
     HIPCHK(hipGetLastError());
@@ -584,6 +700,8 @@ extern "C" int dem_pp_inter_calc(dem_ctx *c) {
 
 extern "C" int dem_cir_pw_calc(dem_ctx *c) {
     if (!c) return dem_fail(DEM_ERR_ARG, "null context");
+    int rc = dem_pair_unpack(c);
+    if (rc) return rc;
     c->Tw = c->Tw + c->S.topv * c->S.dt;  // kernel.cu:966-969
     StepArgs A = dem_step_args(c, false);
     launch_pw_force(c->P, A, c->precision == DEM_PREC_MIXED, c->stream);
@@ -593,6 +711,8 @@ extern "C" int dem_cir_pw_calc(dem_ctx *c) {
 
 extern "C" int dem_update_system(dem_ctx *c, int *reNebld) {
     if (!c) return dem_fail(DEM_ERR_ARG, "null context");
+    int rc = dem_pair_unpack(c);
+    if (rc) return rc;
     StepArgs A = dem_step_args(c, false);
     launch_fill_i32(c->upd_count, 0, 1, c->stream);
     launch_update(c->P, A, c->upd_count, c->stream);
@@ -643,6 +763,8 @@ extern "C" int dem_set_kernel(dem_ctx *c, int variant) {
     if ((variant == DEM_KERNEL_COMPACT || variant == DEM_KERNEL_WAVE) &&
         (c->S.forceCap == 1 || c->K > 64))
         return dem_fail(DEM_ERR_ARG, "the compacting kernels need a dry packing with <= 64 slots");
+    int rc = dem_pair_unpack(c);
+    if (rc) return rc;
     c->variant = variant;
     return DEM_OK;
 }
@@ -651,9 +773,18 @@ static int run_chunk(dem_ctx *c, int nsteps, int *executed) {
     const bool wet = c->S.forceCap == 1;
     const int kern = dem_step_kernel(c);
     const bool mixed = c->precision == DEM_PREC_MIXED;
+    int rc;
+    if (kern == DEM_KERNEL_WAVE && c->pair_on) rc = dem_pair_pack(c);
+    else rc = dem_pair_unpack(c);
+    if (rc) return rc;
+    const bool pair = c->pair_packed;
     int big = INT_MAX;
     HIPCHK(hipMemcpyAsync(c->stop_at, &big, sizeof(int), hipMemcpyHostToDevice, c->stream));
     StepArgs A = dem_step_args(c, true);
+    if (pair) {
+        A.noff = (const int *)(c->poff + 2L * c->cap);
+        A.nrec = c->nrec;
+    }
     double tw = c->Tw;
     HIPCHK(hipEventRecord(c->ev0, c->stream));
     for (int s = 0; s < nsteps; ++s) {
@@ -663,8 +794,12 @@ static int run_chunk(dem_ctx *c, int nsteps, int *executed) {
         A.step = s;
         tw = tw + c->S.topv * c->S.dt;
         A.Tw = tw;
+        if (pair) {
+            A.hq_in = c->hq[(c->hqc + s) & 1];
+            A.hq_out = c->hq[(c->hqc + s + 1) & 1];
+        }
         if (kern == DEM_KERNEL_WAVE)
-            launch_step_wave(c->P, A, mixed, c->stream);
+            launch_step_wave(c->P, A, mixed, pair, c->stream);
         else if (kern == DEM_KERNEL_COMPACT)
             launch_step_compact(c->P, A, c->K, mixed, c->stream);
         else
@@ -688,6 +823,7 @@ int dem_finish_chunk(dem_ctx *c, int nsteps, int stop) {
     c->fused_steps += done;
     c->cur = (c->cur + done) & 1;
     c->Lidx = (c->Lidx + done) & 1;
+    if (c->pair_packed) c->hqc = (c->hqc + done) & 1;
     for (int s = 0; s < done; ++s) c->Tw = c->Tw + c->S.topv * c->S.dt;
     for (int s = 0; s < done; ++s) c->ctime = c->ctime + c->S.dt;
     c->iter += done;
@@ -709,7 +845,14 @@ extern "C" int dem_step(dem_ctx *c, int nsteps) {
             HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
             c->rebuild_ms += ms;
         } else if (c->relayout_every > 0 && c->layout_age >= c->relayout_every) {
-            if ((rc = dem_relayout(c))) return rc;
+            // with pair records the refresh is a write-back and a new pack
+            // (run_chunk), which puts the contacts of now first
+            if (c->pair_packed) {
+                if ((rc = dem_pair_unpack(c))) return rc;
+                c->n_relayout++;
+            } else if ((rc = dem_relayout(c))) {
+                return rc;
+            }
         }
         int chunk = nsteps - done < kChunk ? nsteps - done : kChunk;
         int ex = 0;
@@ -804,6 +947,7 @@ extern "C" int dem_contact_stats(dem_ctx *c, dem_stats *out) {
     if (!c || !out) return dem_fail(DEM_ERR_ARG, "bad argument");
     int rc;
     if (c->rebuild_pending && (rc = dem_rebuild(c))) return rc;
+    if ((rc = dem_pair_unpack(c))) return rc;
     unsigned long long h[8] = {0};
     HIPCHK(hipMemcpyAsync(c->stats, h, sizeof h, hipMemcpyHostToDevice, c->stream));
     StepArgs A = dem_step_args(c, true);

@@ -50,6 +50,20 @@ struct dem_ctx {
     int *actoff[2] = {};                // start of each particle's compact records
     unsigned long long *nmask = nullptr;  // slots that can touch before the next rebuild
     int hcur = 0, hl = 0;
+    // shared pair records of the wave kernel (PAIR_MIRROR in dem_internal.h),
+    // made from the directed records before a chunk and written back before
+    // anything else reads the history
+    bool pair_on = true;         // use them when dem_step runs the wave kernel
+    bool pair_bad = false;       // a mirror record did not match: directed records only
+    bool pair_packed = false;    // the pair records hold the current history
+    int hqc = 0;                 // the pair buffer the next step reads
+    HistRec *hq[2] = {};
+    long hq_cap = 0;             // records per pair buffer
+    int *nrec = nullptr;         // near slot -> pair record (PAIR_MIRROR on the mirror side)
+    long nrec_cap = 0;
+    unsigned *pcnt = nullptr, *poff = nullptr;  // 3 x cap counts and their scans
+    int *pair_err = nullptr;
+    int n_pack = 0;
     int relayout_every = 512;    // steps between layout refreshes (0: never)
     long long layout_age = 0;    // steps since the layout was made
     int n_relayout = 0;
@@ -98,6 +112,12 @@ void dem_fill_inst(const dem_setup &S, DevInst &P);
 void dem_refresh_inst(dem_ctx *c);
 // Internal pieces of the rebuild and step, shared by the C ABI functions.
 int dem_do_rebuild(dem_ctx *c);
+// shared pair records: make them from the directed records, or write them
+// back (a no-op when they are not in use)
+int dem_pair_pack(dem_ctx *c);
+int dem_pair_unpack(dem_ctx *c);
+// frees the pair buffers
+void dem_pair_free(dem_ctx *c);
 // StepArgs of the context's current buffers (fused: read cur, write 1-cur)
 StepArgs dem_step_args(dem_ctx *c, bool fused);
 // DEM_KERNEL_LOOP, _COMPACT or _WAVE: the kernel dem_step launches

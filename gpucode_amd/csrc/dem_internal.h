@@ -80,6 +80,25 @@ struct __attribute__((aligned(64))) HistRec {
     int flags, pad;
 };
 
+// Shared pair records.  For a pair of equal radii the reference's two
+// directed evaluations, (i, j) and (j, i), compute the same overlap, the
+// same load/unload verdict and a tangential spring of opposite sign, bit
+// for bit: every operation either repeats on identical operands or acts on
+// exactly negated ones (d = (|dx| - r) - r, n and the contact-point
+// velocity flip sign, round-to-nearest is symmetric).  Their history
+// records therefore stay exact mirrors, and one record per pair serves
+// both: the owner (the lower storage index) updates it, the mirror side
+// reads it with the spring negated and writes nothing.  Pairs of unequal
+// radii ((|dx| - ri) - rj and (|dx| - rj) - ri can round apart) keep one
+// record per direction, each side the owner of its own.  The records are
+// double-buffered (read hq_in, write hq_out), so the mirror side never sees
+// the owner's update of the same step; a record that is not updated in a
+// step is identical in both buffers, except one updated in the previous
+// step, which the owner copies forward once when the contact ends (bit k of
+// the contact mask kept in the spare word of the particle's velocity
+// record).
+#define PAIR_MIRROR 0x80000000u
+
 typedef double4 dvec4;
 
 struct StateBuf {
@@ -96,6 +115,13 @@ struct StepArgs {
     const unsigned long long *amask;  // per particle: slots whose records are in the compact region
     const int *actoff;         // per particle: start of its compact records
     const unsigned long long *nmask;  // per particle: slots that can touch before the next rebuild
+    // shared pair records (the wave kernel's pair mode, see PAIR_MIRROR):
+    // near slot r of particle i (r-th set bit of nmask[i]) has
+    // nrec[noff[i] + r] = its pair record index, bit 31 set on the mirror
+    // side; records are read from hq_in and written to hq_out
+    const int *noff, *nrec;
+    const HistRec *hq_in;
+    HistRec *hq_out;
     const int *wl, *nwl;
     HistRec *hw;
     double *Fx, *Fy, *Fz, *Mx, *My, *Mz, *Mr;  // optional force output

@@ -9,6 +9,8 @@
 #include <math.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "dem_internal.h"
 #include "dem_launch.h"
 
@@ -682,6 +684,184 @@ void launch_hist_remap(const unsigned *perm, const int *id_new, const int *id_ol
 }
 
 // ---------------------------------------------------------------------
+// Shared pair records (see PAIR_MIRROR in dem_internal.h): made from the
+// directed records of the near slots (nmask), and written back to them,
+// with the lists unchanged.  A slot owns its record unless its partner has
+// the same radius and a lower storage index; owned records of pairs in
+// contact now come first, in (particle, slot) order, then the other owned
+// records, so that the records a step reads are contiguous per wave.
+// ---------------------------------------------------------------------
+DEV unsigned long long near_slots(const unsigned long long *nmask, const int *nnb, int i) {
+    const int n = nnb[i];
+    unsigned long long nm = nmask[i];
+    if (n < 64) nm &= (1ull << n) - 1ull;
+    return nm;
+}
+
+DEV bool owns_record(int i, int j, double ri, double rj) { return j > i || rj != ri; }
+
+// per particle: owned near slots in contact now (c[0]), other owned near
+// slots (c[1]), near slots (c[2]); c[q] is c + q*cap
+__global__ void k_pair_count(const double4 *p, const int *nbr, const int *nnb,
+                             const unsigned long long *nmask, unsigned *c, int cap, int total) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    const double4 pi = p[i];
+    unsigned long long nm = near_slots(nmask, nnb, i);
+    unsigned a = 0, b = 0;
+    const unsigned n = (unsigned)__popcll(nm);
+    while (nm) {
+        const int k = __ffsll((long long)nm) - 1;
+        nm &= nm - 1ull;
+        const int j = nbr[(long)k * cap + i];
+        const double4 q = p[j];
+        if (!owns_record(i, j, pi.w, q.w)) continue;
+        const double dx = pi.x - q.x, dy = pi.y - q.y, dz = pi.z - q.z;
+        const double gap = sqrt(dx * dx + dy * dy + dz * dz) - pi.w - q.w;
+        if (gap < 0.0) a++;
+        else b++;
+    }
+    c[i] = a;
+    c[(long)cap + i] = b;
+    c[2L * cap + i] = n;
+}
+
+// owned slots: assign records, copy the directed records into both
+// buffers; mirror slots are marked -1 until k_pair_fill_mirror.  Clears
+// the contact mask in the spare word of the velocity records the next step
+// reads.
+__global__ void k_pair_fill_owner(const double4 *p, double4 *vw, const int *nbr, const int *nnb,
+                                  const unsigned long long *nmask, const unsigned *o, unsigned base2,
+                                  const HistRec *hd, const unsigned long long *amask,
+                                  const int *actoff, int *nrec, HistRec *q0, HistRec *q1, int cap,
+                                  long Kcap, int total) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    vw[2L * i + 1].w = 0.0;
+    const double4 pi = p[i];
+    unsigned long long nm = near_slots(nmask, nnb, i);
+    const unsigned long long am = amask[i];
+    const int ao = actoff[i];
+    unsigned r1 = o[i], r2 = base2 + o[(long)cap + i], slot = o[2L * cap + i];
+    while (nm) {
+        const int k = __ffsll((long long)nm) - 1;
+        nm &= nm - 1ull;
+        const int j = nbr[(long)k * cap + i];
+        const double4 q = p[j];
+        int rec = -1;
+        if (owns_record(i, j, pi.w, q.w)) {
+            const double dx = pi.x - q.x, dy = pi.y - q.y, dz = pi.z - q.z;
+            const double gap = sqrt(dx * dx + dy * dy + dz * dz) - pi.w - q.w;
+            rec = (int)(gap < 0.0 ? r1++ : r2++);
+            const HistRec h = hd[hist_idx(am, ao, k, i, cap, Kcap)];
+            q0[rec] = h;
+            q1[rec] = h;
+        }
+        nrec[slot++] = rec;
+    }
+}
+
+// mirror slots: the record of the partner's slot for i.  Sets *bad when
+// the partner's list has no near slot for i or when the directed record
+// of this slot is not the exact mirror of the partner's (then the pair
+// records cannot stand for the directed ones).
+__global__ void k_pair_fill_mirror(const double4 *p, const int *nbr, const int *nnb,
+                                   const unsigned long long *nmask, const unsigned *o,
+                                   const HistRec *hd, const unsigned long long *amask,
+                                   const int *actoff, int *nrec, const HistRec *q0, int *bad,
+                                   int cap, long Kcap, int total) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    const double4 pi = p[i];
+    unsigned long long nm = near_slots(nmask, nnb, i);
+    const unsigned long long am = amask[i];
+    const int ao = actoff[i];
+    const unsigned *noff = o + 2L * cap;
+    unsigned slot = noff[i];
+    while (nm) {
+        const int k = __ffsll((long long)nm) - 1;
+        nm &= nm - 1ull;
+        const unsigned me = slot++;
+        const int j = nbr[(long)k * cap + i];
+        if (owns_record(i, j, pi.w, p[j].w)) continue;
+        const int nj = nnb[j];
+        int kk = -1;
+        for (int b = 0; b < nj && kk < 0; ++b)
+            if (nbr[(long)b * cap + j] == i) kk = b;
+        const unsigned long long nmj = near_slots(nmask, nnb, j);
+        if (kk < 0 || kk >= 64 || !((nmj >> kk) & 1ull)) {
+            *bad = 1;
+            nrec[me] = 0;
+            continue;
+        }
+        const int rec = nrec[noff[j] + (unsigned)__popcll(nmj & ((1ull << kk) - 1ull))];
+        if (rec < 0) {
+            *bad = 1;
+            nrec[me] = 0;
+            continue;
+        }
+        const HistRec ho = q0[rec], hm = hd[hist_idx(am, ao, k, i, cap, Kcap)];
+        if (!(hm.dx == -ho.dx && hm.dy == -ho.dy && hm.dz == -ho.dz && hm.nd == ho.nd &&
+              hm.ndm == ho.ndm && hm.prad == ho.prad && hm.pdfm == ho.pdfm && hm.flags == ho.flags))
+            *bad = 1;
+        nrec[me] = (int)((unsigned)rec | PAIR_MIRROR);
+    }
+}
+
+// the pair records (buffer q, the one the next step reads) back into the
+// directed records of the near slots
+__global__ void k_pair_unpack(const int *nnb, const unsigned long long *nmask, const unsigned *noff,
+                              const int *nrec, const HistRec *q, HistRec *hd,
+                              const unsigned long long *amask, const int *actoff, int cap,
+                              long Kcap, int total) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    unsigned long long nm = near_slots(nmask, nnb, i);
+    const unsigned long long am = amask[i];
+    const int ao = actoff[i];
+    unsigned slot = noff[i];
+    while (nm) {
+        const int k = __ffsll((long long)nm) - 1;
+        nm &= nm - 1ull;
+        const int r = nrec[slot++];
+        HistRec h = q[r & 0x7fffffff];
+        if (r < 0) {
+            h.dx = -h.dx; h.dy = -h.dy; h.dz = -h.dz;
+        }
+        hd[hist_idx(am, ao, k, i, cap, Kcap)] = h;
+    }
+}
+
+void launch_pair_count(const double4 *p, const int *nbr, const int *nnb,
+                       const unsigned long long *nmask, unsigned *c, int cap, int total,
+                       hipStream_t s) {
+    if (total <= 0) return;
+    hipLaunchKernelGGL(k_pair_count, dim3(blocks_for(total, 256)), dim3(256), 0, s, p, nbr, nnb, nmask,
+                       c, cap, total);
+}
+
+void launch_pair_fill(const double4 *p, double4 *vw, const int *nbr, const int *nnb,
+                      const unsigned long long *nmask, const unsigned *o, unsigned base2,
+                      const HistRec *hd, const unsigned long long *amask, const int *actoff,
+                      int *nrec, HistRec *q0, HistRec *q1, int *bad, int cap, long Kcap, int total,
+                      hipStream_t s) {
+    if (total <= 0) return;
+    hipLaunchKernelGGL(k_pair_fill_owner, dim3(blocks_for(total, 256)), dim3(256), 0, s, p, vw, nbr,
+                       nnb, nmask, o, base2, hd, amask, actoff, nrec, q0, q1, cap, Kcap, total);
+    hipLaunchKernelGGL(k_pair_fill_mirror, dim3(blocks_for(total, 256)), dim3(256), 0, s, p, nbr, nnb,
+                       nmask, o, hd, amask, actoff, nrec, q0, bad, cap, Kcap, total);
+}
+
+void launch_pair_unpack(const int *nnb, const unsigned long long *nmask, const unsigned *noff,
+                        const int *nrec, const HistRec *q, HistRec *hd,
+                        const unsigned long long *amask, const int *actoff, int cap, long Kcap,
+                        int total, hipStream_t s) {
+    if (total <= 0) return;
+    hipLaunchKernelGGL(k_pair_unpack, dim3(blocks_for(total, 256)), dim3(256), 0, s, nnb, nmask, noff,
+                       nrec, q, hd, amask, actoff, cap, Kcap, total);
+}
+
+// ---------------------------------------------------------------------
 // contact laws, FP64 (kernel.cu:89-359), same evaluation order as the
 // reference expressions (no FP contraction, see above).
 // ---------------------------------------------------------------------
@@ -1114,7 +1294,7 @@ DEV void pw_particle(const DevInst &P, const StepArgs &A, int i, const double4 p
 // semi-implicit Euler (kernel.cu:563-606); returns true when the particle's
 // path length passed the rebuild threshold
 DEV bool integrate(const DevInst &P, const StepArgs &A, int i, const double4 pi, const double4 va,
-                   const double4 vb, V3 F, V3 M, double Mr) {
+                   const double4 vb, V3 F, V3 M, double Mr, double tag = 0.0) {
     const double dt = P.dt;
     const double di = 2.0 * pi.w;
     const double mass = di * di * di;
@@ -1141,7 +1321,7 @@ DEV bool integrate(const DevInst &P, const StepArgs &A, int i, const double4 pi,
     A.out.p[i] = np;
     double4 o0, o1;
     o0.x = vx; o0.y = vy; o0.z = vz; o0.w = wx;
-    o1.x = wy; o1.y = wz; o1.z = pm; o1.w = 0.0;
+    o1.x = wy; o1.y = wz; o1.z = pm; o1.w = tag;  // tag: the pair kernel's contact mask
     A.out.v[2 * i] = o0;
     A.out.v[2 * i + 1] = o1;
     if (A.store_forces) A.Mr[i] = mr;
@@ -1497,8 +1677,62 @@ DEV WvEntry wave_entry(unsigned t, unsigned T, unsigned off, unsigned mlo, unsig
     return e;
 }
 
-// block blk of WV_BS particles; res is this wave's [7][64] LDS slice
-template <bool MIXED>
+// entry t of the wave's list in pair mode (shared pair records, see
+// PAIR_MIRROR): per particle its contact pairs in slot order, its walls in
+// slot order, then the owned contacts of the previous step that ended now,
+// whose records are copied forward once; every lane of the wave must call
+// this
+struct WpEntry {
+    int ii, k, kind;  // kind 0 pair, 1 wall, 2 copy forward
+    bool valid;
+    int nidx;         // pairs and copies: index of the slot's entry in nrec
+};
+
+DEV WpEntry wave_entry_pair(unsigned t, unsigned T, unsigned off, unsigned mlo, unsigned mhi,
+                            unsigned wmask, unsigned blo, unsigned bhi, unsigned nlo, unsigned nhi,
+                            int noff, int wbase) {
+    const unsigned tq = t < T ? t : T - 1;
+    int o = 0;
+#pragma unroll
+    for (int s = 32; s > 0; s >>= 1) {
+        const unsigned om = (unsigned)__shfl((int)off, o + s, 64);
+        if (om <= tq) o += s;
+    }
+    const unsigned olo = (unsigned)__shfl((int)mlo, o, 64);
+    const unsigned ohi = (unsigned)__shfl((int)mhi, o, 64);
+    const unsigned ow = (unsigned)__shfl((int)wmask, o, 64);
+    const unsigned ooff = (unsigned)__shfl((int)off, o, 64);
+    const unsigned oblo = (unsigned)__shfl((int)blo, o, 64);
+    const unsigned obhi = (unsigned)__shfl((int)bhi, o, 64);
+    const unsigned onlo = (unsigned)__shfl((int)nlo, o, 64);
+    const unsigned onhi = (unsigned)__shfl((int)nhi, o, 64);
+    const int onoff = __shfl(noff, o, 64);
+    const unsigned long long om = ((unsigned long long)ohi << 32) | olo;
+    const unsigned long long ob = ((unsigned long long)obhi << 32) | oblo;
+    const unsigned long long on = ((unsigned long long)onhi << 32) | onlo;
+    const unsigned r = tq - ooff, np = (unsigned)__popcll(om), nw = (unsigned)__popc(ow);
+    WpEntry e;
+    e.ii = wbase + o;
+    e.valid = t < T;
+    if (r < np) {
+        e.kind = 0;
+        e.k = select_bit(om, r);
+    } else if (r < np + nw) {
+        e.kind = 1;
+        e.k = select_bit(ow, r - np);
+    } else {
+        e.kind = 2;
+        e.k = select_bit(ob, r - np - nw);
+    }
+    e.nidx = onoff + (int)__popcll(on & ((1ull << e.k) - 1ull));
+    return e;
+}
+
+// block blk of WV_BS particles; res is this wave's [7][64] LDS slice.
+// PAIR: the history comes from the shared pair records (see PAIR_MIRROR)
+// instead of one record per directed slot; the forces are the same bit
+// for bit.
+template <bool MIXED, bool PAIR>
 DEV void step_wave_block(const DevInst &P, const StepArgs &A, unsigned blk, double (*res)[64],
                          WvStash *st) {
     const int tid = threadIdx.x, lane = tid & 63;
@@ -1509,21 +1743,30 @@ DEV void step_wave_block(const DevInst &P, const StepArgs &A, unsigned blk, doub
 
     // phase A: contact masks of the pair slots and the wall slots.  Only
     // the slots of nmask are tested (the others cannot touch before the
-    // next rebuild), WV_NB at a time.
-    unsigned long long mask = 0ull, am_own = 0ull;
+    // next rebuild), WV_NB at a time.  Pair mode also notes which contact
+    // slots own their record (ownm) and which owned contacts of the
+    // previous step (the mask in the spare word of the velocity record)
+    // ended (brk).
+    unsigned long long mask = 0ull, am_own = 0ull, ownm = 0ull, brk = 0ull, nm_own = 0ull;
     unsigned wmask = 0u;
     int aoff_own = 0;
     double grav = 0.0;
     if (own) {
-        am_own = A.amask[i];
-        aoff_own = A.actoff[i];
+        if constexpr (PAIR) {
+            aoff_own = A.noff[i];
+        } else {
+            am_own = A.amask[i];
+            aoff_own = A.actoff[i];
+        }
         unsigned long long nm = A.nmask[i];
         const double4 pi = A.in.p[i];
         const int n = A.nnb[i], nw = A.nwl[i];
+        const double4 vbi = A.in.v[2 * i + 1];
         st->p[lane] = pi;
         st->va[lane] = A.in.v[2 * i];
-        st->vb[lane] = A.in.v[2 * i + 1];
+        st->vb[lane] = vbi;
         if (n < 64) nm &= (1ull << n) - 1ull;
+        nm_own = nm;
         while (nm) {
             int ks[WV_NB], jj[WV_NB];
 #pragma unroll
@@ -1540,9 +1783,13 @@ DEV void step_wave_block(const DevInst &P, const StepArgs &A, unsigned blk, doub
             for (int u = 0; u < WV_NB; ++u) {
                 V3 nv;
                 double gap;
-                if (ks[u] >= 0 && pair_geometry<MIXED>(pi, q[u], nv, gap)) mask |= 1ull << ks[u];
+                if (ks[u] >= 0 && pair_geometry<MIXED>(pi, q[u], nv, gap)) {
+                    mask |= 1ull << ks[u];
+                    if (PAIR && (jj[u] > i || q[u].w != pi.w)) ownm |= 1ull << ks[u];
+                }
             }
         }
+        if constexpr (PAIR) brk = (unsigned long long)__double_as_longlong(vbi.w) & ~mask;
         const V3 ip = v3(pi.x, pi.y, pi.z);
         for (int k = 0; k < nw; ++k) {
             V3 nv;
@@ -1554,8 +1801,10 @@ DEV void step_wave_block(const DevInst &P, const StepArgs &A, unsigned blk, doub
         grav = di * di * di;
     }
     wave_sync_lds();  // the stash is read by other lanes in phase C
-    // phase B: number the wave's entries, pairs before walls per particle
-    const unsigned cnt = (unsigned)__popcll(mask) + (unsigned)__popc(wmask);
+    // phase B: number the wave's entries, pairs before walls (before
+    // copies) per particle
+    const unsigned nforce = (unsigned)__popcll(mask) + (unsigned)__popc(wmask);
+    const unsigned cnt = nforce + (PAIR ? (unsigned)__popcll(brk) : 0u);
     const unsigned incl = wave_incl_scan(cnt);
     const unsigned off = incl - cnt;
     const unsigned T = __shfl(incl, 63, 64);
@@ -1563,57 +1812,87 @@ DEV void step_wave_block(const DevInst &P, const StepArgs &A, unsigned blk, doub
     double Fx = 0.0, Fy = 0.0, Fz = 0.0, Mx = 0.0, My = 0.0, Mz = 0.0, Mr = 0.0;
     Fz = Fz - grav;  // gravity first (kernel.cu:439)
     const unsigned mlo = (unsigned)mask, mhi = (unsigned)(mask >> 32);
+    const unsigned blo = (unsigned)brk, bhi = (unsigned)(brk >> 32);
+    const unsigned nlo = (unsigned)nm_own, nhi = (unsigned)(nm_own >> 32);
 
     // phase C, software-pipelined: a round first issues the loads of its
     // neighbours' state, then those of the next round's neighbour indices
     // (or wall ids) and history records, then computes; the history of
     // round r+1 streams in from HBM while round r computes
     const long Kcap = (long)A.K * cap;
-    WvEntry e;
-    int j = 0;
+    typedef typename std::conditional<PAIR, WpEntry, WvEntry>::type Entry;
+    Entry e;
+    int j = 0, rn = 0;
     long hin = 0;
     HistRec hn;
-    if (T > 0) {
-        e = wave_entry(lane, T, off, mlo, mhi, wmask, wbase, am_own, aoff_own);
-        if (e.valid) {
-            j = (e.wall ? A.wl : A.nbr)[(long)e.k * cap + e.ii];
-            if (!e.wall) {
-                hin = hist_idx(e.am, e.aoff, e.k, e.ii, cap, Kcap);
+    auto entry_at = [&](unsigned t) -> Entry {
+        if constexpr (PAIR)
+            return wave_entry_pair(t, T, off, mlo, mhi, wmask, blo, bhi, nlo, nhi, aoff_own, wbase);
+        else
+            return wave_entry(t, T, off, mlo, mhi, wmask, wbase, am_own, aoff_own);
+    };
+    auto fetch = [&](const Entry &x) {
+        if constexpr (PAIR) {
+            if (x.kind == 1) {
+                j = A.wl[(long)x.k * cap + x.ii];
+            } else {
+                if (x.kind == 0) j = A.nbr[(long)x.k * cap + x.ii];
+                rn = A.nrec[x.nidx];
+                hn = A.hq_in[rn & 0x7fffffff];
+            }
+        } else {
+            j = (x.wall ? A.wl : A.nbr)[(long)x.k * cap + x.ii];
+            if (!x.wall) {
+                hin = hist_idx(x.am, x.aoff, x.k, x.ii, cap, Kcap);
                 hn = A.h[hin];
             }
         }
+    };
+    if (T > 0) {
+        e = entry_at(lane);
+        if (e.valid) fetch(e);
     }
     for (unsigned r0 = 0; r0 < T; r0 += 64) {
-        const WvEntry ec = e;
-        const int jc = j;
+        const Entry ec = e;
+        const int jc = j, rc = rn;
         const long hic = hin;
         HistRec hc = hn;
+        bool is_pair;
+        if constexpr (PAIR) is_pair = ec.kind == 0;
+        else is_pair = !ec.wall;
         double4 qj, aj, bj;
-        if (ec.valid && !ec.wall) {
+        if (ec.valid && is_pair) {
             qj = A.in.p[jc];
             aj = A.in.v[2 * jc];
             bj = A.in.v[2 * jc + 1];
         }
         if (r0 + 64 < T) {
-            e = wave_entry(r0 + 64 + lane, T, off, mlo, mhi, wmask, wbase, am_own, aoff_own);
-            if (e.valid) {
-                j = (e.wall ? A.wl : A.nbr)[(long)e.k * cap + e.ii];
-                if (!e.wall) {
-                    hin = hist_idx(e.am, e.aoff, e.k, e.ii, cap, Kcap);
-                    hn = A.h[hin];
-                }
-            }
+            e = entry_at(r0 + 64 + lane);
+            if (e.valid) fetch(e);
         } else {
             e.valid = false;
         }
-        if (ec.valid) {
+        bool copy_only = false;
+        if constexpr (PAIR) copy_only = ec.kind == 2;
+        if (ec.valid && copy_only) {
+            if constexpr (PAIR) A.hq_out[rc] = hc;  // an owned record: no mirror bit
+        } else if (ec.valid) {
             V3 dF = v3(0.0, 0.0, 0.0), dM = v3(0.0, 0.0, 0.0);
             double dMr = 0.0;
             const int o = ec.ii - wbase;
             const double4 qi = st->p[o], ai = st->va[o], bi = st->vb[o];
-            if (!ec.wall) {
-                pair_compute<MIXED>(P, hc, qi, ai, bi, qj, aj, bj, dF, dM, dMr);
-                A.h[hic] = hc;
+            if (is_pair) {
+                if constexpr (PAIR) {
+                    const bool mirror = rc < 0;
+                    if (mirror) {
+                        hc.dx = -hc.dx; hc.dy = -hc.dy; hc.dz = -hc.dz;
+                    }
+                    pair_compute<MIXED>(P, hc, qi, ai, bi, qj, aj, bj, dF, dM, dMr);
+                    if (!mirror) A.hq_out[rc] = hc;
+                } else {
+                    pair_compute<MIXED>(P, hc, qi, ai, bi, qj, aj, bj, dF, dM, dMr);
+                    A.h[hic] = hc;
+                }
             } else {
                 wall_entry<MIXED>(P, A, ec.ii, ec.k, jc, qi, ai, bi, dF, dM, dMr);
             }
@@ -1622,9 +1901,9 @@ DEV void step_wave_block(const DevInst &P, const StepArgs &A, unsigned blk, doub
             res[6][lane] = dMr;
         }
         wave_sync_lds();
-        if (cnt) {
+        if (nforce) {
             const unsigned a = off > r0 ? off : r0;
-            const unsigned b = off + cnt < r0 + 64 ? off + cnt : r0 + 64;
+            const unsigned b = off + nforce < r0 + 64 ? off + nforce : r0 + 64;
             for (unsigned q = a; q < b; ++q) {
                 const unsigned s = q - r0;
                 Fx = Fx + res[0][s]; Fy = Fy + res[1][s]; Fz = Fz + res[2][s];
@@ -1643,27 +1922,32 @@ DEV void step_wave_block(const DevInst &P, const StepArgs &A, unsigned blk, doub
             A.Fx[i] = F.x; A.Fy[i] = F.y; A.Fz[i] = F.z;
             A.Mx[i] = M.x; A.My[i] = M.y; A.Mz[i] = M.z;
         }
-        over = integrate(P, A, i, pi, va, vb, F, M, Mr);
+        const double tag = PAIR ? __longlong_as_double((long long)(mask & ownm)) : 0.0;
+        over = integrate(P, A, i, pi, va, vb, F, M, Mr, tag);
     }
     request_rebuild(A, over);
 }
 
-template <bool MIXED>
+template <bool MIXED, bool PAIR>
 __global__ void __launch_bounds__(WV_BS) k_step_wave(DevInst P, StepArgs A) {
     if (__hip_atomic_load(A.stop_at, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <= A.step) return;
     __shared__ double res_all[WV_BS / 64][7][64];
     __shared__ WvStash stash[WV_BS / 64];
-    step_wave_block<MIXED>(P, A, xcd_block(gridDim.x), res_all[threadIdx.x >> 6],
-                           &stash[threadIdx.x >> 6]);
+    step_wave_block<MIXED, PAIR>(P, A, xcd_block(gridDim.x), res_all[threadIdx.x >> 6],
+                                 &stash[threadIdx.x >> 6]);
 }
 
-void launch_step_wave(const DevInst &P, const StepArgs &A, bool mixed, hipStream_t s) {
+void launch_step_wave(const DevInst &P, const StepArgs &A, bool mixed, bool pair, hipStream_t s) {
     if (A.total <= 0) return;
     unsigned nb = blocks_for(A.total, WV_BS);
-    if (mixed)
-        hipLaunchKernelGGL((k_step_wave<true>), dim3(nb), dim3(WV_BS), 0, s, P, A);
+    if (mixed && pair)
+        hipLaunchKernelGGL((k_step_wave<true, true>), dim3(nb), dim3(WV_BS), 0, s, P, A);
+    else if (mixed)
+        hipLaunchKernelGGL((k_step_wave<true, false>), dim3(nb), dim3(WV_BS), 0, s, P, A);
+    else if (pair)
+        hipLaunchKernelGGL((k_step_wave<false, true>), dim3(nb), dim3(WV_BS), 0, s, P, A);
     else
-        hipLaunchKernelGGL((k_step_wave<false>), dim3(nb), dim3(WV_BS), 0, s, P, A);
+        hipLaunchKernelGGL((k_step_wave<false, false>), dim3(nb), dim3(WV_BS), 0, s, P, A);
 }
 
 // ---------------------------------------------------------------------
@@ -1697,7 +1981,7 @@ __global__ void __launch_bounds__(WV_BS) k_step_batch_wave(const DevInst *insts,
     if (!batch_step_args(tab[tk.x], step, A)) return;
     __shared__ double res_all[WV_BS / 64][7][64];
     __shared__ WvStash stash[WV_BS / 64];
-    step_wave_block<MIXED>(insts[tk.x], A, (unsigned)tk.y, res_all[threadIdx.x >> 6],
+    step_wave_block<MIXED, false>(insts[tk.x], A, (unsigned)tk.y, res_all[threadIdx.x >> 6],
                            &stash[threadIdx.x >> 6]);
 }
 

@@ -44,7 +44,22 @@ void launch_hist_slots(HistRec *h, const int *nnb, const unsigned long long *ama
 void launch_step_fused(const DevInst &P, const StepArgs &A, bool wet, bool mixed, hipStream_t s);
 // dry step with contact compaction; needs K <= 64
 void launch_step_compact(const DevInst &P, const StepArgs &A, int K, bool mixed, hipStream_t s);
-void launch_step_wave(const DevInst &P, const StepArgs &A, bool mixed, hipStream_t s);
+// pair: shared pair records (A.noff, A.nrec, A.hq_in, A.hq_out; see PAIR_MIRROR)
+void launch_step_wave(const DevInst &P, const StepArgs &A, bool mixed, bool pair, hipStream_t s);
+// shared pair records: counts (c: 3 arrays of cap), fill from the directed
+// records (o: exclusive scans of the counts), write back
+void launch_pair_count(const double4 *p, const int *nbr, const int *nnb,
+                       const unsigned long long *nmask, unsigned *c, int cap, int total,
+                       hipStream_t s);
+void launch_pair_fill(const double4 *p, double4 *vw, const int *nbr, const int *nnb,
+                      const unsigned long long *nmask, const unsigned *o, unsigned base2,
+                      const HistRec *hd, const unsigned long long *amask, const int *actoff,
+                      int *nrec, HistRec *q0, HistRec *q1, int *bad, int cap, long Kcap, int total,
+                      hipStream_t s);
+void launch_pair_unpack(const int *nnb, const unsigned long long *nmask, const unsigned *noff,
+                        const int *nrec, const HistRec *q, HistRec *hd,
+                        const unsigned long long *amask, const int *actoff, int cap, long Kcap,
+                        int total, hipStream_t s);
 void launch_analyze(const DevInst *insts, const AnaArgs *args, double *out, int n, hipStream_t s);
 void launch_rebuild_batch(const DevInst *insts, const RbInst *tab, int ninst, const int2 *ptasks,
                           int nptasks, const int2 *ctasks, int nctasks, hipStream_t s);

@@ -249,6 +249,15 @@ class Engine:
     def n_relayout(self):
         return A.lib().dem_n_relayout(self._ctx)
 
+    def set_pair_records(self, on):
+        """Shared pair records for the wave kernel (default on): one history
+        record per pair of equal radii instead of two mirrored ones."""
+        check(A.lib().dem_set_pair_records(self._ctx, int(bool(on))))
+
+    @property
+    def pair_records_used(self):
+        return bool(A.lib().dem_pair_records_used(self._ctx))
+
     @property
     def iter(self):
         return A.lib().dem_iter(self._ctx)

@@ -235,7 +235,7 @@ int dem_set_kernel(dem_ctx *ctx, int variant);
  * order of the cells, 0 = the reference's row-major cell hash.  Lists,
  * forces and the exported hash/cell tables are identical either way. */
 int dem_set_order(dem_ctx *ctx, int morton);
-/* History layout refresh in dem_step: every `every` steps (default 1024;
+/* History layout refresh in dem_step: every `every` steps (default 512;
  * 0 turns it off) without a rebuild, the contact records are regathered
  * into the compact per-wave region from the contacts of that moment.
  * Records only move, so results are identical with any setting.  No
@@ -243,6 +243,19 @@ int dem_set_order(dem_ctx *ctx, int morton);
 int dem_set_relayout(dem_ctx *ctx, int every);
 /* Layout refreshes done so far. */
 int dem_n_relayout(dem_ctx *ctx);
+/* Shared pair records for the WAVE kernel (on by default): a pair of
+ * equal radii keeps one history record instead of the reference's two
+ * directed ones (kernel.cu:413-552 evaluates (i, j) and (j, i) with
+ * records that stay exact mirrors: equal fields, tangential spring of
+ * opposite sign), updated by the lower storage index and read, spring
+ * negated, by the other.  Results are identical bit for bit; the history
+ * writes of a step halve.  The directed records are restored before
+ * anything else reads them (export, rebuild, other kernels).  A packing
+ * whose records are found not to be mirrors keeps the directed records.
+ * No reference counterpart. */
+int dem_set_pair_records(dem_ctx *ctx, int on);
+/* 1 when dem_step would use shared pair records for this context. */
+int dem_pair_records_used(dem_ctx *ctx);
 int dem_iter(dem_ctx *ctx);
 double dem_ctime(dem_ctx *ctx);
 int dem_n_rebuild(dem_ctx *ctx);
