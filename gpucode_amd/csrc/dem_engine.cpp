@@ -476,7 +476,9 @@ extern "C" int dem_get_cells(dem_ctx *c, int *cellStart, int *cellEnd) {
 void dem_pair_free(dem_ctx *c) {
     for (int b = 0; b < 2; ++b) {
         (void)hipFree(c->hq[b]);
+        (void)hipFree(c->hc[b]);
         c->hq[b] = nullptr;
+        c->hc[b] = nullptr;
     }
     (void)hipFree(c->nrec); (void)hipFree(c->pcnt); (void)hipFree(c->poff);
     (void)hipFree(c->pair_err);
@@ -524,17 +526,20 @@ int dem_pair_pack(dem_ctx *c) {
     const long t1 = (long)last[0] + last[1], t2 = (long)last[2] + last[3], tn = (long)last[4] + last[5];
     if (t1 + t2 > 0x7fffffffL || tn > 0x7fffffffL)
         return dem_fail(DEM_ERR_NOMEM, "too many near slots for 32-bit pair record indices");
-    long h0 = c->hq_cap, h1 = c->hq_cap;
+    long h[4] = {c->hq_cap, c->hq_cap, c->hq_cap, c->hq_cap};
     int rc;
-    if ((rc = grow(c, (void **)&c->hq[0], &h0, t1 + t2, sizeof(HistRec))) ||
-        (rc = grow(c, (void **)&c->hq[1], &h1, t1 + t2, sizeof(HistRec))) ||
+    if ((rc = grow(c, (void **)&c->hq[0], &h[0], t1 + t2, sizeof(PairHot))) ||
+        (rc = grow(c, (void **)&c->hq[1], &h[1], t1 + t2, sizeof(PairHot))) ||
+        (rc = grow(c, (void **)&c->hc[0], &h[2], t1 + t2, sizeof(double2))) ||
+        (rc = grow(c, (void **)&c->hc[1], &h[3], t1 + t2, sizeof(double2))) ||
         (rc = grow(c, (void **)&c->nrec, &c->nrec_cap, tn, sizeof(int))))
         return rc;
-    c->hq_cap = h0 < h1 ? h0 : h1;
+    c->hq_cap = h[0];
+    for (int q = 1; q < 4; ++q) c->hq_cap = h[q] < c->hq_cap ? h[q] : c->hq_cap;
     launch_fill_i32(c->pair_err, 0, 1, s);
     launch_pair_fill(c->P4[c->cur], c->VW[c->cur], c->nbr[c->hcur], c->nnb[c->hcur], c->nmask, c->poff,
                      (unsigned)t1, c->hp[c->hl], c->amask[c->hl], c->actoff[c->hl], c->nrec, c->hq[0],
-                     c->hq[1], c->pair_err, c->cap, Kcap, n, s);
+                     c->hq[1], c->hc[0], c->hc[1], c->pair_err, c->cap, Kcap, n, s);
     HIPCHK(hipGetLastError());
     int bad = 0;
     HIPCHK(hipMemcpyAsync(&bad, c->pair_err, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -553,8 +558,8 @@ int dem_pair_pack(dem_ctx *c) {
 int dem_pair_unpack(dem_ctx *c) {
     if (!c->pair_packed) return DEM_OK;
     c->pair_packed = false;
-    launch_pair_unpack(c->nnb[c->hcur], c->nmask, c->poff + 2L * c->cap, c->nrec, c->hq[c->hqc],
-                       c->hp[c->hl], c->amask[c->hl], c->actoff[c->hl], c->cap,
+    launch_pair_unpack(c->P4[c->cur], c->nbr[c->hcur], c->nnb[c->hcur], c->nmask,
+                       c->poff + 2L * c->cap, c->nrec, c->hq[c->hqc], c->hc[c->hqc], c->hp[c->hl], c->amask[c->hl], c->actoff[c->hl], c->cap,
                        (long)c->K * c->cap, c->total, c->stream);
     HIPCHK(hipGetLastError());
     return DEM_OK;
@@ -797,6 +802,8 @@ static int run_chunk(dem_ctx *c, int nsteps, int *executed) {
         if (pair) {
             A.hq_in = c->hq[(c->hqc + s) & 1];
             A.hq_out = c->hq[(c->hqc + s + 1) & 1];
+            A.hc_in = c->hc[(c->hqc + s) & 1];
+            A.hc_out = c->hc[(c->hqc + s + 1) & 1];
         }
         if (kern == DEM_KERNEL_WAVE)
             launch_step_wave(c->P, A, mixed, pair, c->stream);

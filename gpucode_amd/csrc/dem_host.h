@@ -57,7 +57,8 @@ struct dem_ctx {
     bool pair_bad = false;       // a mirror record did not match: directed records only
     bool pair_packed = false;    // the pair records hold the current history
     int hqc = 0;                 // the pair buffer the next step reads
-    HistRec *hq[2] = {};
+    PairHot *hq[2] = {};         // hot part of the pair records, two buffers
+    double2 *hc[2] = {};         // their plstRad/plstDfm where not implied
     long hq_cap = 0;             // records per pair buffer
     int *nrec = nullptr;         // near slot -> pair record (PAIR_MIRROR on the mirror side)
     long nrec_cap = 0;

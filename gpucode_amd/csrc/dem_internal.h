@@ -99,6 +99,20 @@ struct __attribute__((aligned(64))) HistRec {
 // record).
 #define PAIR_MIRROR 0x80000000u
 
+// A shared pair record as the wave kernel keeps it: the fields every
+// contact step reads and writes (spring, overlap, largest overlap, flags)
+// in 48 bytes, and plstRad/plstDfm in a separate 16-byte array that is
+// touched only when the flags do not imply them.  Implied: plstDfm = +0 and
+// plstRad = the pair's R* (PR_RSTAR) or 0 (neither flag), the values of
+// every contact that has not yielded and of every fresh record; otherwise
+// PR_COLD is set and the pair (plstRad, plstDfm) is stored.  Lossless.
+struct __attribute__((aligned(16))) PairHot {
+    double dx, dy, dz, nd, ndm;
+    int flags, pad;
+};
+#define PR_COLD 0x200
+#define PR_RSTAR 0x400
+
 typedef double4 dvec4;
 
 struct StateBuf {
@@ -120,8 +134,10 @@ struct StepArgs {
     // nrec[noff[i] + r] = its pair record index, bit 31 set on the mirror
     // side; records are read from hq_in and written to hq_out
     const int *noff, *nrec;
-    const HistRec *hq_in;
-    HistRec *hq_out;
+    const PairHot *hq_in;
+    PairHot *hq_out;
+    const double2 *hc_in;      // plstRad, plstDfm of records with PR_COLD
+    double2 *hc_out;
     const int *wl, *nwl;
     HistRec *hw;
     double *Fx, *Fy, *Fz, *Mx, *My, *Mz, *Mr;  // optional force output

@@ -700,6 +700,44 @@ DEV unsigned long long near_slots(const unsigned long long *nmask, const int *nn
 
 DEV bool owns_record(int i, int j, double ri, double rj) { return j > i || rj != ri; }
 
+DEV double pair_rstar(double ri, double rj) { return ri * rj / (ri + rj); }  // as pair_compute
+
+// the full record from its stored form (cold: its plstRad, plstDfm, read
+// only when PR_COLD is set; R: the pair's R*)
+DEV HistRec pair_expand(const PairHot &p, double2 cold, double R) {
+    HistRec h;
+    h.dx = p.dx; h.dy = p.dy; h.dz = p.dz; h.nd = p.nd; h.ndm = p.ndm;
+    if (p.flags & PR_COLD) {
+        h.prad = cold.x;
+        h.pdfm = cold.y;
+    } else {
+        h.prad = (p.flags & PR_RSTAR) ? R : 0.0;
+        h.pdfm = 0.0;
+    }
+    h.flags = p.flags & ~(PR_COLD | PR_RSTAR);
+    h.pad = 0;
+    return h;
+}
+
+// the stored form of h; need_cold: cold must be written as well
+DEV PairHot pair_compress(const HistRec &h, double R, double2 &cold, bool &need_cold) {
+    PairHot p;
+    p.dx = h.dx; p.dy = h.dy; p.dz = h.dz; p.nd = h.nd; p.ndm = h.ndm;
+    int f = h.flags & ~(PR_COLD | PR_RSTAR);
+    const long long bp = __double_as_longlong(h.prad), bd = __double_as_longlong(h.pdfm);
+    need_cold = false;
+    if (bd == 0 && bp == __double_as_longlong(R)) {
+        f |= PR_RSTAR;
+    } else if (!(bd == 0 && bp == 0)) {
+        f |= PR_COLD;
+        cold = make_double2(h.prad, h.pdfm);
+        need_cold = true;
+    }
+    p.flags = f;
+    p.pad = 0;
+    return p;
+}
+
 // per particle: owned near slots in contact now (c[0]), other owned near
 // slots (c[1]), near slots (c[2]); c[q] is c + q*cap
 __global__ void k_pair_count(const double4 *p, const int *nbr, const int *nnb,
@@ -726,15 +764,14 @@ __global__ void k_pair_count(const double4 *p, const int *nbr, const int *nnb,
     c[2L * cap + i] = n;
 }
 
-// owned slots: assign records, copy the directed records into both
-// buffers; mirror slots are marked -1 until k_pair_fill_mirror.  Clears
-// the contact mask in the spare word of the velocity records the next step
-// reads.
+// owned slots: assign records, store the directed records in both buffers;
+// mirror slots are marked -1 until k_pair_fill_mirror.  Clears the contact
+// mask in the spare word of the velocity records the next step reads.
 __global__ void k_pair_fill_owner(const double4 *p, double4 *vw, const int *nbr, const int *nnb,
                                   const unsigned long long *nmask, const unsigned *o, unsigned base2,
                                   const HistRec *hd, const unsigned long long *amask,
-                                  const int *actoff, int *nrec, HistRec *q0, HistRec *q1, int cap,
-                                  long Kcap, int total) {
+                                  const int *actoff, int *nrec, PairHot *q0, PairHot *q1,
+                                  double2 *c0, double2 *c1, int cap, long Kcap, int total) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= total) return;
     vw[2L * i + 1].w = 0.0;
@@ -753,9 +790,16 @@ __global__ void k_pair_fill_owner(const double4 *p, double4 *vw, const int *nbr,
             const double dx = pi.x - q.x, dy = pi.y - q.y, dz = pi.z - q.z;
             const double gap = sqrt(dx * dx + dy * dy + dz * dz) - pi.w - q.w;
             rec = (int)(gap < 0.0 ? r1++ : r2++);
-            const HistRec h = hd[hist_idx(am, ao, k, i, cap, Kcap)];
+            double2 cold;
+            bool need = false;
+            const PairHot h = pair_compress(hd[hist_idx(am, ao, k, i, cap, Kcap)],
+                                            pair_rstar(pi.w, q.w), cold, need);
             q0[rec] = h;
             q1[rec] = h;
+            if (need) {
+                c0[rec] = cold;
+                c1[rec] = cold;
+            }
         }
         nrec[slot++] = rec;
     }
@@ -768,8 +812,8 @@ __global__ void k_pair_fill_owner(const double4 *p, double4 *vw, const int *nbr,
 __global__ void k_pair_fill_mirror(const double4 *p, const int *nbr, const int *nnb,
                                    const unsigned long long *nmask, const unsigned *o,
                                    const HistRec *hd, const unsigned long long *amask,
-                                   const int *actoff, int *nrec, const HistRec *q0, int *bad,
-                                   int cap, long Kcap, int total) {
+                                   const int *actoff, int *nrec, const PairHot *q0,
+                                   const double2 *c0, int *bad, int cap, long Kcap, int total) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= total) return;
     const double4 pi = p[i];
@@ -783,7 +827,8 @@ __global__ void k_pair_fill_mirror(const double4 *p, const int *nbr, const int *
         nm &= nm - 1ull;
         const unsigned me = slot++;
         const int j = nbr[(long)k * cap + i];
-        if (owns_record(i, j, pi.w, p[j].w)) continue;
+        const double rj = p[j].w;
+        if (owns_record(i, j, pi.w, rj)) continue;
         const int nj = nnb[j];
         int kk = -1;
         for (int b = 0; b < nj && kk < 0; ++b)
@@ -800,7 +845,10 @@ __global__ void k_pair_fill_mirror(const double4 *p, const int *nbr, const int *
             nrec[me] = 0;
             continue;
         }
-        const HistRec ho = q0[rec], hm = hd[hist_idx(am, ao, k, i, cap, Kcap)];
+        const PairHot qo = q0[rec];
+        const double2 cold = (qo.flags & PR_COLD) ? c0[rec] : make_double2(0.0, 0.0);
+        const HistRec ho = pair_expand(qo, cold, pair_rstar(pi.w, rj));
+        const HistRec hm = hd[hist_idx(am, ao, k, i, cap, Kcap)];
         if (!(hm.dx == -ho.dx && hm.dy == -ho.dy && hm.dz == -ho.dz && hm.nd == ho.nd &&
               hm.ndm == ho.ndm && hm.prad == ho.prad && hm.pdfm == ho.pdfm && hm.flags == ho.flags))
             *bad = 1;
@@ -808,14 +856,16 @@ __global__ void k_pair_fill_mirror(const double4 *p, const int *nbr, const int *
     }
 }
 
-// the pair records (buffer q, the one the next step reads) back into the
-// directed records of the near slots
-__global__ void k_pair_unpack(const int *nnb, const unsigned long long *nmask, const unsigned *noff,
-                              const int *nrec, const HistRec *q, HistRec *hd,
+// the pair records (buffers q, c: the ones the next step reads) back into
+// the directed records of the near slots
+__global__ void k_pair_unpack(const double4 *p, const int *nbr, const int *nnb,
+                              const unsigned long long *nmask, const unsigned *noff,
+                              const int *nrec, const PairHot *q, const double2 *c, HistRec *hd,
                               const unsigned long long *amask, const int *actoff, int cap,
                               long Kcap, int total) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= total) return;
+    const double ri = p[i].w;
     unsigned long long nm = near_slots(nmask, nnb, i);
     const unsigned long long am = amask[i];
     const int ao = actoff[i];
@@ -824,7 +874,10 @@ __global__ void k_pair_unpack(const int *nnb, const unsigned long long *nmask, c
         const int k = __ffsll((long long)nm) - 1;
         nm &= nm - 1ull;
         const int r = nrec[slot++];
-        HistRec h = q[r & 0x7fffffff];
+        const int rec = r & 0x7fffffff;
+        const PairHot ph = q[rec];
+        const double2 cold = (ph.flags & PR_COLD) ? c[rec] : make_double2(0.0, 0.0);
+        HistRec h = pair_expand(ph, cold, pair_rstar(ri, p[nbr[(long)k * cap + i]].w));
         if (r < 0) {
             h.dx = -h.dx; h.dy = -h.dy; h.dz = -h.dz;
         }
@@ -843,22 +896,24 @@ void launch_pair_count(const double4 *p, const int *nbr, const int *nnb,
 void launch_pair_fill(const double4 *p, double4 *vw, const int *nbr, const int *nnb,
                       const unsigned long long *nmask, const unsigned *o, unsigned base2,
                       const HistRec *hd, const unsigned long long *amask, const int *actoff,
-                      int *nrec, HistRec *q0, HistRec *q1, int *bad, int cap, long Kcap, int total,
-                      hipStream_t s) {
+                      int *nrec, PairHot *q0, PairHot *q1, double2 *c0, double2 *c1, int *bad,
+                      int cap, long Kcap, int total, hipStream_t s) {
     if (total <= 0) return;
     hipLaunchKernelGGL(k_pair_fill_owner, dim3(blocks_for(total, 256)), dim3(256), 0, s, p, vw, nbr,
-                       nnb, nmask, o, base2, hd, amask, actoff, nrec, q0, q1, cap, Kcap, total);
+                       nnb, nmask, o, base2, hd, amask, actoff, nrec, q0, q1, c0, c1, cap, Kcap,
+                       total);
     hipLaunchKernelGGL(k_pair_fill_mirror, dim3(blocks_for(total, 256)), dim3(256), 0, s, p, nbr, nnb,
-                       nmask, o, hd, amask, actoff, nrec, q0, bad, cap, Kcap, total);
+                       nmask, o, hd, amask, actoff, nrec, q0, c0, bad, cap, Kcap, total);
 }
 
-void launch_pair_unpack(const int *nnb, const unsigned long long *nmask, const unsigned *noff,
-                        const int *nrec, const HistRec *q, HistRec *hd,
+void launch_pair_unpack(const double4 *p, const int *nbr, const int *nnb,
+                        const unsigned long long *nmask, const unsigned *noff, const int *nrec,
+                        const PairHot *q, const double2 *c, HistRec *hd,
                         const unsigned long long *amask, const int *actoff, int cap, long Kcap,
                         int total, hipStream_t s) {
     if (total <= 0) return;
-    hipLaunchKernelGGL(k_pair_unpack, dim3(blocks_for(total, 256)), dim3(256), 0, s, nnb, nmask, noff,
-                       nrec, q, hd, amask, actoff, cap, Kcap, total);
+    hipLaunchKernelGGL(k_pair_unpack, dim3(blocks_for(total, 256)), dim3(256), 0, s, p, nbr, nnb, nmask,
+                       noff, nrec, q, c, hd, amask, actoff, cap, Kcap, total);
 }
 
 // ---------------------------------------------------------------------
@@ -1821,10 +1876,11 @@ DEV void step_wave_block(const DevInst &P, const StepArgs &A, unsigned blk, doub
     // round r+1 streams in from HBM while round r computes
     const long Kcap = (long)A.K * cap;
     typedef typename std::conditional<PAIR, WpEntry, WvEntry>::type Entry;
+    typedef typename std::conditional<PAIR, PairHot, HistRec>::type Rec;
     Entry e;
     int j = 0, rn = 0;
     long hin = 0;
-    HistRec hn;
+    Rec hn;
     auto entry_at = [&](unsigned t) -> Entry {
         if constexpr (PAIR)
             return wave_entry_pair(t, T, off, mlo, mhi, wmask, blo, bhi, nlo, nhi, aoff_own, wbase);
@@ -1856,15 +1912,19 @@ DEV void step_wave_block(const DevInst &P, const StepArgs &A, unsigned blk, doub
         const Entry ec = e;
         const int jc = j, rc = rn;
         const long hic = hin;
-        HistRec hc = hn;
+        const Rec hr = hn;
         bool is_pair;
         if constexpr (PAIR) is_pair = ec.kind == 0;
         else is_pair = !ec.wall;
         double4 qj, aj, bj;
+        double2 cold = make_double2(0.0, 0.0);
         if (ec.valid && is_pair) {
             qj = A.in.p[jc];
             aj = A.in.v[2 * jc];
             bj = A.in.v[2 * jc + 1];
+            if constexpr (PAIR) {
+                if (hr.flags & PR_COLD) cold = A.hc_in[rc & 0x7fffffff];
+            }
         }
         if (r0 + 64 < T) {
             e = entry_at(r0 + 64 + lane);
@@ -1875,7 +1935,11 @@ DEV void step_wave_block(const DevInst &P, const StepArgs &A, unsigned blk, doub
         bool copy_only = false;
         if constexpr (PAIR) copy_only = ec.kind == 2;
         if (ec.valid && copy_only) {
-            if constexpr (PAIR) A.hq_out[rc] = hc;  // an owned record: no mirror bit
+            if constexpr (PAIR) {
+                // an owned record: no mirror bit
+                A.hq_out[rc] = hr;
+                if (hr.flags & PR_COLD) A.hc_out[rc] = A.hc_in[rc];
+            }
         } else if (ec.valid) {
             V3 dF = v3(0.0, 0.0, 0.0), dM = v3(0.0, 0.0, 0.0);
             double dMr = 0.0;
@@ -1884,12 +1948,19 @@ DEV void step_wave_block(const DevInst &P, const StepArgs &A, unsigned blk, doub
             if (is_pair) {
                 if constexpr (PAIR) {
                     const bool mirror = rc < 0;
+                    const double R = pair_rstar(qi.w, qj.w);
+                    HistRec hc = pair_expand(hr, cold, R);
                     if (mirror) {
                         hc.dx = -hc.dx; hc.dy = -hc.dy; hc.dz = -hc.dz;
                     }
                     pair_compute<MIXED>(P, hc, qi, ai, bi, qj, aj, bj, dF, dM, dMr);
-                    if (!mirror) A.hq_out[rc] = hc;
+                    if (!mirror) {
+                        bool need = false;
+                        A.hq_out[rc] = pair_compress(hc, R, cold, need);
+                        if (need) A.hc_out[rc] = cold;
+                    }
                 } else {
+                    HistRec hc = hr;
                     pair_compute<MIXED>(P, hc, qi, ai, bi, qj, aj, bj, dF, dM, dMr);
                     A.h[hic] = hc;
                 }

@@ -77,6 +77,34 @@ def test_pair_records_settling_bed_matches_oracle():
         assert np.array_equal(ha[f], ho[f]), f
 
 
+@pytest.mark.parametrize("precision", [G.PREC_FP64, G.PREC_MIXED])
+def test_plastic_contacts_store_their_plastic_fields(precision):
+    # yield at 0.05 % of d: most contacts deform plastically, so many pair
+    # records keep plstRad/plstDfm explicitly, and unloading brings others
+    # back to the implied values; the run still equals the directed one
+    # and, in FP64, the oracle
+    d = 1.0e-3
+    n, R_d = 3000, 8.0
+    H = d * (n / (np.pi * R_d * R_d * 0.5) + 6)
+    inp = W.make_input(n, d, R_d * d, H, 0.5, cutGap=0.1, yield_frac=5e-4)
+    w = dict(inp=inp, dia=None, pos=W.pocket_bed(n, R_d, 1.0, jitter=0.03, seed=7))
+    vel = np.random.default_rng(6).normal(size=(n, 3)) * 20.0
+    a, b = engines(w, precision, vel)
+    a.step(400)
+    b.step(400)
+    assert a.pair_records_used
+    ha = a.hist_pp()
+    plastic = np.count_nonzero(ha["plstDfm"])
+    assert plastic > 100, plastic
+    assert_same(a, b)
+    if precision == G.PREC_FP64:
+        o = O.Oracle(inp)
+        o.set_particles(w["pos"], vel)
+        o.step(400)
+        for u, v in zip(a.particles(), o.particles()):
+            assert np.array_equal(u, v)
+
+
 def test_export_and_kernel_switch_in_the_middle_of_a_run():
     # every export writes the pair records back and the next chunk makes
     # them again; a switch to the loop kernel and back does the same
