@@ -65,7 +65,7 @@ struct dem_ctx {
     unsigned *pcnt = nullptr, *poff = nullptr;  // 3 x cap counts and their scans
     int *pair_err = nullptr;
     int n_pack = 0;
-    int relayout_every = 512;    // steps between layout refreshes (0: never)
+    int relayout_every = 1024;   // steps between layout refreshes (0: never)
     long long layout_age = 0;    // steps since the layout was made
     int n_relayout = 0;
     bool has_lists = false, saved = false;

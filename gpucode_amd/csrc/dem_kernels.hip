@@ -1870,10 +1870,11 @@ DEV void step_wave_block(const DevInst &P, const StepArgs &A, unsigned blk, doub
     const unsigned blo = (unsigned)brk, bhi = (unsigned)(brk >> 32);
     const unsigned nlo = (unsigned)nm_own, nhi = (unsigned)(nm_own >> 32);
 
-    // phase C, software-pipelined: a round first issues the loads of its
-    // neighbours' state, then those of the next round's neighbour indices
-    // (or wall ids) and history records, then computes; the history of
-    // round r+1 streams in from HBM while round r computes
+    // phase C, software-pipelined.  Directed records: a round first issues
+    // the loads of its neighbours' state, then those of the next round's
+    // neighbour indices (or wall ids) and history records, then computes;
+    // the history of round r+1 streams in from HBM while round r computes.
+    // Pair records: one round deeper (below).
     const long Kcap = (long)A.K * cap;
     typedef typename std::conditional<PAIR, WpEntry, WvEntry>::type Entry;
     typedef typename std::conditional<PAIR, PairHot, HistRec>::type Rec;
@@ -1904,9 +1905,47 @@ DEV void step_wave_block(const DevInst &P, const StepArgs &A, unsigned blk, doub
             }
         }
     };
-    if (T > 0) {
-        e = entry_at(lane);
-        if (e.valid) fetch(e);
+    // pair mode: a two-deep pipeline; while round r computes, the records
+    // and neighbour state of round r+1 and the indices of round r+2 load
+    Entry e2;
+    int j2 = 0, rn2 = 0;
+    double4 qn, an, bn;
+    auto fetch_idx = [&](const Entry &x, int &jj, int &rr) {
+        if constexpr (PAIR) {
+            if (x.kind == 1) {
+                jj = A.wl[(long)x.k * cap + x.ii];
+            } else {
+                if (x.kind == 0) jj = A.nbr[(long)x.k * cap + x.ii];
+                rr = A.nrec[x.nidx];
+            }
+        }
+    };
+    auto fetch_state = [&](const Entry &x) {
+        if constexpr (PAIR) {
+            if (x.kind != 1) hn = A.hq_in[rn & 0x7fffffff];
+            if (x.kind == 0) {
+                qn = A.in.p[j];
+                an = A.in.v[2 * j];
+                bn = A.in.v[2 * j + 1];
+            }
+        }
+    };
+    if constexpr (PAIR) {
+        e2.valid = false;
+        if (T > 0) {
+            e = entry_at(lane);
+            if (e.valid) fetch_idx(e, j, rn);
+            if (64 < T) {
+                e2 = entry_at(64 + lane);
+                if (e2.valid) fetch_idx(e2, j2, rn2);
+            }
+            if (e.valid) fetch_state(e);
+        }
+    } else {
+        if (T > 0) {
+            e = entry_at(lane);
+            if (e.valid) fetch(e);
+        }
     }
     for (unsigned r0 = 0; r0 < T; r0 += 64) {
         const Entry ec = e;
@@ -1918,19 +1957,28 @@ DEV void step_wave_block(const DevInst &P, const StepArgs &A, unsigned blk, doub
         else is_pair = !ec.wall;
         double4 qj, aj, bj;
         double2 cold = make_double2(0.0, 0.0);
-        if (ec.valid && is_pair) {
-            qj = A.in.p[jc];
-            aj = A.in.v[2 * jc];
-            bj = A.in.v[2 * jc + 1];
-            if constexpr (PAIR) {
-                if (hr.flags & PR_COLD) cold = A.hc_in[rc & 0x7fffffff];
+        if constexpr (PAIR) {
+            qj = qn; aj = an; bj = bn;
+            e = e2; j = j2; rn = rn2;
+            if (e.valid) fetch_state(e);
+            e2.valid = false;
+            if (r0 + 128 < T) {
+                e2 = entry_at(r0 + 128 + lane);
+                if (e2.valid) fetch_idx(e2, j2, rn2);
             }
-        }
-        if (r0 + 64 < T) {
-            e = entry_at(r0 + 64 + lane);
-            if (e.valid) fetch(e);
+            if (ec.valid && is_pair && (hr.flags & PR_COLD)) cold = A.hc_in[rc & 0x7fffffff];
         } else {
-            e.valid = false;
+            if (ec.valid && is_pair) {
+                qj = A.in.p[jc];
+                aj = A.in.v[2 * jc];
+                bj = A.in.v[2 * jc + 1];
+            }
+            if (r0 + 64 < T) {
+                e = entry_at(r0 + 64 + lane);
+                if (e.valid) fetch(e);
+            } else {
+                e.valid = false;
+            }
         }
         bool copy_only = false;
         if constexpr (PAIR) copy_only = ec.kind == 2;
