@@ -480,10 +480,10 @@ void dem_pair_free(dem_ctx *c) {
         c->hq[b] = nullptr;
         c->hc[b] = nullptr;
     }
-    (void)hipFree(c->nrec); (void)hipFree(c->pcnt); (void)hipFree(c->poff);
+    (void)hipFree(c->nl); (void)hipFree(c->pcnt); (void)hipFree(c->poff);
     (void)hipFree(c->pair_err);
-    c->nrec = nullptr; c->pcnt = nullptr; c->poff = nullptr; c->pair_err = nullptr;
-    c->hq_cap = c->nrec_cap = 0;
+    c->nl = nullptr; c->pcnt = nullptr; c->poff = nullptr; c->pair_err = nullptr;
+    c->hq_cap = c->nl_cap = 0;
     c->pair_packed = false;
 }
 
@@ -509,7 +509,7 @@ int dem_pair_pack(dem_ctx *c) {
     hipStream_t s = c->stream;
     if (!c->pcnt) {
         HIPCHK(hipMalloc((void **)&c->pcnt, 3 * (size_t)cap * sizeof(unsigned)));
-        HIPCHK(hipMalloc((void **)&c->poff, 3 * (size_t)cap * sizeof(unsigned)));
+        HIPCHK(hipMalloc((void **)&c->poff, (3 * (size_t)cap + 1) * sizeof(unsigned)));
         HIPCHK(hipMalloc((void **)&c->pair_err, sizeof(int)));
     }
     launch_pair_count(c->P4[c->cur], c->nbr[c->hcur], c->nnb[c->hcur], c->nmask, c->pcnt, c->cap, n, s);
@@ -532,13 +532,16 @@ int dem_pair_pack(dem_ctx *c) {
         (rc = grow(c, (void **)&c->hq[1], &h[1], t1 + t2, sizeof(PairHot))) ||
         (rc = grow(c, (void **)&c->hc[0], &h[2], t1 + t2, sizeof(double2))) ||
         (rc = grow(c, (void **)&c->hc[1], &h[3], t1 + t2, sizeof(double2))) ||
-        (rc = grow(c, (void **)&c->nrec, &c->nrec_cap, tn, sizeof(int))))
+        (rc = grow(c, (void **)&c->nl, &c->nl_cap, tn, sizeof(int2))))
         return rc;
+    // the near lists end at noff[n]
+    const unsigned tnu = (unsigned)tn;
+    HIPCHK(hipMemcpyAsync(c->poff + 2 * cap + n, &tnu, sizeof(unsigned), hipMemcpyHostToDevice, s));
     c->hq_cap = h[0];
     for (int q = 1; q < 4; ++q) c->hq_cap = h[q] < c->hq_cap ? h[q] : c->hq_cap;
     launch_fill_i32(c->pair_err, 0, 1, s);
     launch_pair_fill(c->P4[c->cur], c->VW[c->cur], c->nbr[c->hcur], c->nnb[c->hcur], c->nmask, c->poff,
-                     (unsigned)t1, c->hp[c->hl], c->amask[c->hl], c->actoff[c->hl], c->nrec, c->hq[0],
+                     (unsigned)t1, c->hp[c->hl], c->amask[c->hl], c->actoff[c->hl], c->nl, c->hq[0],
                      c->hq[1], c->hc[0], c->hc[1], c->pair_err, c->cap, Kcap, n, s);
     HIPCHK(hipGetLastError());
     int bad = 0;
@@ -558,9 +561,9 @@ int dem_pair_pack(dem_ctx *c) {
 int dem_pair_unpack(dem_ctx *c) {
     if (!c->pair_packed) return DEM_OK;
     c->pair_packed = false;
-    launch_pair_unpack(c->P4[c->cur], c->nbr[c->hcur], c->nnb[c->hcur], c->nmask,
-                       c->poff + 2L * c->cap, c->nrec, c->hq[c->hqc], c->hc[c->hqc], c->hp[c->hl], c->amask[c->hl], c->actoff[c->hl], c->cap,
-                       (long)c->K * c->cap, c->total, c->stream);
+    launch_pair_unpack(c->P4[c->cur], c->nnb[c->hcur], c->nmask, c->poff + 2L * c->cap, c->nl,
+                       c->hq[c->hqc], c->hc[c->hqc], c->hp[c->hl], c->amask[c->hl],
+                       c->actoff[c->hl], c->cap, (long)c->K * c->cap, c->total, c->stream);
     HIPCHK(hipGetLastError());
     return DEM_OK;
 }
@@ -788,7 +791,7 @@ static int run_chunk(dem_ctx *c, int nsteps, int *executed) {
     StepArgs A = dem_step_args(c, true);
     if (pair) {
         A.noff = (const int *)(c->poff + 2L * c->cap);
-        A.nrec = c->nrec;
+        A.nl = c->nl;
     }
     double tw = c->Tw;
     HIPCHK(hipEventRecord(c->ev0, c->stream));

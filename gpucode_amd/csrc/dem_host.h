@@ -60,9 +60,9 @@ struct dem_ctx {
     PairHot *hq[2] = {};         // hot part of the pair records, two buffers
     double2 *hc[2] = {};         // their plstRad/plstDfm where not implied
     long hq_cap = 0;             // records per pair buffer
-    int *nrec = nullptr;         // near slot -> pair record (PAIR_MIRROR on the mirror side)
-    long nrec_cap = 0;
-    unsigned *pcnt = nullptr, *poff = nullptr;  // 3 x cap counts and their scans
+    int2 *nl = nullptr;          // near lists: (partner, pair record | PAIR_MIRROR)
+    long nl_cap = 0;
+    unsigned *pcnt = nullptr, *poff = nullptr;  // 3 x cap counts and their scans (+1: near total)
     int *pair_err = nullptr;
     int n_pack = 0;
     int relayout_every = 1024;   // steps between layout refreshes (0: never)

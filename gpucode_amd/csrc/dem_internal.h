@@ -130,10 +130,12 @@ struct StepArgs {
     const int *actoff;         // per particle: start of its compact records
     const unsigned long long *nmask;  // per particle: slots that can touch before the next rebuild
     // shared pair records (the wave kernel's pair mode, see PAIR_MIRROR):
-    // near slot r of particle i (r-th set bit of nmask[i]) has
-    // nrec[noff[i] + r] = its pair record index, bit 31 set on the mirror
-    // side; records are read from hq_in and written to hq_out
-    const int *noff, *nrec;
+    // the near list of particle i, nl[noff[i] .. noff[i+1]), holds its near
+    // slots (the set bits of nmask[i]) in slot order as (partner, pair
+    // record index, bit 31 set on the mirror side); records are read from
+    // hq_in and written to hq_out
+    const int *noff;
+    const int2 *nl;
     const PairHot *hq_in;
     PairHot *hq_out;
     const double2 *hc_in;      // plstRad, plstDfm of records with PR_COLD

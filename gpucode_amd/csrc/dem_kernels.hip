@@ -764,13 +764,14 @@ __global__ void k_pair_count(const double4 *p, const int *nbr, const int *nnb,
     c[2L * cap + i] = n;
 }
 
-// owned slots: assign records, store the directed records in both buffers;
-// mirror slots are marked -1 until k_pair_fill_mirror.  Clears the contact
+// owned slots: assign records, store the directed records in both buffers,
+// and write the near list (partner, record) of particle i in slot order;
+// mirror slots get record -1 until k_pair_fill_mirror.  Clears the contact
 // mask in the spare word of the velocity records the next step reads.
 __global__ void k_pair_fill_owner(const double4 *p, double4 *vw, const int *nbr, const int *nnb,
                                   const unsigned long long *nmask, const unsigned *o, unsigned base2,
                                   const HistRec *hd, const unsigned long long *amask,
-                                  const int *actoff, int *nrec, PairHot *q0, PairHot *q1,
+                                  const int *actoff, int2 *nl, PairHot *q0, PairHot *q1,
                                   double2 *c0, double2 *c1, int cap, long Kcap, int total) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= total) return;
@@ -801,18 +802,18 @@ __global__ void k_pair_fill_owner(const double4 *p, double4 *vw, const int *nbr,
                 c1[rec] = cold;
             }
         }
-        nrec[slot++] = rec;
+        nl[slot++] = make_int2(j, rec);
     }
 }
 
-// mirror slots: the record of the partner's slot for i.  Sets *bad when
-// the partner's list has no near slot for i or when the directed record
-// of this slot is not the exact mirror of the partner's (then the pair
-// records cannot stand for the directed ones).
-__global__ void k_pair_fill_mirror(const double4 *p, const int *nbr, const int *nnb,
+// mirror slots: the record of the partner's entry for i in its near list.
+// Sets *bad when the partner's near list has no entry for i or when the
+// directed record of this slot is not the exact mirror of the partner's
+// (then the pair records cannot stand for the directed ones).
+__global__ void k_pair_fill_mirror(const double4 *p, const int *nnb,
                                    const unsigned long long *nmask, const unsigned *o,
                                    const HistRec *hd, const unsigned long long *amask,
-                                   const int *actoff, int *nrec, const PairHot *q0,
+                                   const int *actoff, int2 *nl, const PairHot *q0,
                                    const double2 *c0, int *bad, int cap, long Kcap, int total) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= total) return;
@@ -826,23 +827,17 @@ __global__ void k_pair_fill_mirror(const double4 *p, const int *nbr, const int *
         const int k = __ffsll((long long)nm) - 1;
         nm &= nm - 1ull;
         const unsigned me = slot++;
-        const int j = nbr[(long)k * cap + i];
+        const int j = nl[me].x;
         const double rj = p[j].w;
         if (owns_record(i, j, pi.w, rj)) continue;
-        const int nj = nnb[j];
-        int kk = -1;
-        for (int b = 0; b < nj && kk < 0; ++b)
-            if (nbr[(long)b * cap + j] == i) kk = b;
-        const unsigned long long nmj = near_slots(nmask, nnb, j);
-        if (kk < 0 || kk >= 64 || !((nmj >> kk) & 1ull)) {
-            *bad = 1;
-            nrec[me] = 0;
-            continue;
+        int rec = -1;
+        for (unsigned b = noff[j], e = noff[j + 1]; b < e && rec < 0; ++b) {
+            const int2 v = nl[b];
+            if (v.x == i) rec = v.y >= 0 ? v.y : -2;
         }
-        const int rec = nrec[noff[j] + (unsigned)__popcll(nmj & ((1ull << kk) - 1ull))];
         if (rec < 0) {
             *bad = 1;
-            nrec[me] = 0;
+            nl[me].y = 0;
             continue;
         }
         const PairHot qo = q0[rec];
@@ -852,17 +847,16 @@ __global__ void k_pair_fill_mirror(const double4 *p, const int *nbr, const int *
         if (!(hm.dx == -ho.dx && hm.dy == -ho.dy && hm.dz == -ho.dz && hm.nd == ho.nd &&
               hm.ndm == ho.ndm && hm.prad == ho.prad && hm.pdfm == ho.pdfm && hm.flags == ho.flags))
             *bad = 1;
-        nrec[me] = (int)((unsigned)rec | PAIR_MIRROR);
+        nl[me].y = (int)((unsigned)rec | PAIR_MIRROR);
     }
 }
 
 // the pair records (buffers q, c: the ones the next step reads) back into
 // the directed records of the near slots
-__global__ void k_pair_unpack(const double4 *p, const int *nbr, const int *nnb,
-                              const unsigned long long *nmask, const unsigned *noff,
-                              const int *nrec, const PairHot *q, const double2 *c, HistRec *hd,
-                              const unsigned long long *amask, const int *actoff, int cap,
-                              long Kcap, int total) {
+__global__ void k_pair_unpack(const double4 *p, const int *nnb, const unsigned long long *nmask,
+                              const unsigned *noff, const int2 *nl, const PairHot *q,
+                              const double2 *c, HistRec *hd, const unsigned long long *amask,
+                              const int *actoff, int cap, long Kcap, int total) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= total) return;
     const double ri = p[i].w;
@@ -873,12 +867,12 @@ __global__ void k_pair_unpack(const double4 *p, const int *nbr, const int *nnb,
     while (nm) {
         const int k = __ffsll((long long)nm) - 1;
         nm &= nm - 1ull;
-        const int r = nrec[slot++];
-        const int rec = r & 0x7fffffff;
+        const int2 v = nl[slot++];
+        const int rec = v.y & 0x7fffffff;
         const PairHot ph = q[rec];
         const double2 cold = (ph.flags & PR_COLD) ? c[rec] : make_double2(0.0, 0.0);
-        HistRec h = pair_expand(ph, cold, pair_rstar(ri, p[nbr[(long)k * cap + i]].w));
-        if (r < 0) {
+        HistRec h = pair_expand(ph, cold, pair_rstar(ri, p[v.x].w));
+        if (v.y < 0) {
             h.dx = -h.dx; h.dy = -h.dy; h.dz = -h.dz;
         }
         hd[hist_idx(am, ao, k, i, cap, Kcap)] = h;
@@ -896,24 +890,23 @@ void launch_pair_count(const double4 *p, const int *nbr, const int *nnb,
 void launch_pair_fill(const double4 *p, double4 *vw, const int *nbr, const int *nnb,
                       const unsigned long long *nmask, const unsigned *o, unsigned base2,
                       const HistRec *hd, const unsigned long long *amask, const int *actoff,
-                      int *nrec, PairHot *q0, PairHot *q1, double2 *c0, double2 *c1, int *bad,
+                      int2 *nl, PairHot *q0, PairHot *q1, double2 *c0, double2 *c1, int *bad,
                       int cap, long Kcap, int total, hipStream_t s) {
     if (total <= 0) return;
     hipLaunchKernelGGL(k_pair_fill_owner, dim3(blocks_for(total, 256)), dim3(256), 0, s, p, vw, nbr,
-                       nnb, nmask, o, base2, hd, amask, actoff, nrec, q0, q1, c0, c1, cap, Kcap,
+                       nnb, nmask, o, base2, hd, amask, actoff, nl, q0, q1, c0, c1, cap, Kcap,
                        total);
-    hipLaunchKernelGGL(k_pair_fill_mirror, dim3(blocks_for(total, 256)), dim3(256), 0, s, p, nbr, nnb,
-                       nmask, o, hd, amask, actoff, nrec, q0, c0, bad, cap, Kcap, total);
+    hipLaunchKernelGGL(k_pair_fill_mirror, dim3(blocks_for(total, 256)), dim3(256), 0, s, p, nnb,
+                       nmask, o, hd, amask, actoff, nl, q0, c0, bad, cap, Kcap, total);
 }
 
-void launch_pair_unpack(const double4 *p, const int *nbr, const int *nnb,
-                        const unsigned long long *nmask, const unsigned *noff, const int *nrec,
-                        const PairHot *q, const double2 *c, HistRec *hd,
-                        const unsigned long long *amask, const int *actoff, int cap, long Kcap,
-                        int total, hipStream_t s) {
+void launch_pair_unpack(const double4 *p, const int *nnb, const unsigned long long *nmask,
+                        const unsigned *noff, const int2 *nl, const PairHot *q, const double2 *c,
+                        HistRec *hd, const unsigned long long *amask, const int *actoff, int cap,
+                        long Kcap, int total, hipStream_t s) {
     if (total <= 0) return;
-    hipLaunchKernelGGL(k_pair_unpack, dim3(blocks_for(total, 256)), dim3(256), 0, s, p, nbr, nnb, nmask,
-                       noff, nrec, q, c, hd, amask, actoff, cap, Kcap, total);
+    hipLaunchKernelGGL(k_pair_unpack, dim3(blocks_for(total, 256)), dim3(256), 0, s, p, nnb, nmask,
+                       noff, nl, q, c, hd, amask, actoff, cap, Kcap, total);
 }
 
 // ---------------------------------------------------------------------
@@ -1733,19 +1726,18 @@ DEV WvEntry wave_entry(unsigned t, unsigned T, unsigned off, unsigned mlo, unsig
 }
 
 // entry t of the wave's list in pair mode (shared pair records, see
-// PAIR_MIRROR): per particle its contact pairs in slot order, its walls in
-// slot order, then the owned contacts of the previous step that ended now,
-// whose records are copied forward once; every lane of the wave must call
-// this
+// PAIR_MIRROR): per particle its contact pairs in near-list (= slot)
+// order, its walls in slot order, then the owned contacts of the previous
+// step that ended now, whose records are copied forward once; pair masks
+// are over near-list positions.  Every lane of the wave must call this.
 struct WpEntry {
-    int ii, k, kind;  // kind 0 pair, 1 wall, 2 copy forward
+    int ii, k, kind;  // kind 0 pair, 1 wall, 2 copy forward; k: near-list position or wall slot
     bool valid;
-    int nidx;         // pairs and copies: index of the slot's entry in nrec
+    int nidx;         // pairs and copies: index of the entry in the near list
 };
 
 DEV WpEntry wave_entry_pair(unsigned t, unsigned T, unsigned off, unsigned mlo, unsigned mhi,
-                            unsigned wmask, unsigned blo, unsigned bhi, unsigned nlo, unsigned nhi,
-                            int noff, int wbase) {
+                            unsigned wmask, unsigned blo, unsigned bhi, int noff, int wbase) {
     const unsigned tq = t < T ? t : T - 1;
     int o = 0;
 #pragma unroll
@@ -1759,12 +1751,9 @@ DEV WpEntry wave_entry_pair(unsigned t, unsigned T, unsigned off, unsigned mlo, 
     const unsigned ooff = (unsigned)__shfl((int)off, o, 64);
     const unsigned oblo = (unsigned)__shfl((int)blo, o, 64);
     const unsigned obhi = (unsigned)__shfl((int)bhi, o, 64);
-    const unsigned onlo = (unsigned)__shfl((int)nlo, o, 64);
-    const unsigned onhi = (unsigned)__shfl((int)nhi, o, 64);
     const int onoff = __shfl(noff, o, 64);
     const unsigned long long om = ((unsigned long long)ohi << 32) | olo;
     const unsigned long long ob = ((unsigned long long)obhi << 32) | oblo;
-    const unsigned long long on = ((unsigned long long)onhi << 32) | onlo;
     const unsigned r = tq - ooff, np = (unsigned)__popcll(om), nw = (unsigned)__popc(ow);
     WpEntry e;
     e.ii = wbase + o;
@@ -1779,7 +1768,7 @@ DEV WpEntry wave_entry_pair(unsigned t, unsigned T, unsigned off, unsigned mlo, 
         e.kind = 2;
         e.k = select_bit(ob, r - np - nw);
     }
-    e.nidx = onoff + (int)__popcll(on & ((1ull << e.k) - 1ull));
+    e.nidx = onoff + e.k;
     return e;
 }
 
@@ -1798,11 +1787,12 @@ DEV void step_wave_block(const DevInst &P, const StepArgs &A, unsigned blk, doub
 
     // phase A: contact masks of the pair slots and the wall slots.  Only
     // the slots of nmask are tested (the others cannot touch before the
-    // next rebuild), WV_NB at a time.  Pair mode also notes which contact
-    // slots own their record (ownm) and which owned contacts of the
-    // previous step (the mask in the spare word of the velocity record)
-    // ended (brk).
-    unsigned long long mask = 0ull, am_own = 0ull, ownm = 0ull, brk = 0ull, nm_own = 0ull;
+    // next rebuild), WV_NB at a time.  Pair mode walks the particle's near
+    // list instead (the same slots, in slot order; masks are over its
+    // positions) and also notes which contacts own their record (ownm) and
+    // which owned contacts of the previous step (the mask in the spare word
+    // of the velocity record) ended (brk).
+    unsigned long long mask = 0ull, am_own = 0ull, ownm = 0ull, brk = 0ull;
     unsigned wmask = 0u;
     int aoff_own = 0;
     double grav = 0.0;
@@ -1813,34 +1803,52 @@ DEV void step_wave_block(const DevInst &P, const StepArgs &A, unsigned blk, doub
             am_own = A.amask[i];
             aoff_own = A.actoff[i];
         }
-        unsigned long long nm = A.nmask[i];
         const double4 pi = A.in.p[i];
-        const int n = A.nnb[i], nw = A.nwl[i];
+        const int nw = A.nwl[i];
         const double4 vbi = A.in.v[2 * i + 1];
         st->p[lane] = pi;
         st->va[lane] = A.in.v[2 * i];
         st->vb[lane] = vbi;
-        if (n < 64) nm &= (1ull << n) - 1ull;
-        nm_own = nm;
-        while (nm) {
-            int ks[WV_NB], jj[WV_NB];
+        if constexpr (PAIR) {
+            const int nn = A.noff[i + 1] - aoff_own;
+            for (int r0 = 0; r0 < nn; r0 += WV_NB) {
+                int jj[WV_NB];
 #pragma unroll
-            for (int u = 0; u < WV_NB; ++u) {
-                ks[u] = nm ? __ffsll((long long)nm) - 1 : -1;
-                nm &= nm - 1ull;
+                for (int u = 0; u < WV_NB; ++u) jj[u] = r0 + u < nn ? A.nl[aoff_own + r0 + u].x : i;
+                double4 q[WV_NB];
+#pragma unroll
+                for (int u = 0; u < WV_NB; ++u) q[u] = A.in.p[jj[u]];
+#pragma unroll
+                for (int u = 0; u < WV_NB; ++u) {
+                    V3 nv;
+                    double gap;
+                    if (r0 + u < nn && pair_geometry<MIXED>(pi, q[u], nv, gap)) {
+                        mask |= 1ull << (r0 + u);
+                        if (owns_record(i, jj[u], pi.w, q[u].w)) ownm |= 1ull << (r0 + u);
+                    }
+                }
             }
+        } else {
+            unsigned long long nm = A.nmask[i];
+            const int n = A.nnb[i];
+            if (n < 64) nm &= (1ull << n) - 1ull;
+            while (nm) {
+                int ks[WV_NB], jj[WV_NB];
 #pragma unroll
-            for (int u = 0; u < WV_NB; ++u) jj[u] = ks[u] >= 0 ? A.nbr[(long)ks[u] * cap + i] : i;
-            double4 q[WV_NB];
+                for (int u = 0; u < WV_NB; ++u) {
+                    ks[u] = nm ? __ffsll((long long)nm) - 1 : -1;
+                    nm &= nm - 1ull;
+                }
 #pragma unroll
-            for (int u = 0; u < WV_NB; ++u) q[u] = A.in.p[jj[u]];
+                for (int u = 0; u < WV_NB; ++u) jj[u] = ks[u] >= 0 ? A.nbr[(long)ks[u] * cap + i] : i;
+                double4 q[WV_NB];
 #pragma unroll
-            for (int u = 0; u < WV_NB; ++u) {
-                V3 nv;
-                double gap;
-                if (ks[u] >= 0 && pair_geometry<MIXED>(pi, q[u], nv, gap)) {
-                    mask |= 1ull << ks[u];
-                    if (PAIR && (jj[u] > i || q[u].w != pi.w)) ownm |= 1ull << ks[u];
+                for (int u = 0; u < WV_NB; ++u) q[u] = A.in.p[jj[u]];
+#pragma unroll
+                for (int u = 0; u < WV_NB; ++u) {
+                    V3 nv;
+                    double gap;
+                    if (ks[u] >= 0 && pair_geometry<MIXED>(pi, q[u], nv, gap)) mask |= 1ull << ks[u];
                 }
             }
         }
@@ -1868,7 +1876,6 @@ DEV void step_wave_block(const DevInst &P, const StepArgs &A, unsigned blk, doub
     Fz = Fz - grav;  // gravity first (kernel.cu:439)
     const unsigned mlo = (unsigned)mask, mhi = (unsigned)(mask >> 32);
     const unsigned blo = (unsigned)brk, bhi = (unsigned)(brk >> 32);
-    const unsigned nlo = (unsigned)nm_own, nhi = (unsigned)(nm_own >> 32);
 
     // phase C, software-pipelined.  Directed records: a round first issues
     // the loads of its neighbours' state, then those of the next round's
@@ -1884,20 +1891,12 @@ DEV void step_wave_block(const DevInst &P, const StepArgs &A, unsigned blk, doub
     Rec hn;
     auto entry_at = [&](unsigned t) -> Entry {
         if constexpr (PAIR)
-            return wave_entry_pair(t, T, off, mlo, mhi, wmask, blo, bhi, nlo, nhi, aoff_own, wbase);
+            return wave_entry_pair(t, T, off, mlo, mhi, wmask, blo, bhi, aoff_own, wbase);
         else
             return wave_entry(t, T, off, mlo, mhi, wmask, wbase, am_own, aoff_own);
     };
     auto fetch = [&](const Entry &x) {
-        if constexpr (PAIR) {
-            if (x.kind == 1) {
-                j = A.wl[(long)x.k * cap + x.ii];
-            } else {
-                if (x.kind == 0) j = A.nbr[(long)x.k * cap + x.ii];
-                rn = A.nrec[x.nidx];
-                hn = A.hq_in[rn & 0x7fffffff];
-            }
-        } else {
+        if constexpr (!PAIR) {
             j = (x.wall ? A.wl : A.nbr)[(long)x.k * cap + x.ii];
             if (!x.wall) {
                 hin = hist_idx(x.am, x.aoff, x.k, x.ii, cap, Kcap);
@@ -1915,8 +1914,9 @@ DEV void step_wave_block(const DevInst &P, const StepArgs &A, unsigned blk, doub
             if (x.kind == 1) {
                 jj = A.wl[(long)x.k * cap + x.ii];
             } else {
-                if (x.kind == 0) jj = A.nbr[(long)x.k * cap + x.ii];
-                rr = A.nrec[x.nidx];
+                const int2 v = A.nl[x.nidx];
+                jj = v.x;
+                rr = v.y;
             }
         }
     };

@@ -54,13 +54,12 @@ void launch_pair_count(const double4 *p, const int *nbr, const int *nnb,
 void launch_pair_fill(const double4 *p, double4 *vw, const int *nbr, const int *nnb,
                       const unsigned long long *nmask, const unsigned *o, unsigned base2,
                       const HistRec *hd, const unsigned long long *amask, const int *actoff,
-                      int *nrec, PairHot *q0, PairHot *q1, double2 *c0, double2 *c1, int *bad,
+                      int2 *nl, PairHot *q0, PairHot *q1, double2 *c0, double2 *c1, int *bad,
                       int cap, long Kcap, int total, hipStream_t s);
-void launch_pair_unpack(const double4 *p, const int *nbr, const int *nnb,
-                        const unsigned long long *nmask, const unsigned *noff, const int *nrec,
-                        const PairHot *q, const double2 *c, HistRec *hd,
-                        const unsigned long long *amask, const int *actoff, int cap, long Kcap,
-                        int total, hipStream_t s);
+void launch_pair_unpack(const double4 *p, const int *nnb, const unsigned long long *nmask,
+                        const unsigned *noff, const int2 *nl, const PairHot *q, const double2 *c,
+                        HistRec *hd, const unsigned long long *amask, const int *actoff, int cap,
+                        long Kcap, int total, hipStream_t s);
 void launch_analyze(const DevInst *insts, const AnaArgs *args, double *out, int n, hipStream_t s);
 void launch_rebuild_batch(const DevInst *insts, const RbInst *tab, int ninst, const int2 *ptasks,
                           int nptasks, const int2 *ctasks, int nctasks, hipStream_t s);
