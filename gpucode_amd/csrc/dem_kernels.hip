@@ -1772,15 +1772,15 @@ DEV WpEntry wave_entry_pair(unsigned t, unsigned T, unsigned off, unsigned mlo, 
     return e;
 }
 
-// block blk of WV_BS particles; res is this wave's [7][64] LDS slice.
-// PAIR: the history comes from the shared pair records (see PAIR_MIRROR)
-// instead of one record per directed slot; the forces are the same bit
-// for bit.
-template <bool MIXED, bool PAIR>
+// block blk of BS particles (one wave per 64); res is this wave's [7][64]
+// LDS slice.  PAIR: the history comes from the shared pair records (see
+// PAIR_MIRROR) instead of one record per directed slot; the forces are the
+// same bit for bit.
+template <bool MIXED, bool PAIR, int BS = WV_BS>
 DEV void step_wave_block(const DevInst &P, const StepArgs &A, unsigned blk, double (*res)[64],
                          WvStash *st) {
     const int tid = threadIdx.x, lane = tid & 63;
-    const int i = (int)(blk * WV_BS) + tid;
+    const int i = (int)(blk * BS) + tid;
     const int wbase = i - lane;
     const bool own = i < A.total;
     const long cap = A.cap;
@@ -2047,26 +2047,29 @@ DEV void step_wave_block(const DevInst &P, const StepArgs &A, unsigned blk, doub
     request_rebuild(A, over);
 }
 
+// One wave per block: a wave slot that frees up takes the next 64
+// particles at once, instead of waiting until four slots of one CU are
+// free.  The XCD remap gives each XCD a contiguous range of particles.
 template <bool MIXED, bool PAIR>
-__global__ void __launch_bounds__(WV_BS) k_step_wave(DevInst P, StepArgs A) {
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3)))
+k_step_wave(DevInst P, StepArgs A) {
     if (__hip_atomic_load(A.stop_at, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <= A.step) return;
-    __shared__ double res_all[WV_BS / 64][7][64];
-    __shared__ WvStash stash[WV_BS / 64];
-    step_wave_block<MIXED, PAIR>(P, A, xcd_block(gridDim.x), res_all[threadIdx.x >> 6],
-                                 &stash[threadIdx.x >> 6]);
+    __shared__ double res[7][64];
+    __shared__ WvStash stash;
+    step_wave_block<MIXED, PAIR, 64>(P, A, xcd_block(gridDim.x), res, &stash);
+}
+
+template <bool MIXED, bool PAIR>
+static void launch_wave(const DevInst &P, const StepArgs &A, hipStream_t s) {
+    hipLaunchKernelGGL((k_step_wave<MIXED, PAIR>), dim3(blocks_for(A.total, 64)), dim3(64), 0, s, P, A);
 }
 
 void launch_step_wave(const DevInst &P, const StepArgs &A, bool mixed, bool pair, hipStream_t s) {
     if (A.total <= 0) return;
-    unsigned nb = blocks_for(A.total, WV_BS);
-    if (mixed && pair)
-        hipLaunchKernelGGL((k_step_wave<true, true>), dim3(nb), dim3(WV_BS), 0, s, P, A);
-    else if (mixed)
-        hipLaunchKernelGGL((k_step_wave<true, false>), dim3(nb), dim3(WV_BS), 0, s, P, A);
-    else if (pair)
-        hipLaunchKernelGGL((k_step_wave<false, true>), dim3(nb), dim3(WV_BS), 0, s, P, A);
-    else
-        hipLaunchKernelGGL((k_step_wave<false, false>), dim3(nb), dim3(WV_BS), 0, s, P, A);
+    if (mixed && pair) launch_wave<true, true>(P, A, s);
+    else if (mixed) launch_wave<true, false>(P, A, s);
+    else if (pair) launch_wave<false, true>(P, A, s);
+    else launch_wave<false, false>(P, A, s);
 }
 
 // ---------------------------------------------------------------------
