@@ -2050,26 +2050,35 @@ DEV void step_wave_block(const DevInst &P, const StepArgs &A, unsigned blk, doub
 // One wave per block: a wave slot that frees up takes the next 64
 // particles at once, instead of waiting until four slots of one CU are
 // free.  The XCD remap gives each XCD a contiguous range of particles.
+// The mixed-precision form is held to 3 waves per SIMD (it would take 171
+// registers and fit only 2); the FP64 form fits 3 by itself and is
+// scheduled better without the hint.
 template <bool MIXED, bool PAIR>
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3)))
-k_step_wave(DevInst P, StepArgs A) {
+DEV void step_wave_one(const DevInst &P, const StepArgs &A) {
     if (__hip_atomic_load(A.stop_at, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <= A.step) return;
     __shared__ double res[7][64];
     __shared__ WvStash stash;
     step_wave_block<MIXED, PAIR, 64>(P, A, xcd_block(gridDim.x), res, &stash);
 }
 
-template <bool MIXED, bool PAIR>
-static void launch_wave(const DevInst &P, const StepArgs &A, hipStream_t s) {
-    hipLaunchKernelGGL((k_step_wave<MIXED, PAIR>), dim3(blocks_for(A.total, 64)), dim3(64), 0, s, P, A);
+template <bool PAIR>
+__global__ void __launch_bounds__(64) k_step_wave(DevInst P, StepArgs A) {
+    step_wave_one<false, PAIR>(P, A);
+}
+
+template <bool PAIR>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3)))
+k_step_wave_mixed(DevInst P, StepArgs A) {
+    step_wave_one<true, PAIR>(P, A);
 }
 
 void launch_step_wave(const DevInst &P, const StepArgs &A, bool mixed, bool pair, hipStream_t s) {
     if (A.total <= 0) return;
-    if (mixed && pair) launch_wave<true, true>(P, A, s);
-    else if (mixed) launch_wave<true, false>(P, A, s);
-    else if (pair) launch_wave<false, true>(P, A, s);
-    else launch_wave<false, false>(P, A, s);
+    const dim3 g(blocks_for(A.total, 64)), b(64);
+    if (mixed && pair) hipLaunchKernelGGL((k_step_wave_mixed<true>), g, b, 0, s, P, A);
+    else if (mixed) hipLaunchKernelGGL((k_step_wave_mixed<false>), g, b, 0, s, P, A);
+    else if (pair) hipLaunchKernelGGL((k_step_wave<true>), g, b, 0, s, P, A);
+    else hipLaunchKernelGGL((k_step_wave<false>), g, b, 0, s, P, A);
 }
 
 // ---------------------------------------------------------------------
