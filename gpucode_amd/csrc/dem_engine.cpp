@@ -508,13 +508,18 @@ int dem_pair_pack(dem_ctx *c) {
     const long cap = c->cap, Kcap = (long)c->K * c->cap;
     hipStream_t s = c->stream;
     if (!c->pcnt) {
-        HIPCHK(hipMalloc((void **)&c->pcnt, 3 * (size_t)cap * sizeof(unsigned)));
+        HIPCHK(hipMalloc((void **)&c->pcnt, 4 * (size_t)cap * sizeof(unsigned)));
         HIPCHK(hipMalloc((void **)&c->poff, (3 * (size_t)cap + 1) * sizeof(unsigned)));
         HIPCHK(hipMalloc((void **)&c->pair_err, sizeof(int)));
     }
     launch_pair_count(c->P4[c->cur], c->nbr[c->hcur], c->nnb[c->hcur], c->nmask, c->pcnt, c->cap, n, s);
     for (int q = 0; q < 3; ++q)
         launch_excl_scan(c->pcnt + q * cap, c->poff + q * cap, n, c->sort_tmp, s);
+    // work of each chunk of 64 particles, for the XCD ranges of the wave kernel
+    const int nch = (n + 63) / 64;
+    std::vector<unsigned> work(nch);
+    launch_chunk_work(c->pcnt + 3 * cap, c->keys2, n, s);
+    HIPCHK(hipMemcpyAsync(work.data(), c->keys2, nch * sizeof(unsigned), hipMemcpyDeviceToHost, s));
     unsigned last[6];
     for (int q = 0; q < 3; ++q) {
         HIPCHK(hipMemcpyAsync(&last[2 * q], c->pcnt + q * cap + (n - 1), sizeof(unsigned),
@@ -524,6 +529,20 @@ int dem_pair_pack(dem_ctx *c) {
     }
     HIPCHK(hipStreamSynchronize(s));
     const long t1 = (long)last[0] + last[1], t2 = (long)last[2] + last[3], tn = (long)last[4] + last[5];
+    {
+        double W = 0.0;
+        for (unsigned w : work) W += w;
+        double acc = 0.0;
+        int x = 1;
+        c->xb_work[0] = 0;
+        for (int ch = 0; ch < nch && x < 8; ++ch) {
+            while (x < 8 && acc >= W * x / 8.0) c->xb_work[x++] = (unsigned)ch;
+            acc += work[ch];
+        }
+        while (x < 8) c->xb_work[x++] = (unsigned)nch;
+        c->xb_work[8] = (unsigned)nch;
+        c->xb_work_ok = true;
+    }
     if (t1 + t2 > 0x7fffffffL || tn > 0x7fffffffL)
         return dem_fail(DEM_ERR_NOMEM, "too many near slots for 32-bit pair record indices");
     long h[4] = {c->hq_cap, c->hq_cap, c->hq_cap, c->hq_cap};
@@ -792,6 +811,12 @@ static int run_chunk(dem_ctx *c, int nsteps, int *executed) {
     if (pair) {
         A.noff = (const int *)(c->poff + 2L * c->cap);
         A.nl = c->nl;
+    }
+    if (pair && c->xb_work_ok) {
+        for (int x = 0; x < 9; ++x) A.xb[x] = c->xb_work[x];
+    } else {
+        const unsigned nch = (unsigned)(c->total + 63) / 64u, per = nch / 8u, rem = nch % 8u;
+        for (unsigned x = 0; x < 9; ++x) A.xb[x] = x * per + (x < rem ? x : rem);
     }
     double tw = c->Tw;
     HIPCHK(hipEventRecord(c->ev0, c->stream));

@@ -65,6 +65,8 @@ struct dem_ctx {
     unsigned *pcnt = nullptr, *poff = nullptr;  // 3 x cap counts and their scans (+1: near total)
     int *pair_err = nullptr;
     int n_pack = 0;
+    unsigned xb_work[9] = {};    // wave kernel: XCD ranges of chunks cut for equal work at the pack
+    bool xb_work_ok = false;
     int relayout_every = 1024;   // steps between layout refreshes (0: never)
     long long layout_age = 0;    // steps since the layout was made
     int n_relayout = 0;

@@ -146,6 +146,7 @@ struct StepArgs {
     int *stop_at;              // device rebuild request (step index)
     int cap, total, step, store_forces;
     int K;                     // neighbour slots per particle (list stride)
+    unsigned xb[9];            // wave kernel: chunks [xb[x], xb[x+1]) go to XCD x
     double Tw;                 // lid height for this step
 };
 

@@ -739,29 +739,51 @@ DEV PairHot pair_compress(const HistRec &h, double R, double2 &cold, bool &need_
 }
 
 // per particle: owned near slots in contact now (c[0]), other owned near
-// slots (c[1]), near slots (c[2]); c[q] is c + q*cap
+// slots (c[1]), near slots (c[2]), near slots in contact now (c[3]); c[q]
+// is c + q*cap
 __global__ void k_pair_count(const double4 *p, const int *nbr, const int *nnb,
                              const unsigned long long *nmask, unsigned *c, int cap, int total) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= total) return;
     const double4 pi = p[i];
     unsigned long long nm = near_slots(nmask, nnb, i);
-    unsigned a = 0, b = 0;
+    unsigned a = 0, b = 0, t = 0;
     const unsigned n = (unsigned)__popcll(nm);
     while (nm) {
         const int k = __ffsll((long long)nm) - 1;
         nm &= nm - 1ull;
         const int j = nbr[(long)k * cap + i];
         const double4 q = p[j];
-        if (!owns_record(i, j, pi.w, q.w)) continue;
         const double dx = pi.x - q.x, dy = pi.y - q.y, dz = pi.z - q.z;
-        const double gap = sqrt(dx * dx + dy * dy + dz * dz) - pi.w - q.w;
-        if (gap < 0.0) a++;
+        const bool touch = sqrt(dx * dx + dy * dy + dz * dz) - pi.w - q.w < 0.0;
+        t += touch ? 1u : 0u;
+        if (!owns_record(i, j, pi.w, q.w)) continue;
+        if (touch) a++;
         else b++;
     }
     c[i] = a;
     c[(long)cap + i] = b;
     c[2L * cap + i] = n;
+    c[3L * cap + i] = t;
+}
+
+// work of each chunk of 64 particles for the wave kernel: 72 + its contacts
+// (a wave's time, fitted on the 2M bed, is about 6.8 us + 0.094 us per
+// contact entry)
+__global__ void k_chunk_work(const unsigned *contacts, unsigned *work, int total) {
+    const int ch = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lo = ch * 64;
+    if (lo >= total) return;
+    const int hi = lo + 64 < total ? lo + 64 : total;
+    unsigned w = 72;
+    for (int i = lo; i < hi; ++i) w += contacts[i];
+    work[ch] = w;
+}
+
+void launch_chunk_work(const unsigned *contacts, unsigned *work, int total, hipStream_t s) {
+    if (total <= 0) return;
+    hipLaunchKernelGGL(k_chunk_work, dim3(blocks_for(blocks_for(total, 64), 256)), dim3(256), 0, s,
+                       contacts, work, total);
 }
 
 // owned slots: assign records, store the directed records in both buffers,
@@ -2049,16 +2071,23 @@ DEV void step_wave_block(const DevInst &P, const StepArgs &A, unsigned blk, doub
 
 // One wave per block: a wave slot that frees up takes the next 64
 // particles at once, instead of waiting until four slots of one CU are
-// free.  The XCD remap gives each XCD a contiguous range of particles.
+// free.  Each XCD walks a contiguous range of chunks of 64 particles
+// (A.xb), so neighbouring particles share its L2; the ranges are cut for
+// equal work (k_chunk_work) where the pair records are in use, else for
+// equal particle counts.
 // The mixed-precision form is held to 3 waves per SIMD (it would take 171
 // registers and fit only 2); the FP64 form fits 3 by itself and is
 // scheduled better without the hint.
 template <bool MIXED, bool PAIR>
 DEV void step_wave_one(const DevInst &P, const StepArgs &A) {
     if (__hip_atomic_load(A.stop_at, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <= A.step) return;
+    // blocks are dispatched round-robin over the 8 XCDs: block b runs on
+    // XCD b % 8 and takes the (b / 8)-th chunk of that XCD's range
+    const unsigned x = blockIdx.x % 8u, c = A.xb[x] + blockIdx.x / 8u;
+    if (c >= A.xb[x + 1]) return;
     __shared__ double res[7][64];
     __shared__ WvStash stash;
-    step_wave_block<MIXED, PAIR, 64>(P, A, xcd_block(gridDim.x), res, &stash);
+    step_wave_block<MIXED, PAIR, 64>(P, A, c, res, &stash);
 }
 
 template <bool PAIR>
@@ -2074,7 +2103,9 @@ k_step_wave_mixed(DevInst P, StepArgs A) {
 
 void launch_step_wave(const DevInst &P, const StepArgs &A, bool mixed, bool pair, hipStream_t s) {
     if (A.total <= 0) return;
-    const dim3 g(blocks_for(A.total, 64)), b(64);
+    unsigned most = 0;
+    for (int x = 0; x < 8; ++x) most = A.xb[x + 1] - A.xb[x] > most ? A.xb[x + 1] - A.xb[x] : most;
+    const dim3 g(8 * most), b(64);
     if (mixed && pair) hipLaunchKernelGGL((k_step_wave_mixed<true>), g, b, 0, s, P, A);
     else if (mixed) hipLaunchKernelGGL((k_step_wave_mixed<false>), g, b, 0, s, P, A);
     else if (pair) hipLaunchKernelGGL((k_step_wave<true>), g, b, 0, s, P, A);

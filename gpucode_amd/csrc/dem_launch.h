@@ -46,8 +46,10 @@ void launch_step_fused(const DevInst &P, const StepArgs &A, bool wet, bool mixed
 void launch_step_compact(const DevInst &P, const StepArgs &A, int K, bool mixed, hipStream_t s);
 // pair: shared pair records (A.noff, A.nrec, A.hq_in, A.hq_out; see PAIR_MIRROR)
 void launch_step_wave(const DevInst &P, const StepArgs &A, bool mixed, bool pair, hipStream_t s);
-// shared pair records: counts (c: 3 arrays of cap), fill from the directed
+// shared pair records: counts (c: 4 arrays of cap), fill from the directed
 // records (o: exclusive scans of the counts), write back
+// work of each chunk of 64 particles (contacts: per-particle contact counts)
+void launch_chunk_work(const unsigned *contacts, unsigned *work, int total, hipStream_t s);
 void launch_pair_count(const double4 *p, const int *nbr, const int *nnb,
                        const unsigned long long *nmask, unsigned *c, int cap, int total,
                        hipStream_t s);
