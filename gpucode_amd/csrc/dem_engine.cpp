@@ -15,6 +15,7 @@
 #include "dem_launch.h"
 
 static const int kChunk = 64;
+static const int kPairMinInterval = 64;
 
 StepArgs dem_step_args(dem_ctx *c, bool fused) {
     StepArgs A;
@@ -64,6 +65,8 @@ static int check_overflow(dem_ctx *c) {
 static void reset_lists(dem_ctx *c) {
     c->pair_packed = false;
     c->pair_bad = false;
+    c->iter_at_rebuild = -1;
+    c->rebuild_interval = -1;
     c->has_lists = false;
     c->saved = false;
     c->old_total = 0;
@@ -596,7 +599,8 @@ extern "C" int dem_set_pair_records(dem_ctx *c, int on) {
 }
 
 extern "C" int dem_pair_records_used(dem_ctx *c) {
-    return c && c->pair_on && !c->pair_bad && dem_step_kernel(c) == DEM_KERNEL_WAVE;
+    return c && c->pair_on && !c->pair_bad && dem_step_kernel(c) == DEM_KERNEL_WAVE &&
+           (c->rebuild_interval < 0 || c->rebuild_interval >= kPairMinInterval);
 }
 
 // ---------------------------------------------------------------------
@@ -669,6 +673,8 @@ int dem_do_rebuild(dem_ctx *c) {
         return rc;
     c->rebuild_pending = false;
     c->n_rebuild++;
+    c->rebuild_interval = c->iter_at_rebuild >= 0 ? c->iter - c->iter_at_rebuild : -1;
+    c->iter_at_rebuild = c->iter;
     c->forces_valid = false;
     c->layout_age = 0;
     return DEM_OK;
@@ -800,9 +806,15 @@ static int run_chunk(dem_ctx *c, int nsteps, int *executed) {
     const bool wet = c->S.forceCap == 1;
     const int kern = dem_step_kernel(c);
     const bool mixed = c->precision == DEM_PREC_MIXED;
+    // A pack and its write-back cost about as much as 30 steps gain from
+    // the pair records, so they are made only while rebuilds come at least
+    // kPairMinInterval steps apart (or their spacing is not known yet).
     int rc;
-    if (kern == DEM_KERNEL_WAVE && c->pair_on) rc = dem_pair_pack(c);
-    else rc = dem_pair_unpack(c);
+    if (kern == DEM_KERNEL_WAVE && c->pair_on &&
+        (c->rebuild_interval < 0 || c->rebuild_interval >= kPairMinInterval))
+        rc = dem_pair_pack(c);
+    else
+        rc = dem_pair_unpack(c);
     if (rc) return rc;
     const bool pair = c->pair_packed;
     int big = INT_MAX;

@@ -90,6 +90,8 @@ struct dem_ctx {
 
     // host bookkeeping (main.cpp:425-623)
     int iter = 0, n_rebuild = 0;
+    int iter_at_rebuild = -1;     // iter of the last rebuild
+    int rebuild_interval = -1;    // steps between the last two rebuilds (-1: not known)
     double ctime = 0.0;
     bool rebuild_pending = true;
     bool forces_valid = false;

@@ -145,6 +145,25 @@ def test_unequal_radii_keep_one_record_per_direction():
     assert_same(a, b)
 
 
+def test_frequent_rebuilds_keep_the_directed_records():
+    # a skin of 0.01 d: a rebuild every few dozen steps, too often for a
+    # pack to pay; after the first rebuild interval is known the engine
+    # keeps to the directed records, with the same results
+    d = 1.0e-3
+    n, R_d = 3000, 8.0
+    H = d * (n / (np.pi * R_d * R_d * 0.5) + 6)
+    inp = W.make_input(n, d, R_d * d, H, 0.5, cutGap=0.01)
+    w = dict(inp=inp, dia=None, pos=W.pocket_bed(n, R_d, 1.0, jitter=0.03, seed=7))
+    vel = np.random.default_rng(8).normal(size=(n, 3)) * 20.0
+    a, b = engines(w, G.PREC_FP64, vel)
+    assert a.pair_records_used
+    a.step(300)
+    b.step(300)
+    assert a.n_rebuild == b.n_rebuild and a.n_rebuild >= 4
+    assert not a.pair_records_used
+    assert_same(a, b)
+
+
 def test_records_that_are_not_mirrors_fall_back(tmp_path):
     # a restart whose history breaks the mirror property of one contact:
     # the pair records are refused and the directed records give the same
