@@ -1826,6 +1826,10 @@ DEV void step_wave_block(const DevInst &P, const StepArgs &A, unsigned blk, doub
             aoff_own = A.actoff[i];
         }
         const double4 pi = A.in.p[i];
+        // the wave's first loads and the near-slot gathers that depend on
+        // them are issued at raised priority, so a new wave reaches its
+        // contact rounds sooner while older waves compute
+        __builtin_amdgcn_s_setprio(2);
         const int nw = A.nwl[i];
         const double4 vbi = A.in.v[2 * i + 1];
         st->p[lane] = pi;
@@ -1874,6 +1878,7 @@ DEV void step_wave_block(const DevInst &P, const StepArgs &A, unsigned blk, doub
                 }
             }
         }
+        __builtin_amdgcn_s_setprio(0);
         if constexpr (PAIR) brk = (unsigned long long)__double_as_longlong(vbi.w) & ~mask;
         const V3 ip = v3(pi.x, pi.y, pi.z);
         for (int k = 0; k < nw; ++k) {
