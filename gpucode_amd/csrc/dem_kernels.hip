@@ -1,10 +1,12 @@
 // dem_kernels.hip -- hand-written gfx950 kernels of the DEM time step.
 //
 // One thread owns one particle i (cell-sorted order) and gathers every
-// directed pair (i, j) itself, so forces accumulate in registers with no
-// atomics; each directed pair has its own history slot, as in the reference
-// (kernel.cu:413-552).  Work is HBM-bound integer/FP64 streaming; nothing
-// here is a dense contraction, so there is no MFMA.
+// directed pair (i, j) for itself, so forces accumulate in registers with
+// no atomics.  Each directed pair has its own history slot, as in the
+// reference (kernel.cu:413-552), except in the wave kernel's pair mode,
+// where a pair of equal radii shares one record (see PAIR_MIRROR in
+// dem_internal.h).  Work is HBM-bound integer/FP64 streaming; nothing here
+// is a dense contraction, so there is no MFMA.
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <stdint.h>
