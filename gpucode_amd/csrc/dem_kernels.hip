@@ -2089,9 +2089,13 @@ template <bool MIXED, bool PAIR>
 DEV void step_wave_one(const DevInst &P, const StepArgs &A) {
     if (__hip_atomic_load(A.stop_at, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <= A.step) return;
     // blocks are dispatched round-robin over the 8 XCDs: block b runs on
-    // XCD b % 8 and takes the (b / 8)-th chunk of that XCD's range
-    const unsigned x = blockIdx.x % 8u, c = A.xb[x] + blockIdx.x / 8u;
-    if (c >= A.xb[x + 1]) return;
+    // XCD b % 8 and takes the (b / 8)-th chunk of that XCD's range.  Odd
+    // steps walk each range backwards, so a launch starts where the one
+    // before ended, on state and records still in L2 (the order changes
+    // nothing else: each particle's step reads only the previous step).
+    const unsigned x = blockIdx.x % 8u, k = blockIdx.x / 8u;
+    if (A.xb[x] + k >= A.xb[x + 1]) return;
+    const unsigned c = (A.step & 1) ? A.xb[x + 1] - 1u - k : A.xb[x] + k;
     __shared__ double res[7][64];
     __shared__ WvStash stash;
     step_wave_block<MIXED, PAIR, 64>(P, A, c, res, &stash);
