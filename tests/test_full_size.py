@@ -1,11 +1,15 @@
-"""Full size: the 2,000,000-sphere bed of the throughput benchmark (d = 0.3 mm,
-cylinder radius 100 d, skin 0.1 d, mu 0.5, FP64), on the GPU path the
-benchmark times (wave kernel, shared pair records).
+"""Full sizes: the 2,000,000-sphere bed of the throughput benchmark (d = 0.3
+mm, cylinder radius 100 d, skin 0.1 d, mu 0.5, FP64) on the GPU path the
+benchmark times (wave kernel, shared pair records), and 200,000 wet
+polydisperse spheres (lognormal diameters, three size levels, capillary
+bridges), the pour of the second benchmark workload, and a dense wet bed of
+200,000 spheres in which bridges and contacts are active.
 
 Bars:
-  * after 40 steps from the seeded bed, every neighbour list, position,
-    velocity, spin and contact-history field equals the FP64 oracle's, bit
-    for bit (the same bar as the small parity tests, at the full size);
+  * every neighbour list, position, velocity, spin and contact-history field
+    (bridge flags and counts for the wet cases) equals the FP64 oracle's, bit
+    for bit, after 40 steps of the dry bed, 300 of the pour and 60 of the wet
+    bed (the bar of the small parity tests, at the full sizes);
   * size-independent properties of the binning: the reference-order
     (hash, index) table is sorted by hash and its index column is a
     permutation of the particles; the cell ranges tile it.
@@ -48,4 +52,46 @@ def test_two_million_spheres_bit_exact_against_oracle():
     used = cs >= 0
     assert np.array_equal(ce[used] - cs[used], np.bincount(h, minlength=len(cs))[used])
     assert (ce[used] - cs[used]).sum() == n
+    e.close()
+
+
+def compare_all(e, o):
+    assert e.n_rebuild == o.n_rebuild
+    for u, v in zip(e.particles(), o.particles()):
+        assert np.array_equal(u, v)
+    for u, v in zip(e.nebor(), o.nebor()):
+        assert np.array_equal(u, v)
+    assert np.array_equal(e.lqdcnt(), o.lqdcnt())
+    he, ho = e.hist_pp(), o.hist_pp()
+    for f in list(he):
+        assert np.array_equal(he[f], ho[f]), f
+
+
+def test_wet_polydisperse_pour_full_size():
+    w = W.wet_poly(n=200_000)
+    O.threads(16)
+    e = G.Engine(w["inp"], w["dia"])
+    o = O.Oracle(w["inp"], w["dia"])
+    e.set_particles(w["pos"])
+    o.set_particles(w["pos"])
+    e.step(300)
+    o.step(300)
+    compare_all(e, o)
+    e.close()
+
+
+def test_dense_wet_bed_200k():
+    # monodisperse, so the pocket-deposited spheres touch from the start
+    # (the pour above covers the polydisperse size levels)
+    w = W.wet_bed(n=200_000, R_d=60.0, reference_law=False, theta_deg=20.0, liquid=0.01)
+    O.threads(16)
+    e = G.Engine(w["inp"], w["dia"])
+    o = O.Oracle(w["inp"], w["dia"])
+    e.set_particles(w["pos"])
+    o.set_particles(w["pos"])
+    e.step(60)
+    o.step(60)
+    st = e.stats()
+    assert st["n_contact"] > 10_000 and st["n_bridge"] > 10_000
+    compare_all(e, o)
     e.close()
