@@ -67,7 +67,7 @@ struct dem_ctx {
     int n_pack = 0;
     unsigned xb_work[9] = {};    // wave kernel: XCD ranges of chunks cut for equal work at the pack
     bool xb_work_ok = false;
-    int relayout_every = 1024;   // steps between layout refreshes (0: never)
+    int relayout_every = 2048;   // steps between layout refreshes (0: never)
     long long layout_age = 0;    // steps since the layout was made
     int n_relayout = 0;
     bool has_lists = false, saved = false;

@@ -235,7 +235,7 @@ int dem_set_kernel(dem_ctx *ctx, int variant);
  * order of the cells, 0 = the reference's row-major cell hash.  Lists,
  * forces and the exported hash/cell tables are identical either way. */
 int dem_set_order(dem_ctx *ctx, int morton);
-/* History layout refresh in dem_step: every `every` steps (default 1024;
+/* History layout refresh in dem_step: every `every` steps (default 2048;
  * 0 turns it off) without a rebuild, the contact records are regathered
  * into the compact per-wave region from the contacts of that moment.
  * Records only move, so results are identical with any setting.  No
