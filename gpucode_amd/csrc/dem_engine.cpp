@@ -938,7 +938,7 @@ extern "C" int dem_get_timing(dem_ctx *c, double *fused_ms, long long *fused_ste
 extern "C" const char *dem_step_kernel_name(dem_ctx *c) {
     if (!c) return "";
     switch (dem_step_kernel(c)) {
-    case DEM_KERNEL_WAVE: return "k_step_wave";
+    case DEM_KERNEL_WAVE: return c->precision == DEM_PREC_MIXED ? "k_step_wave_mixed" : "k_step_wave";
     case DEM_KERNEL_COMPACT: return "k_step_compact";
     default: return "k_step_fused";
     }

@@ -269,8 +269,8 @@ int dem_get_timing(dem_ctx *ctx, double *fused_ms, long long *fused_steps,
 /* Bytes of device memory held by the context. */
 long long dem_device_bytes(dem_ctx *ctx);
 /* Name of the step kernel dem_step launches for this context's packing
- * ("k_step_wave", "k_step_compact" or "k_step_fused"); a measurement aid
- * with no reference counterpart. */
+ * ("k_step_wave", "k_step_wave_mixed", "k_step_compact" or "k_step_fused");
+ * a measurement aid with no reference counterpart. */
 const char *dem_step_kernel_name(dem_ctx *ctx);
 /* Measurement utility with no reference counterpart: a plain device copy
  * of nbytes, timed over iters launches with HIP events; *gbps = read +
