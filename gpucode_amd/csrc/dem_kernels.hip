@@ -1641,6 +1641,7 @@ void launch_step_compact(const DevInst &P, const StepArgs &A, int K, bool mixed,
 // ---------------------------------------------------------------------
 #define WV_BS 256
 #define WV_NB 8
+#define WV_TAIL 192u  // chunks per XCD done as half-waves at the end of a launch
 
 // per-wave LDS copy of the owners' records (read by the contact lanes of
 // phase C and by the owners in phase D instead of re-reading them)
@@ -1802,17 +1803,18 @@ DEV WpEntry wave_entry_pair(unsigned t, unsigned T, unsigned off, unsigned mlo, 
     return e;
 }
 
-// block blk of BS particles (one wave per 64); res is this wave's [7][64]
+// block blk of BS particles (one wave per 64; lanes outside [lane_lo,
+// lane_hi) own no particle, for half-waves); res is this wave's [7][64]
 // LDS slice.  PAIR: the history comes from the shared pair records (see
 // PAIR_MIRROR) instead of one record per directed slot; the forces are the
 // same bit for bit.
 template <bool MIXED, bool PAIR, int BS = WV_BS>
 DEV void step_wave_block(const DevInst &P, const StepArgs &A, unsigned blk, double (*res)[64],
-                         WvStash *st) {
+                         WvStash *st, int lane_lo = 0, int lane_hi = 64) {
     const int tid = threadIdx.x, lane = tid & 63;
     const int i = (int)(blk * BS) + tid;
     const int wbase = i - lane;
-    const bool own = i < A.total;
+    const bool own = i < A.total && lane >= lane_lo && lane < lane_hi;
     const long cap = A.cap;
 
     // phase A: contact masks of the pair slots and the wall slots.  Only
@@ -2099,12 +2101,23 @@ DEV void step_wave_one(const DevInst &P, const StepArgs &A) {
     // steps walk each range backwards, so a launch starts where the one
     // before ended, on state and records still in L2 (the order changes
     // nothing else: each particle's step reads only the previous step).
+    // The last L chunks an XCD takes are split into two half-waves of 32
+    // particles each, so the launch ends on short waves.
     const unsigned x = blockIdx.x % 8u, k = blockIdx.x / 8u;
-    if (A.xb[x] + k >= A.xb[x + 1]) return;
-    const unsigned c = (A.step & 1) ? A.xb[x + 1] - 1u - k : A.xb[x] + k;
+    const unsigned n = A.xb[x + 1] - A.xb[x], L = n / 4u < WV_TAIL ? n / 4u : WV_TAIL;
+    unsigned pos = k;
+    int lo = 0, hi = 64;
+    if (k >= n - L) {
+        const unsigned j = k - (n - L);
+        if (j >= 2u * L) return;
+        pos = n - L + j / 2u;
+        lo = (int)(j & 1u) * 32;
+        hi = lo + 32;
+    }
+    const unsigned c = (A.step & 1) ? A.xb[x + 1] - 1u - pos : A.xb[x] + pos;
     __shared__ double res[7][64];
     __shared__ WvStash stash;
-    step_wave_block<MIXED, PAIR, 64>(P, A, c, res, &stash);
+    step_wave_block<MIXED, PAIR, 64>(P, A, c, res, &stash, lo, hi);
 }
 
 template <bool PAIR>
@@ -2121,7 +2134,10 @@ k_step_wave_mixed(DevInst P, StepArgs A) {
 void launch_step_wave(const DevInst &P, const StepArgs &A, bool mixed, bool pair, hipStream_t s) {
     if (A.total <= 0) return;
     unsigned most = 0;
-    for (int x = 0; x < 8; ++x) most = A.xb[x + 1] - A.xb[x] > most ? A.xb[x + 1] - A.xb[x] : most;
+    for (int x = 0; x < 8; ++x) {
+        const unsigned n = A.xb[x + 1] - A.xb[x], L = n / 4u < WV_TAIL ? n / 4u : WV_TAIL;
+        most = n + L > most ? n + L : most;
+    }
     const dim3 g(8 * most), b(64);
     if (mixed && pair) hipLaunchKernelGGL((k_step_wave_mixed<true>), g, b, 0, s, P, A);
     else if (mixed) hipLaunchKernelGGL((k_step_wave_mixed<false>), g, b, 0, s, P, A);
