@@ -1642,6 +1642,7 @@ void launch_step_compact(const DevInst &P, const StepArgs &A, int K, bool mixed,
 #define WV_BS 256
 #define WV_NB 8
 #define WV_TAIL 192u  // chunks per XCD done as half-waves at the end of a launch
+#define WV_QTAIL 32u  // the last of those done as quarter-waves
 
 // per-wave LDS copy of the owners' records (read by the contact lanes of
 // phase C and by the owners in phase D instead of re-reading them)
@@ -1804,8 +1805,8 @@ DEV WpEntry wave_entry_pair(unsigned t, unsigned T, unsigned off, unsigned mlo, 
 }
 
 // block blk of BS particles (one wave per 64; lanes outside [lane_lo,
-// lane_hi) own no particle, for half-waves); res is this wave's [7][64]
-// LDS slice.  PAIR: the history comes from the shared pair records (see
+// lane_hi) own no particle, for half- and quarter-waves); res is this
+// wave's [7][64] LDS slice.  PAIR: the history comes from the shared pair records (see
 // PAIR_MIRROR) instead of one record per directed slot; the forces are the
 // same bit for bit.
 template <bool MIXED, bool PAIR, int BS = WV_BS>
@@ -2102,17 +2103,26 @@ DEV void step_wave_one(const DevInst &P, const StepArgs &A) {
     // before ended, on state and records still in L2 (the order changes
     // nothing else: each particle's step reads only the previous step).
     // The last L chunks an XCD takes are split into two half-waves of 32
-    // particles each, so the launch ends on short waves.
+    // particles each, and the last Qn of them into four quarter-waves of
+    // 16, so the launch ends on short waves.
     const unsigned x = blockIdx.x % 8u, k = blockIdx.x / 8u;
     const unsigned n = A.xb[x + 1] - A.xb[x], L = n / 4u < WV_TAIL ? n / 4u : WV_TAIL;
+    const unsigned Qn = L / 6u < WV_QTAIL ? L / 6u : WV_QTAIL;
     unsigned pos = k;
     int lo = 0, hi = 64;
     if (k >= n - L) {
         const unsigned j = k - (n - L);
-        if (j >= 2u * L) return;
-        pos = n - L + j / 2u;
-        lo = (int)(j & 1u) * 32;
-        hi = lo + 32;
+        if (j < 2u * (L - Qn)) {
+            pos = n - L + j / 2u;
+            lo = (int)(j & 1u) * 32;
+            hi = lo + 32;
+        } else {
+            const unsigned q = j - 2u * (L - Qn);
+            if (q >= 4u * Qn) return;
+            pos = n - Qn + q / 4u;
+            lo = (int)(q & 3u) * 16;
+            hi = lo + 16;
+        }
     }
     const unsigned c = (A.step & 1) ? A.xb[x + 1] - 1u - pos : A.xb[x] + pos;
     __shared__ double res[7][64];
@@ -2136,7 +2146,8 @@ void launch_step_wave(const DevInst &P, const StepArgs &A, bool mixed, bool pair
     unsigned most = 0;
     for (int x = 0; x < 8; ++x) {
         const unsigned n = A.xb[x + 1] - A.xb[x], L = n / 4u < WV_TAIL ? n / 4u : WV_TAIL;
-        most = n + L > most ? n + L : most;
+        const unsigned Qn = L / 6u < WV_QTAIL ? L / 6u : WV_QTAIL;
+        most = n + L + 2u * Qn > most ? n + L + 2u * Qn : most;
     }
     const dim3 g(8 * most), b(64);
     if (mixed && pair) hipLaunchKernelGGL((k_step_wave_mixed<true>), g, b, 0, s, P, A);
