@@ -145,7 +145,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None,
-                    help="timed steps (default 5,000 for packed_bed; 50,000, the whole pour, for "
+                    help="timed steps (default 10,000 for packed_bed; 50,000, the whole pour, for "
                          "wet_poly)")
     ap.add_argument("--warmup", type=int, default=None,
                     help="untimed steps first (default 300 for packed_bed, 0 for wet_poly)")
@@ -187,7 +187,7 @@ def main():
         desc = f"packed_bed_{n_req // 1000000}M (dry, monodisperse d=0.3mm, R=30mm, skin 0.1d, mu 0.5)"
         metric = "particle-steps/s at 2M spheres"
         tag = f"packed_bed_{n_req // 1000000}M"
-        steps = 5000 if args.steps is None else args.steps
+        steps = 10000 if args.steps is None else args.steps
         warmup = 300 if args.warmup is None else args.warmup
         nseg = 1
     else:
