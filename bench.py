@@ -244,7 +244,8 @@ def main():
             log(f"segment {i + 1}/{nseg}: {k} steps, {el:.2f}s so far, rebuilds "
                 f"{e.n_rebuild - rb0}, contacts/particle {st['n_contact'] / n:.2f}")
         st_prev, tm_prev = st, tm
-    log(f"timed {steps} steps in {el:.3f}s, rebuilds {e.n_rebuild - rb0}")
+    log(f"timed {steps} steps in {el:.3f}s, rebuilds {e.n_rebuild - rb0}, contacts/particle "
+        f"{st_first['n_contact'] / n:.2f} -> {st_prev['n_contact'] / n:.2f}")
 
     st = {k: 0.5 * (st_first[k] + st_prev[k]) for k in st_first}
     _, n_nb, n_c, n_w = algorithmic_bytes(st, n)
