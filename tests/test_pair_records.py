@@ -201,3 +201,22 @@ def test_records_that_are_not_mirrors_fall_back(tmp_path):
     b.step(100)
     assert not a.pair_records_used
     assert_same(a, b)
+
+
+def test_wave_kernel_steps_every_particle_once_at_100k():
+    # 100,000 spheres: 1,563 chunks of 64, about 195 per XCD, of which the
+    # last 48 run as half-waves and 8 of those as quarter-waves; the wave
+    # kernel (with its XCD ranges, alternating direction and split tail)
+    # must equal the one-thread loop kernel bit for bit
+    w = W.small_bed(n=100_000, R_d=40.0)
+    vel = np.random.default_rng(3).normal(size=(100_000, 3)) * 5.0
+    a = G.Engine(w["inp"])
+    b = G.Engine(w["inp"])
+    b.set_kernel(G.Engine.KERNEL_LOOP)
+    for x in (a, b):
+        x.set_particles(w["pos"], vel)
+        x.store_forces(True)
+    a.step(51)
+    b.step(51)
+    assert a.step_kernel_name() == "k_step_wave" and a.pair_records_used
+    assert_same(a, b, hist=False)
