@@ -171,7 +171,8 @@ def main():
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}, WORLD_SIZE "
-              f"{world}); a packing is not split across GPUs in this build.", file=sys.stderr)
+              f"{world}); the engine steps a packing on one GPU and does not split it "
+              f"across GPUs.", file=sys.stderr)
         sys.exit(2)
 
     import gpucode_amd as G
