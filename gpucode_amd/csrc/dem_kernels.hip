@@ -1409,14 +1409,23 @@ DEV void request_rebuild(const StepArgs &A, bool mine) {
         atomicMin(A.stop_at, A.step + 1);
 }
 
-// one thread per particle: block blk of 256 particles
+// one thread per particle: block blk of 256 particles.  The rebuild
+// request (stop, loaded by the caller; INT_MAX, "none due", when it
+// checked it itself)
+// is tested once the particle's own records are in flight, so its round
+// trip overlaps theirs.
 template <bool WET, bool MIXED>
-DEV void step_loop_block(const DevInst &P, const StepArgs &A, unsigned blk) {
+DEV void step_loop_block(const DevInst &P, const StepArgs &A, unsigned blk, int stop = INT_MAX) {
     const int i = (int)(blk * 256u + threadIdx.x);
     bool over = false;
+    double4 pi = make_double4(0.0, 0.0, 0.0, 0.0), va = pi, vb = pi;
     if (i < A.total) {
-        const double4 pi = A.in.p[i];
-        const double4 va = A.in.v[2 * i], vb = A.in.v[2 * i + 1];
+        pi = A.in.p[i];
+        va = A.in.v[2 * i];
+        vb = A.in.v[2 * i + 1];
+    }
+    if (__builtin_amdgcn_readfirstlane(stop) <= A.step) return;
+    if (i < A.total) {
         const V3 iv = v3(va.x, va.y, va.z), iw = v3(va.w, vb.x, vb.y);
         V3 F = v3(0.0, 0.0, 0.0), M = v3(0.0, 0.0, 0.0);
         double Mr = 0.0;
@@ -1441,9 +1450,9 @@ DEV void step_loop_block(const DevInst &P, const StepArgs &A, unsigned blk) {
 template <bool WET, bool MIXED>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
 k_step_fused(DevInst P, StepArgs A) {
-    // a rebuild is due before this step
-    if (__hip_atomic_load(A.stop_at, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <= A.step) return;
-    step_loop_block<WET, MIXED>(P, A, xcd_block(gridDim.x));
+    // a rebuild is due before this step when stop <= A.step
+    const int stop = __hip_atomic_load(A.stop_at, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    step_loop_block<WET, MIXED>(P, A, xcd_block(gridDim.x), stop);
 }
 
 void launch_step_fused(const DevInst &P, const StepArgs &A, bool wet, bool mixed, hipStream_t s) {
@@ -1811,7 +1820,7 @@ DEV WpEntry wave_entry_pair(unsigned t, unsigned T, unsigned off, unsigned mlo, 
 // same bit for bit.
 template <bool MIXED, bool PAIR, int BS = WV_BS>
 DEV void step_wave_block(const DevInst &P, const StepArgs &A, unsigned blk, double (*res)[64],
-                         WvStash *st, int lane_lo = 0, int lane_hi = 64) {
+                         WvStash *st, int lane_lo = 0, int lane_hi = 64, int stop = INT_MAX) {
     const int tid = threadIdx.x, lane = tid & 63;
     const int i = (int)(blk * BS) + tid;
     const int wbase = i - lane;
@@ -1829,25 +1838,38 @@ DEV void step_wave_block(const DevInst &P, const StepArgs &A, unsigned blk, doub
     unsigned wmask = 0u;
     int aoff_own = 0;
     double grav = 0.0;
+    // The wave's first loads and the near-slot gathers that depend on them
+    // are issued at raised priority, so a new wave reaches its contact
+    // rounds sooner while older waves compute.  The rebuild request (stop,
+    // loaded by the caller; INT_MAX, "none due", when it checked it itself) is tested
+    // only once the particle's own records are in flight, so its round
+    // trip overlaps theirs; nothing has been written before the test.
+    double4 pi = make_double4(0.0, 0.0, 0.0, 0.0), vai = pi, vbi = pi;
+    int nw = 0, nend = 0;
+    __builtin_amdgcn_s_setprio(2);
     if (own) {
         if constexpr (PAIR) {
             aoff_own = A.noff[i];
+            nend = A.noff[i + 1];
         } else {
             am_own = A.amask[i];
             aoff_own = A.actoff[i];
         }
-        const double4 pi = A.in.p[i];
-        // the wave's first loads and the near-slot gathers that depend on
-        // them are issued at raised priority, so a new wave reaches its
-        // contact rounds sooner while older waves compute
-        __builtin_amdgcn_s_setprio(2);
-        const int nw = A.nwl[i];
-        const double4 vbi = A.in.v[2 * i + 1];
+        pi = A.in.p[i];
+        nw = A.nwl[i];
+        vbi = A.in.v[2 * i + 1];
+        vai = A.in.v[2 * i];
+    }
+    if (__builtin_amdgcn_readfirstlane(stop) <= A.step) {
+        __builtin_amdgcn_s_setprio(0);
+        return;
+    }
+    if (own) {
         st->p[lane] = pi;
-        st->va[lane] = A.in.v[2 * i];
+        st->va[lane] = vai;
         st->vb[lane] = vbi;
         if constexpr (PAIR) {
-            const int nn = A.noff[i + 1] - aoff_own;
+            const int nn = nend - aoff_own;
             for (int r0 = 0; r0 < nn; r0 += WV_NB) {
                 int jj[WV_NB];
 #pragma unroll
@@ -2096,7 +2118,9 @@ DEV void step_wave_block(const DevInst &P, const StepArgs &A, unsigned blk, doub
 // scheduled better without the hint.
 template <bool MIXED, bool PAIR>
 DEV void step_wave_one(const DevInst &P, const StepArgs &A) {
-    if (__hip_atomic_load(A.stop_at, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <= A.step) return;
+    // a rebuild is due before this step when stop <= A.step (tested in
+    // step_wave_block once the first loads are issued)
+    const int stop = __hip_atomic_load(A.stop_at, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     // blocks are dispatched round-robin over the 8 XCDs: block b runs on
     // XCD b % 8 and takes the (b / 8)-th chunk of that XCD's range.  Odd
     // steps walk each range backwards, so a launch starts where the one
@@ -2127,7 +2151,7 @@ DEV void step_wave_one(const DevInst &P, const StepArgs &A) {
     const unsigned c = (A.step & 1) ? A.xb[x + 1] - 1u - pos : A.xb[x] + pos;
     __shared__ double res[7][64];
     __shared__ WvStash stash;
-    step_wave_block<MIXED, PAIR, 64>(P, A, c, res, &stash, lo, hi);
+    step_wave_block<MIXED, PAIR, 64>(P, A, c, res, &stash, lo, hi, stop);
 }
 
 template <bool PAIR>
