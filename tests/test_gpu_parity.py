@@ -109,6 +109,29 @@ def test_radix_sort_edge_sizes():
         e.close()
 
 
+@pytest.mark.parametrize("n", [0, 1, 2, 63, 64, 65, 129, 700])
+def test_ragged_sizes_step_bit_exact(n):
+    # an empty system, a lone sphere and counts around the 64-particle wave
+    # (a last wave with one particle, an exact number of waves, a few
+    # waves per XCD): 300 steps of the default step kernel with fast random
+    # velocities (several rebuilds requested on the device) equal the
+    # oracle bit for bit
+    w = W.small_bed(n=max(n, 2))
+    pos = w["pos"][:n]
+    vel = np.random.default_rng(n).normal(size=(n, 3)) * 50.0
+    e = G.Engine(w["inp"])
+    o = O.Oracle(w["inp"])
+    e.set_particles(pos, vel)
+    o.set_particles(pos, vel)
+    e.step(300)
+    o.step(300)
+    assert e.iter == o.iter == 300 and e.total == n
+    assert e.n_rebuild == o.n_rebuild
+    for u, v in zip(e.particles(), o.particles()):
+        assert np.array_equal(u, v)
+    e.close()
+
+
 def one_step_forces(w, vel=None, angv=None, precision=G.PREC_FP64):
     e, o = pair(w, precision, vel, angv)
     e.store_forces(True)
