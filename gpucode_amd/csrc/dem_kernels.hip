@@ -2187,12 +2187,14 @@ void launch_step_wave(const DevInst &P, const StepArgs &A, bool mixed, bool pair
 // contiguous and run on one XCD.  The instance's constants (DevInst) and
 // lists come from per-instance tables in device memory instead of kernel
 // arguments; each instance keeps its own rebuild flag and step limit, so
-// every instance follows exactly the steps it would take on its own.
+// every instance follows exactly the steps it would take on its own.  The
+// instance's rebuild request comes back in stop, for the step block to test
+// once its first loads are in flight.
 // ---------------------------------------------------------------------
-DEV bool batch_step_args(const BatchInst &B, int step, StepArgs &A) {
+DEV bool batch_step_args(const BatchInst &B, int step, StepArgs &A, int &stop) {
     if (step >= B.limit) return false;
     A = B.A;
-    if (__hip_atomic_load(A.stop_at, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <= step) return false;
+    stop = __hip_atomic_load(A.stop_at, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     const int b = (B.cur + step) & 1, l = (B.Lidx + step) & 1;
     A.in.p = B.P4[b]; A.in.v = B.VW[b]; A.in.lqd = B.lq[l];
     A.out.p = B.P4[1 - b]; A.out.v = B.VW[1 - b]; A.out.lqd = B.lq[1 - l];
@@ -2208,11 +2210,12 @@ __global__ void __launch_bounds__(WV_BS) k_step_batch_wave(const DevInst *insts,
                                                            const int2 *tasks, int step) {
     const int2 tk = tasks[xcd_block(gridDim.x)];
     StepArgs A;
-    if (!batch_step_args(tab[tk.x], step, A)) return;
+    int stop;
+    if (!batch_step_args(tab[tk.x], step, A, stop)) return;
     __shared__ double res_all[WV_BS / 64][7][64];
     __shared__ WvStash stash[WV_BS / 64];
     step_wave_block<MIXED, false>(insts[tk.x], A, (unsigned)tk.y, res_all[threadIdx.x >> 6],
-                           &stash[threadIdx.x >> 6]);
+                                  &stash[threadIdx.x >> 6], 0, 64, stop);
 }
 
 template <bool MIXED>
@@ -2220,12 +2223,13 @@ __global__ void __launch_bounds__(256) k_step_batch_loop(const DevInst *insts, c
                                                          const int2 *tasks, int step) {
     const int2 tk = tasks[xcd_block(gridDim.x)];
     StepArgs A;
-    if (!batch_step_args(tab[tk.x], step, A)) return;
+    int stop;
+    if (!batch_step_args(tab[tk.x], step, A, stop)) return;
     const DevInst &P = insts[tk.x];
     if (P.forceCap == 1)
-        step_loop_block<true, MIXED>(P, A, (unsigned)tk.y);
+        step_loop_block<true, MIXED>(P, A, (unsigned)tk.y, stop);
     else
-        step_loop_block<false, MIXED>(P, A, (unsigned)tk.y);
+        step_loop_block<false, MIXED>(P, A, (unsigned)tk.y, stop);
 }
 
 void launch_step_batch(const DevInst *insts, const BatchInst *tab, const int2 *tasks, int ntasks,
