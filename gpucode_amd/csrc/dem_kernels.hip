@@ -1447,11 +1447,11 @@ DEV void step_loop_block(const DevInst &P, const StepArgs &A, unsigned blk, int 
     request_rebuild(A, over);
 }
 
-// Held to 4 waves per SIMD (128 registers, a few spilled in the contact
-// and bridge paths): the kernel is bound by each thread's chain of
-// dependent gathers, and a fourth wave hides more of it than the spills
-// cost (a dense wet bed of 200,000 spheres: 9 % faster in FP64, 15 % in
-// mixed precision).
+// Held to 4 waves per SIMD (128 registers; no spill code is emitted): the
+// kernel is bound by each thread's chain of dependent gathers, and a
+// fourth wave hides more of it than a third (a dense wet bed of 200,000
+// spheres: 9 % faster in FP64, 15 % in mixed precision, measured when the
+// hint still cost a few spilled registers).
 template <bool WET, bool MIXED>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
 k_step_fused(DevInst P, StepArgs A) {
