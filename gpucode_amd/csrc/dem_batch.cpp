@@ -35,6 +35,7 @@ struct dem_batch {
     double *d_out = nullptr, *h_out = nullptr;     // analysis results, h_out pinned
     int *d_stop = nullptr, *h_stop = nullptr;      // h_stop pinned
     int *h_ovf = nullptr;                          // pinned
+    int *d_ovf = nullptr;                          // overflow flags of a batched rebuild's entries
     int2 *d_tasks = nullptr;
     int ntask_wave = 0, ntask_loop = 0, task_cap = 0;
     std::vector<int> task_totals;                  // totals the task table was built for
@@ -55,6 +56,7 @@ static void batch_free(dem_batch *b) {
     (void)hipFree(b->d_ana);
     (void)hipFree(b->d_out);
     (void)hipFree(b->d_stop);
+    (void)hipFree(b->d_ovf);
     (void)hipFree(b->d_tasks);
     (void)hipFree(b->d_rb);
     (void)hipFree(b->d_rtask);
@@ -95,6 +97,7 @@ extern "C" int dem_batch_create(int n, const dem_input *in, const double *const 
         if (e == hipSuccess) e = hipMalloc((void **)&b->d_ana, un * sizeof(AnaArgs));
         if (e == hipSuccess) e = hipMalloc((void **)&b->d_out, un * 3 * sizeof(double));
         if (e == hipSuccess) e = hipMalloc((void **)&b->d_stop, un * sizeof(int));
+        if (e == hipSuccess) e = hipMalloc((void **)&b->d_ovf, un * sizeof(int));
         if (e == hipSuccess) e = hipHostMalloc((void **)&b->h_tab, un * sizeof(BatchInst), 0);
         if (e == hipSuccess) e = hipHostMalloc((void **)&b->h_ana, un * sizeof(AnaArgs), 0);
         if (e == hipSuccess) e = hipHostMalloc((void **)&b->h_out, un * 3 * sizeof(double), 0);
@@ -221,28 +224,32 @@ static void rebuild_entry(dem_ctx *c, int k, RbInst &R) {
 
 // rebuilds every instance in need among those with steps left (the small
 // ones in one batched rebuild, the large ones one by one), then checks
-// their overflow flags with one synchronisation
+// their overflow flags with one synchronisation.  The entries of the
+// batched rebuild flag overflow in d_ovf[entry], read back by one copy
+// (not one per instance); the others use their own flag.
 static int batch_rebuilds(dem_batch *b, const std::vector<int> &left) {
     const int n = (int)b->inst.size();
-    std::vector<int> done;
+    std::vector<int> done, single;
     std::vector<int2> pt, ct;
     int nr = 0;
     HIPCHK(hipEventRecord(b->ev0, b->stream));
     for (int k = 0; k < n; ++k) {
         dem_ctx *c = b->inst[k];
         if (left[k] <= 0 || !c->rebuild_pending) continue;
-        done.push_back(k);
         if (c->total > kBatchRebuildMax || c->total < 1) {
             int rc = dem_do_rebuild(c);
             if (rc) return rc;
+            single.push_back(k);
             continue;
         }
+        done.push_back(k);
         rebuild_entry(c, k, b->h_rb[nr]);
+        b->h_rb[nr].overflow = b->d_ovf + nr;
         for (int q = 0; q < (c->total + 255) / 256; ++q) pt.push_back(make_int2(nr, q));
         for (int q = 0; q < (c->S.sysNCells + 255) / 256; ++q) ct.push_back(make_int2(nr, q));
         ++nr;
     }
-    if (done.empty()) return DEM_OK;
+    if (done.empty() && single.empty()) return DEM_OK;
     if (nr > 0) {
         const long need = (long)(pt.size() + ct.size());
         if (need > b->rtask_cap) {
@@ -263,20 +270,27 @@ static int batch_rebuilds(dem_batch *b, const std::vector<int> &left) {
         HIPCHK(hipGetLastError());
     }
     HIPCHK(hipEventRecord(b->ev1, b->stream));
-    for (size_t q = 0; q < done.size(); ++q)
-        HIPCHK(hipMemcpyAsync(&b->h_ovf[q], b->inst[done[q]]->overflow, sizeof(int),
+    // flags: h_ovf[0, nr) the batched entries (done[q] is entry q), then one
+    // per instance rebuilt on its own
+    if (nr > 0)
+        HIPCHK(hipMemcpyAsync(b->h_ovf, b->d_ovf, (size_t)nr * sizeof(int), hipMemcpyDeviceToHost,
+                              b->stream));
+    for (size_t q = 0; q < single.size(); ++q)
+        HIPCHK(hipMemcpyAsync(&b->h_ovf[nr + q], b->inst[single[q]]->overflow, sizeof(int),
                               hipMemcpyDeviceToHost, b->stream));
     // the host vectors copied above must outlive the copies: synchronise
     HIPCHK(hipStreamSynchronize(b->stream));
     float ms = 0.f;
     HIPCHK(hipEventElapsedTime(&ms, b->ev0, b->ev1));
     b->rebuild_ms += ms;
-    b->n_rebuild += (long long)done.size();
-    for (size_t q = 0; q < done.size(); ++q)
+    b->n_rebuild += (long long)(done.size() + single.size());
+    for (size_t q = 0; q < done.size() + single.size(); ++q) {
+        const int k = q < done.size() ? done[q] : single[q - done.size()];
         if (b->h_ovf[q] > 0)
             return dem_fail(DEM_ERR_OVERFLOW,
                             "instance %d: a neighbour list needed %d slots, more than its size level "
-                            "allows", done[q], b->h_ovf[q]);
+                            "allows", k, b->h_ovf[q]);
+    }
     return DEM_OK;
 }
 

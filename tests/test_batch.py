@@ -151,6 +151,24 @@ def test_batch_equals_separate_runs(precision):
 
 
 @pytest.mark.gpu
+def test_batch_reports_the_overflowing_instance():
+    # three instances rebuilt by one batched rebuild; the middle one has 60
+    # spheres crammed together, more neighbours than its lists hold: the
+    # step fails with the overflow status and names that instance, and
+    # nothing is written past the lists
+    ws = [W.small_bed(n=200, seed=s) for s in (3, 4, 5)]
+    pos = [w["pos"].copy() for w in ws]
+    pos[1][:60] = pos[1][0] + np.random.default_rng(0).normal(size=(60, 3)) * 0.01
+    b = G.Batch([w["inp"] for w in ws])
+    for k in range(3):
+        b[k].set_particles(pos[k])
+    with pytest.raises(G.DemError) as err:
+        b.step(1)
+    assert err.value.code == -3 and "instance 1:" in str(err.value)
+    b.close()
+
+
+@pytest.mark.gpu
 def test_analysis_matches_cpu_reference():
     # a bed settled for a while: contacts, wall contacts and a free top
     w = W.small_bed(n=3000, seed=9)
