@@ -191,3 +191,14 @@ def test_calibration_round_trip():
     d = restitution_to_dmpn(0.5, 1e-3, 2500, 70e9, 0.25, v=0.5)
     assert collision_restitution(d, 1e-3, 2500, 70e9, 0.25, 0.5) == pytest.approx(0.5, abs=1e-4)
     assert collision_restitution(0.0, 1e-3, 2500, 70e9, 0.25, 0.5) == pytest.approx(1.0, abs=1e-3)
+
+
+def test_bench_refuses_more_than_one_gpu():
+    # the engine steps one packing on one GPU: asked for more, the bench
+    # exits with status 2 and says so before touching a device
+    import subprocess
+    import sys
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 2 and r.stdout == ""
+    assert "one GPU only" in r.stderr
