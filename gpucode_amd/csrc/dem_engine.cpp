@@ -3,9 +3,9 @@
 // The reference's time loop (main.cpp:425-623) syncs with the device on
 // every step to read the rebuild flag (main.cpp:470).  dem_step launches a
 // chunk of fused steps back to back instead; a step kernel that finds a
-// rebuild request from an earlier step of the chunk exits at once, and the
-// host rebuilds and re-runs the skipped steps after reading the request
-// once per chunk.
+// rebuild request from an earlier step of the chunk exits before it writes
+// anything, and the host rebuilds and re-runs the skipped steps after
+// reading the request once per chunk.
 #include <limits.h>
 #include <string.h>
 
