@@ -17,9 +17,14 @@
  *                         ratio at |xi| = xi_max/2; damping linear in v_n;
  *                         p-w law equals the p-p law with R* = r_i and the
  *                         wall constants.
- *   orc_fcap              Willett (2000) theta = 0 regression evaluated at
- *                         independent points; theta > 0 is checked only for
- *                         continuity at theta -> 0 (parity unpinned beyond).
+ *   orc_fcap              against Young-Laplace solutions of the pendular
+ *                         bridge (tests/golden/capillary_yl.py, pinned by
+ *                         virtual work and the 2 pi R gamma cos(theta)
+ *                         contact limit): within the regression's accuracy
+ *                         for theta <= 40 degrees, tight enough that a 5 %
+ *                         error in any of the 14 theta = 0 coefficients or
+ *                         the theta terms fails (tests/test_capillary_pin.py).
+ *   orc_rupture           both rules against the Young-Laplace rupture gap.
  *   orc_calc_hash/sort/cell_range  integer closed forms, a stable sort and
  *                         brute-force cell occupancy counts.
  *   orc_neighbor_find     brute-force O(N^2) pair search (set equality for
@@ -516,35 +521,45 @@ void orc_contact_pw(const orc_setup *S, orc_hist1 *h, double nrmDsp,
     *Mr = 0.5 * (S->rfrc[0] + S->rfrc[0]) * fN * iRad;
 }
 
-/* Willett et al. (2000) regression for the bridge force between two spheres.
- * theta = 0 gives exactly the coefficients of kernel.cu:73-76; the theta
- * terms are the published second-order fit in the contact angle (radians). */
-double orc_fcap(const orc_setup *S, double Splus, double ijVol, double effectR) {
+/* Willett et al. (2000) regression for the bridge force between two spheres,
+ * F = 2 pi R gamma exp(f1 - f2 exp(f3 ln S+ + f4 ln^2 S+)) with every
+ * coefficient of f1..f4 a quadratic in the contact angle (radians):
+ * a = c0 + theta (c1 + theta c2).  Rows: f1 (ln V)^0..3, f2 (ln V)^0..2,
+ * f3 (ln V)^0..3, f4 (ln V)^0..2.  Column c0 (theta = 0) is exactly the
+ * coefficients of kernel.cu:73-76; c1 and c2 are the published fit in the
+ * contact angle. */
+const double ORC_WILLETT[14][3] = {
+    {-0.44507, 0.050832, -1.1466},       {-0.1119, -0.000411, -0.1490},
+    {-0.012101, -0.0036456, -0.01255},   {-0.0005, -0.0003505, -0.00029076},
+    {1.9222, -0.57473, -1.2918},         {-0.0668, -0.1201, -0.22574},
+    {-0.0013375, -0.0068988, -0.01137},
+    {1.268, -0.01396, -0.23566},         {0.198, 0.092, -0.06418},
+    {0.02232, 0.02238, -0.009853},       {0.0008585, 0.001318, -0.00053},
+    {-0.010703, 0.073776, -0.34742},     {0.03345, 0.04543, -0.09056},
+    {0.0018574, 0.004456, -0.006257},
+};
+
+/* fCap (kernel.cu:65-81) with the coefficient table as an argument, so that
+ * tests can show which coefficient errors the Young-Laplace check detects. */
+double orc_fcap_coef(const double coef[14][3], const orc_setup *S, double Splus, double ijVol,
+                     double effectR) {
     const double th = S->theta;
+    double a[14];
+    for (int k = 0; k < 14; ++k) a[k] = coef[k][0] + th * (coef[k][1] + th * coef[k][2]);
     Splus = fmax(S->gapmn, Splus);
     Splus = Splus / sqrt(ijVol);
     double LV = orc_log(ijVol);
-    double a10 = -0.44507 + th * (0.050832 + th * (-1.1466));
-    double a11 = -0.1119 + th * (-0.000411 + th * (-0.1490));
-    double a12 = -0.012101 + th * (-0.0036456 + th * (-0.01255));
-    double a13 = -0.0005 + th * (-0.0003505 + th * (-0.00029076));
-    double a20 = 1.9222 + th * (-0.57473 + th * (-1.2918));
-    double a21 = -0.0668 + th * (-0.1201 + th * (-0.22574));
-    double a22 = -0.0013375 + th * (-0.0068988 + th * (-0.01137));
-    double a30 = 1.268 + th * (-0.01396 + th * (-0.23566));
-    double a31 = 0.198 + th * (0.092 + th * (-0.06418));
-    double a32 = 0.02232 + th * (0.02238 + th * (-0.009853));
-    double a33 = 0.0008585 + th * (0.001318 + th * (-0.00053));
-    double a40 = -0.010703 + th * (0.073776 + th * (-0.34742));
-    double a41 = 0.03345 + th * (0.04543 + th * (-0.09056));
-    double a42 = 0.0018574 + th * (0.004456 + th * (-0.006257));
     double L2 = LV * LV, L3 = L2 * LV;
-    double f1 = a10 + a11 * LV + a12 * L2 + a13 * L3;
-    double f2 = a20 + a21 * LV + a22 * L2;
-    double f3 = a30 + a31 * LV + a32 * L2 + a33 * L3;
-    double f4 = a40 + a41 * LV + a42 * L2;
+    double f1 = a[0] + a[1] * LV + a[2] * L2 + a[3] * L3;
+    double f2 = a[4] + a[5] * LV + a[6] * L2;
+    double f3 = a[7] + a[8] * LV + a[9] * L2 + a[10] * L3;
+    double f4 = a[11] + a[12] * LV + a[13] * L2;
     double lS = orc_log(Splus);
     return 2 * ORC_PI * effectR * S->stension * orc_exp(f1 - f2 * orc_exp(f3 * lS + f4 * (lS * lS)));
+}
+
+double orc_fcap(const orc_setup *S, double Splus, double ijVol, double effectR) {
+    return orc_fcap_coef(ORC_WILLETT, S, Splus, ijVol, effectR);
 }
 
 /* bridge rupture distance: kernel.cu:522, or Lian et al. (1993)

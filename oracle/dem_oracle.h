@@ -127,6 +127,10 @@ void orc_contact_pw(const orc_setup *S, orc_hist1 *h, double nrmDsp,
                     const double iAngv[3], const double jVel[3], int stage,
                     double F[3], double M[3], double *Mr);
 double orc_fcap(const orc_setup *S, double Splus, double ijVol, double effectR);
+/* Willett's coefficients, [f1 0..3, f2 0..2, f3 0..3, f4 0..2][theta^0..2] */
+extern const double ORC_WILLETT[14][3];
+double orc_fcap_coef(const double coef[14][3], const orc_setup *S, double Splus,
+                     double ijVol, double effectR);
 /* rounding-reproducible exp / log / x^y (x > 0) used for the reference's
  * libm calls; see dem_oracle.c */
 double orc_exp(double x);

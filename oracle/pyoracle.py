@@ -74,6 +74,8 @@ _SIGS = {
     "orc_contact_pw": (None, [C.POINTER(OrcSetup), C.POINTER(OrcHist1), C.c_double, D,
                               C.c_double, D, D, D, C.c_int, D, D, D]),
     "orc_fcap": (C.c_double, [C.POINTER(OrcSetup), C.c_double, C.c_double, C.c_double]),
+    "orc_fcap_coef": (C.c_double, [D, C.POINTER(OrcSetup), C.c_double, C.c_double,
+                                   C.c_double]),
     "orc_rupture": (C.c_double, [C.POINTER(OrcSetup), C.c_double]),
     "orc_exp": (C.c_double, [C.c_double]),
     "orc_log": (C.c_double, [C.c_double]),
