@@ -204,6 +204,13 @@ int dem_create_on(const dem_input *in, const double *dia_m, int precision, hipSt
     if (precision != DEM_PREC_FP64 && precision != DEM_PREC_MIXED)
         return dem_fail(DEM_ERR_ARG, "unknown precision %d", precision);
     if (in->NP < 1) return dem_fail(DEM_ERR_ARG, "NP must be positive");
+    // kernel.cu:627-629 clamps cell coordinates to gridSize, one past the
+    // last cell, so a particle outside the box hashes into another cell's
+    // slot or past the cell table.  That behaviour is not reproduced: cells
+    // clamp to gridSize - 1 (the same cell for every particle inside the box)
+    if (in->gridClampQuirk)
+        return dem_fail(DEM_ERR_ARG, "gridClampQuirk is not supported: cell coordinates clamp to "
+                        "gridSize - 1");
     *out = nullptr;
     dem_ctx *c = new dem_ctx();
     c->in = *in;

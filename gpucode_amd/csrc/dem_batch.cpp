@@ -230,7 +230,7 @@ static void rebuild_entry(dem_ctx *c, int k, RbInst &R) {
 static int batch_rebuilds(dem_batch *b, const std::vector<int> &left) {
     const int n = (int)b->inst.size();
     std::vector<int> done, single;
-    std::vector<int2> pt, ct;
+    std::vector<int2> pt, ct, all;  // all: the task list copied to the device
     int nr = 0;
     HIPCHK(hipEventRecord(b->ev0, b->stream));
     for (int k = 0; k < n; ++k) {
@@ -259,7 +259,7 @@ static int batch_rebuilds(dem_batch *b, const std::vector<int> &left) {
             HIPCHK(hipMalloc((void **)&b->d_rtask, (size_t)need * 2 * sizeof(int2)));
             b->rtask_cap = need * 2;
         }
-        std::vector<int2> all(pt);
+        all = pt;
         all.insert(all.end(), ct.begin(), ct.end());
         HIPCHK(hipMemcpyAsync(b->d_rb, b->h_rb, (size_t)nr * sizeof(RbInst), hipMemcpyHostToDevice,
                               b->stream));
@@ -284,13 +284,16 @@ static int batch_rebuilds(dem_batch *b, const std::vector<int> &left) {
     HIPCHK(hipEventElapsedTime(&ms, b->ev0, b->ev1));
     b->rebuild_ms += ms;
     b->n_rebuild += (long long)(done.size() + single.size());
+    int first = -1;
     for (size_t q = 0; q < done.size() + single.size(); ++q) {
         const int k = q < done.size() ? done[q] : single[q - done.size()];
-        if (b->h_ovf[q] > 0)
-            return dem_fail(DEM_ERR_OVERFLOW,
-                            "instance %d: a neighbour list needed %d slots, more than its size level "
-                            "allows", k, b->h_ovf[q]);
+        b->inst[k]->lists_overflow = b->h_ovf[q] > 0 ? b->h_ovf[q] : 0;
+        if (b->h_ovf[q] > 0 && first < 0) first = k;
     }
+    if (first >= 0)
+        return dem_fail(DEM_ERR_OVERFLOW,
+                        "instance %d: a neighbour list needed %d slots, more than its size level "
+                        "allows", first, b->inst[first]->lists_overflow);
     return DEM_OK;
 }
 
@@ -298,8 +301,12 @@ extern "C" int dem_batch_step(dem_batch *b, int nsteps) {
     if (!b || nsteps < 0) return dem_fail(DEM_ERR_ARG, "bad argument");
     const int n = (int)b->inst.size();
     int rc;
-    for (dem_ctx *c : b->inst) {
+    for (int k = 0; k < n; ++k) {
+        dem_ctx *c = b->inst[k];
         if (c->precision != b->precision) return dem_fail(DEM_ERR_STATE, "mixed precisions in batch");
+        if (!c->rebuild_pending && c->lists_overflow > 0)
+            return dem_fail(DEM_ERR_OVERFLOW, "instance %d: a neighbour list needed %d slots, more "
+                            "than its size level allows", k, c->lists_overflow);
         // the batched kernels step the directed records
         if ((rc = dem_pair_unpack(c))) return rc;
     }
@@ -405,6 +412,7 @@ static int analyze(const std::vector<dem_ctx *> &cs, hipStream_t st, DevInst *d_
         int ov = 0;
         HIPCHK(hipMemcpyAsync(&ov, cs[k]->overflow, sizeof(int), hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
+        cs[k]->lists_overflow = ov > 0 ? ov : 0;
         if (ov > 0)
             return dem_fail(DEM_ERR_OVERFLOW, "instance %d: a neighbour list needed %d slots", k, ov);
     }

@@ -53,9 +53,19 @@ static int check_overflow(dem_ctx *c) {
     int ov = 0;
     HIPCHK(hipMemcpyAsync(&ov, c->overflow, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
-    if (ov > 0)
+    c->lists_overflow = ov > 0 ? ov : 0;
+    return dem_lists_ok(c);
+}
+
+// Refuses to step on lists that a rebuild truncated (kernel.cu:833-836
+// writes past the list instead); the state stays refused until it is
+// replaced (dem_set_particles, dem_set_contacts, dem_dump_read) or a
+// rebuild fits.
+int dem_lists_ok(dem_ctx *c) {
+    if (c->lists_overflow > 0)
         return dem_fail(DEM_ERR_OVERFLOW,
-                        "a neighbour list needed %d slots, more than its size level allows", ov);
+                        "a neighbour list needed %d slots, more than its size level allows",
+                        c->lists_overflow);
     return DEM_OK;
 }
 
@@ -72,6 +82,7 @@ static void reset_lists(dem_ctx *c) {
     c->old_total = 0;
     c->rebuild_pending = true;
     c->forces_valid = false;
+    c->lists_overflow = 0;
 }
 
 static int upload_particles(dem_ctx *c, int off, int n, const double *pos, const double *vel,
@@ -723,8 +734,8 @@ extern "C" int dem_var_init(dem_ctx *c) {
 
 extern "C" int dem_pp_inter_calc(dem_ctx *c) {
     if (!c) return dem_fail(DEM_ERR_ARG, "null context");
-    int rc = dem_pair_unpack(c);
-    if (rc) return rc;
+    int rc = dem_lists_ok(c);
+    if (rc || (rc = dem_pair_unpack(c))) return rc;
     StepArgs A = dem_step_args(c, false);
     launch_pp_force(c->P, A, c->S.forceCap == 1, c->precision == DEM_PREC_MIXED, c->stream);
     HIPCHK(hipGetLastError());
@@ -762,6 +773,7 @@ extern "C" int dem_step_unfused(dem_ctx *c) {
     if (!c) return dem_fail(DEM_ERR_ARG, "null context");
     int rc;
     if (c->rebuild_pending && (rc = dem_rebuild(c))) return rc;
+    if ((rc = dem_lists_ok(c))) return rc;
     int flag = 0;
     if ((rc = dem_var_init(c)) || (rc = dem_pp_inter_calc(c)) || (rc = dem_cir_pw_calc(c)) ||
         (rc = dem_update_system(c, &flag)))
@@ -882,6 +894,7 @@ int dem_finish_chunk(dem_ctx *c, int nsteps, int stop) {
 extern "C" int dem_step(dem_ctx *c, int nsteps) {
     if (!c || nsteps < 0) return dem_fail(DEM_ERR_ARG, "bad argument");
     int done = 0, rc;
+    if (!c->rebuild_pending && (rc = dem_lists_ok(c))) return rc;
     while (done < nsteps) {
         if (c->rebuild_pending) {
             HIPCHK(hipEventRecord(c->ev0, c->stream));
@@ -914,6 +927,7 @@ extern "C" int dem_time_steps(dem_ctx *c, int nsteps, double *ms_per_step, int *
     if (!c || nsteps <= 0) return dem_fail(DEM_ERR_ARG, "bad argument");
     int rc;
     if (c->rebuild_pending && (rc = dem_rebuild(c))) return rc;
+    if ((rc = dem_lists_ok(c))) return rc;
     double ms0 = c->fused_ms;
     long long s0 = c->fused_steps, l0 = c->fused_launches;
     int ex = 0;

@@ -94,6 +94,10 @@ struct dem_ctx {
     int rebuild_interval = -1;    // steps between the last two rebuilds (-1: not known)
     double ctime = 0.0;
     bool rebuild_pending = true;
+    // > 0 after a rebuild whose lists overflowed (the slots a list needed):
+    // the lists are truncated, so stepping is refused until the state is
+    // replaced or a rebuild fits
+    int lists_overflow = 0;
     bool forces_valid = false;
 
     // timing of the fused path
@@ -117,6 +121,7 @@ void dem_fill_inst(const dem_setup &S, DevInst &P);
 void dem_refresh_inst(dem_ctx *c);
 // Internal pieces of the rebuild and step, shared by the C ABI functions.
 int dem_do_rebuild(dem_ctx *c);
+int dem_lists_ok(dem_ctx *c);
 // shared pair records: make them from the directed records, or write them
 // back (a no-op when they are not in use)
 int dem_pair_pack(dem_ctx *c);

@@ -74,9 +74,12 @@ typedef struct dem_input {
     double theta;           /* liquid contact angle [rad]; 0 = reference law */
     int ruptureMode;        /* 0: 0.99 V^0.33 (kernel.cu:522);
                                1: (1 + theta/2) V^(1/3) */
-    int gridClampQuirk;     /* accepted for deck compatibility; cells are
-                               always clamped to gridSize-1 (identical for
-                               every in-domain particle) */
+    int gridClampQuirk;     /* must be 0: dem_create refuses 1 with
+                               DEM_ERR_ARG.  kernel.cu:627-629 clamps cell
+                               coordinates to gridSize, which sends a particle
+                               outside the box past the cell table; here cells
+                               clamp to gridSize-1 (identical for every
+                               particle inside the box) */
 } dem_input;
 
 /* Derived quantities in reduced units (ReadData.cpp:243-375,

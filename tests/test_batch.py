@@ -165,6 +165,10 @@ def test_batch_reports_the_overflowing_instance():
     with pytest.raises(G.DemError) as err:
         b.step(1)
     assert err.value.code == -3 and "instance 1:" in str(err.value)
+    # a second step does not run on the truncated lists either
+    with pytest.raises(G.DemError) as err:
+        b.step(1)
+    assert err.value.code == -3 and "instance 1:" in str(err.value)
     b.close()
 
 

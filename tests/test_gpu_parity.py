@@ -387,6 +387,15 @@ def test_overflow_is_reported_not_written():
     with pytest.raises(G.DemError) as err:
         e.step(1)
     assert err.value.code == -3
+    # the truncated lists are never stepped on: later calls fail the same way
+    with pytest.raises(G.DemError) as err:
+        e.step(1)
+    assert err.value.code == -3
+    # until the state is replaced by one whose lists fit
+    e.set_particles(w["pos"])
+    it = e.iter
+    e.step(2)
+    assert e.iter == it + 2
 
 
 def test_dump_load_round_trip(tmp_path):
