@@ -3,16 +3,30 @@
 Default workload (the 2M-sphere throughput case): 2,000,000 monodisperse
 dry spheres, d = 0.3 mm, in a cylinder of radius 30 mm (100 d) with a
 floor, neighbour-list skin 0.1 d, mu = 0.5, rho 2500, E 70 GPa, nu 0.25,
-e 0.7, yield at 1 % d, FP64 state.  The bed is a seeded layer-by-layer
-pocket deposition (about 54 d tall); W warmup steps settle it, then K steps
-are timed, rebuilds included.  A "step" is one pass of the time step over
-all particles (forces, walls, integration; a neighbour rebuild when due).
+e 0.7, yield at 1 % d, FP64 state.  The seeded feed is a layer-by-layer
+pocket deposition (about 54 d tall); --settle steps (default 20,000) settle
+it, the settled state goes through a restart dump (dem_dump_write_binary)
+and a fresh context loads it (dem_dump_read), and from there W warmup steps
+run and K steps are timed, rebuilds included.  A "step" is one pass of the
+time step over all particles (forces, walls, integration; a neighbour
+rebuild when due).  After the timed region every rebuild stage is timed on
+the same state (dem_profile_rebuild).
+
+--workload pour: the same 2,000,000 spheres and material as a loose cloud
+(jittered lattice, spacing 1.5 d) falling at 2 m/s onto the floor of a
+cylinder 75 mm tall: the unsettled variant, whose lists are rebuilt every
+few hundred steps, timed with its rebuilds.
 
 --workload wet_poly: 200,000 wet polydisperse spheres (lognormal d, median
 0.5 mm, sigma 0.25, clipped 0.25-1.0 mm, three size levels; capillary
 bridges with liquid 1 % of the mean particle volume, theta 20 deg) poured
 from a seeded random feed into a cylinder of radius 25 mm; by default the
 whole 50,000-step pour is timed, from the fed state, with no warmup.
+
+--workload wet_bed: the same 200,000 lognormal wet spheres with soft
+spheres (E = 0.1 MPa), poured and settled for --settle steps (default
+30,000) before the timed region, so contacts, plastic contacts and
+capillary bridges all act while it is timed.
 
 Prints ONE JSON line on stdout.  Progress goes to stderr.
 """
@@ -141,6 +155,132 @@ def sweep_main(args):
     print(json.dumps(out), flush=True)
 
 
+REBUILD_BYTES = {
+    # algorithmic bytes per particle of each rebuild stage (n_nb neighbours,
+    # n_c contacts, c_per_p cells per particle, passes of the radix sort)
+    "save": lambda q: 0.0,
+    "hash": lambda q: 24.0 + 8.0,
+    "sort": lambda q: 16.0 * q["passes"],
+    "permute": lambda q: 2.0 * 108.0,
+    "cell_range": lambda q: 8.0 + 8.0 * q["c_per_p"],
+    "neighbor_find": lambda q: 32.0 + 4.0 * q["n_nb"],
+    "active_slots": lambda q: 12.0 + 32.0 * q["n_nb"],
+    "wall_list": lambda q: 32.0 + 16.0,
+    "hist_remap": lambda q: 128.0 * q["n_c"] + 8.0 * q["n_nb"],
+}
+
+
+def prepare(args, G, W):
+    """The workload's engine, ready for the timed region, and its labels."""
+    t0 = time.time()
+    prec = G.PREC_FP64 if args.precision == "fp64" else G.PREC_MIXED
+    kern = {"auto": G.Engine.KERNEL_AUTO, "wave": G.Engine.KERNEL_WAVE,
+            "compact": G.Engine.KERNEL_COMPACT, "loop": G.Engine.KERNEL_LOOP}[args.kernel]
+    info = {}
+    if args.workload in ("packed_bed", "pour"):
+        n_req = args.n or 2_000_000
+        nm = f"{n_req / 1e6:g}M"
+        if args.workload == "packed_bed":
+            w = W.packed_bed(n=n_req)
+            settle = 20000 if args.settle is None else args.settle
+            desc = (f"packed_bed_{nm} (dry, monodisperse d=0.3mm, R=30mm, skin 0.1d, mu 0.5; "
+                    f"pocket-deposited feed settled {settle} steps, restarted from its dump)")
+            tag = f"packed_bed_{nm}"
+        else:
+            w = W.pour_2M(n=n_req)
+            settle = 0
+            desc = (f"pour_{nm} (dry, monodisperse d=0.3mm, R=30mm, skin 0.1d, mu 0.5; a loose "
+                    f"lattice falling at 2 m/s onto the floor)")
+            tag = f"pour_{nm}"
+        metric = "particle-steps/s at 2M spheres"
+        steps = 10000 if args.steps is None else args.steps
+        warmup = 300 if args.warmup is None else args.warmup
+        nseg = 1 if args.workload == "packed_bed" else 10
+        seed = 0xC3
+    elif args.workload == "wet_poly":
+        n_req = args.n or 200_000
+        w = W.wet_poly(n=n_req)
+        settle = 0
+        steps = w["steps"] if args.steps is None else args.steps
+        warmup = 0 if args.warmup is None else args.warmup
+        desc = (f"wet_poly_{n_req // 1000}k (lognormal d 0.25-1.0mm, 3 size levels, liquid 1%, "
+                f"theta 20deg, R=25mm; the {steps}-step pour from the seeded feed)")
+        metric = f"particle-steps/s at {n_req // 1000}k wet polydisperse spheres"
+        tag = f"wet_poly_{n_req // 1000}k"
+        nseg = 10
+        seed = 0xC2
+    else:
+        n_req = args.n or 200_000
+        w = W.wet_poly(n=n_req, E=1e5)
+        settle = 30000 if args.settle is None else args.settle
+        steps = 5000 if args.steps is None else args.steps
+        warmup = 100 if args.warmup is None else args.warmup
+        desc = (f"wet_bed_{n_req // 1000}k (lognormal d 0.25-1.0mm, 3 size levels, liquid 1%, "
+                f"theta 20deg, R=25mm, soft spheres E=0.1MPa; the pour settled {settle} steps)")
+        metric = f"particle-steps/s at {n_req // 1000}k wet polydisperse spheres"
+        tag = f"wet_bed_{n_req // 1000}k"
+        nseg = 5
+        seed = 0xC2
+    log(f"{args.workload} of {n_req} spheres built in {time.time() - t0:.1f}s")
+    e = G.Engine(w["inp"], w["dia"], prec)
+    e.set_kernel(kern)
+    e.set_particles(w["pos"], w.get("vel"))
+    if settle:
+        t0 = time.time()
+        e.step(settle)
+        e.sync()
+        st = e.stats()
+        info["settle_steps"] = settle
+        info["settle_s"] = round(time.time() - t0, 2)
+        log(f"settled {settle} steps in {info['settle_s']:.1f}s, contacts/particle "
+            f"{st['n_contact'] / st['n_particles']:.2f}")
+        if args.workload == "packed_bed":
+            # restart through the dump, as a run resumed from its DumpLoad file
+            import tempfile
+            d = tempfile.mkdtemp(prefix="dem_bench_")
+            path = os.path.join(d, "heyi_dump.bin")
+            t0 = time.time()
+            e.dump(path, binary=True)
+            size = os.path.getsize(path)
+            t1 = time.time()
+            e.close()
+            e = G.Engine(w["inp"], w["dia"], prec)
+            e.set_kernel(kern)
+            e.load(path)
+            t2 = time.time()
+            os.remove(path)
+            os.rmdir(d)
+            info["dump_bytes"] = size
+            info["dump_write_s"] = round(t1 - t0, 2)
+            info["dump_read_s"] = round(t2 - t1, 2)
+            log(f"dump {size / 2**20:.0f} MiB written in {t1 - t0:.1f}s, read into a fresh "
+                f"context in {t2 - t1:.1f}s")
+    args.steps, args.warmup = steps, warmup
+    return dict(e=e, w=w, desc=desc, metric=metric, tag=tag, nseg=nseg, seed=seed, info=info,
+                prec=prec, settle=settle)
+
+
+def rebuild_profile(e, n, reps, interval):
+    """Rebuild stage times on the current state, their algorithmic bytes and
+    the rebuild cost per step at the measured interval."""
+    st = e.stats()
+    S = e.setup
+    q = {"n_nb": st["n_nebor"] / n, "n_c": st["n_contact"] / n,
+         "c_per_p": S.sysNCells / n, "passes": max(1, (e.key_bits() + 7) // 8)}
+    ms = e.profile_rebuild(reps)
+    stages = {}
+    for k, t in ms.items():
+        b = REBUILD_BYTES[k](q) * n
+        stages[k] = {"ms": round(t, 4), "bytes": round(b),
+                     "gbs": round(b / (t * 1e-3) / 1e9, 1) if t > 0 else None}
+    total = sum(ms.values())
+    out = {"ms": round(total, 4), "stages": stages, "reps": reps}
+    if interval:
+        out["interval_steps"] = round(interval, 1)
+        out["amortized_ms_per_step"] = round(total / interval, 5)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -151,9 +291,16 @@ def main():
                     help="untimed steps first (default 300 for packed_bed, 0 for wet_poly)")
     ap.add_argument("--n", type=int, default=None,
                     help="particles (default 2,000,000 for packed_bed, 200,000 for wet_poly)")
-    ap.add_argument("--workload", choices=["packed_bed", "wet_poly"], default="packed_bed",
-                    help="packed_bed: the 2M-sphere dry bed (the headline); wet_poly: the 200k "
-                         "polydisperse wet pour (capillary bridges, three size levels)")
+    ap.add_argument("--workload", choices=["packed_bed", "pour", "wet_poly", "wet_bed"],
+                    default="packed_bed",
+                    help="packed_bed: the settled 2M-sphere dry bed (the headline); pour: the "
+                         "same spheres falling (rebuilds); wet_poly: the 200k polydisperse wet "
+                         "pour; wet_bed: that pour settled (contacts and bridges)")
+    ap.add_argument("--settle", type=int, default=None,
+                    help="untimed settling steps before the restart dump (packed_bed, default "
+                         "20,000) or the timed region (wet_bed, default 30,000)")
+    ap.add_argument("--rebuild-reps", type=int, default=3,
+                    help="rebuilds of the final state timed stage by stage")
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
     ap.add_argument("--kernel", choices=["auto", "wave", "compact", "loop"], default="auto",
                     help="fused step kernel: per-wave contact compaction (auto, wave), "
@@ -181,35 +328,14 @@ def main():
     if args.sweep:
         return sweep_main(args)
 
-    t0 = time.time()
-    if args.workload == "packed_bed":
-        n_req = args.n or 2_000_000
-        w = W.packed_bed(n=n_req)
-        desc = f"packed_bed_{n_req // 1000000}M (dry, monodisperse d=0.3mm, R=30mm, skin 0.1d, mu 0.5)"
-        metric = "particle-steps/s at 2M spheres"
-        tag = f"packed_bed_{n_req // 1000000}M"
-        steps = 10000 if args.steps is None else args.steps
-        warmup = 300 if args.warmup is None else args.warmup
-        nseg = 1
-    else:
-        n_req = args.n or 200_000
-        w = W.wet_poly(n=n_req)
-        steps = w["steps"] if args.steps is None else args.steps
-        warmup = 0 if args.warmup is None else args.warmup
-        desc = (f"wet_poly_{n_req // 1000}k (lognormal d 0.25-1.0mm, 3 size levels, liquid 1%, "
-                f"theta 20deg, R=25mm; the {steps}-step pour from the seeded feed)")
-        metric = f"particle-steps/s at {n_req // 1000}k wet polydisperse spheres"
-        tag = f"wet_poly_{n_req // 1000}k"
-        nseg = 10
-    args.steps, args.warmup = steps, warmup
-    log(f"{args.workload} of {n_req} spheres built in {time.time() - t0:.1f}s")
-    prec = G.PREC_FP64 if args.precision == "fp64" else G.PREC_MIXED
-    e = G.Engine(w["inp"], w["dia"], prec)
-    e.set_kernel({"auto": G.Engine.KERNEL_AUTO, "wave": G.Engine.KERNEL_WAVE,
-                  "compact": G.Engine.KERNEL_COMPACT, "loop": G.Engine.KERNEL_LOOP}[args.kernel])
-    e.set_particles(w["pos"])
+    P = prepare(args, G, W)
+    e, w, desc, metric, tag, nseg = P["e"], P["w"], P["desc"], P["metric"], P["tag"], P["nseg"]
+    prec = P["prec"]
+    steps, warmup = args.steps, args.warmup
     n = e.total
     log(f"engine up: {e.device_bytes() / 2**30:.2f} GiB on device")
+    if P["settle"] and args.workload == "packed_bed" and e.iter == 0:
+        raise RuntimeError("the restart did not carry the clock")
     t0 = time.time()
     e.step(warmup)
     e.sync()
@@ -256,10 +382,16 @@ def main():
     # (a launch after a rebuild became due returns at once and moves nothing)
     achieved = bytes_alg / (fused_ms * 1e-3) / 1e9
     kname = e.step_kernel_name()
-    traffic = None
+    traffic = traffic_nc = None
     tr = load_traffic(kname, tag)
-    if tr:
+    if tr and tr.get("settle_steps", 0) == P["settle"] and tr.get("n_particles") == n:
         traffic = tr.get("bytes_per_launch")
+        traffic_nc = tr.get("n_contact")
+    interval = executed / (e.n_rebuild - rb0) if e.n_rebuild > rb0 else None
+    reb = rebuild_profile(e, n, args.rebuild_reps, interval) if args.rebuild_reps > 0 else None
+    if reb:
+        log(f"rebuild {reb['ms']:.3f} ms: " + ", ".join(
+            f"{k} {v['ms']:.3f}" for k, v in reb["stages"].items()))
 
     copy_gbs = G.engine.stream_copy(1 << 30, 20)
     log(f"plain copy {copy_gbs:.0f} GB/s; {kname} {kern_ms:.4f} ms, {achieved:.0f} GB/s "
@@ -267,11 +399,16 @@ def main():
 
     base = None
     if not args.no_cpu_baseline:
-        thr = min(16, os.cpu_count() or 1)
+        # every core the process may use, at most the CPU share of one GPU
+        # (16 host cores per GPU on these machines)
+        aff = len(os.sched_getaffinity(0))
+        thr = min(16, aff)
         t0 = time.time()
         base = cpu_baseline(e, w["inp"], thr, dia=w["dia"], start=start)
+        base["nproc"] = os.cpu_count()
+        base["affinity_cores"] = aff
         log(f"cpu baseline {base['value']:.3e} particle-steps/s on {thr} threads "
-            f"({time.time() - t0:.1f}s)")
+            f"(nproc {os.cpu_count()}, {aff} usable; {time.time() - t0:.1f}s)")
 
     value = n * args.steps / el
     out = {
@@ -289,7 +426,7 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": desc,
-            "n_particles": n, "seed": 0xC3 if args.workload == "packed_bed" else 0xC2,
+            "n_particles": n, "seed": P["seed"], **P["info"],
             "parallelism": "single-gpu",
             "rebuilds_timed": e.n_rebuild - rb0,
             "launches_timed": launches, "steps_executed": executed,
@@ -298,10 +435,12 @@ def main():
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+            "traffic_n_contact": traffic_nc,
             "kernel": kname, "kernel_ms": kern_ms, "bytes_per_particle": B,
             "attainable_copy_gbs": copy_gbs,
         },
         "cpu_baseline": base,
+        "rebuild": reb,
     }
     print(json.dumps(out), flush=True)
 

@@ -145,7 +145,15 @@ _SIGS = {
                                     C.POINTER(C.c_ulonglong)]),
     "dem_write_tec": (C.c_int, [C.c_char_p, C.c_int, C.c_double, C.c_int, D, D, D, D, D]),
     "dem_load_para": (C.c_int, [C.c_char_p, I, I, I]),
+    "dem_profile_rebuild": (C.c_int, [P, C.c_int, D]),
+    "dem_log_header": (C.c_int, [C.c_char_p, C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "dem_log_output": (C.c_int, [C.c_char_p, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int,
+                                 C.c_int, C.c_int]),
+    "dem_log_total": (C.c_int, [C.c_char_p, C.c_double]),
+    "dem_key_bits": (C.c_int, [P]),
+    "dem_scale_velocities": (C.c_int, [P, C.c_double]),
     "dem_dump_write": (C.c_int, [P, C.c_char_p, C.c_int]),
+    "dem_dump_write_binary": (C.c_int, [P, C.c_char_p, C.c_int]),
     "dem_dump_read": (C.c_int, [P, C.c_char_p, I]),
     "dem_set_lqdcnt": (C.c_int, [P, I]),
     "dem_set_clock": (C.c_int, [P, C.c_int, C.c_double, C.c_double]),

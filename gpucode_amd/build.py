@@ -26,6 +26,7 @@ SOURCES = [
     "dem_batch.cpp",
     "surrogate.hip",
     "dem_tune.cpp",
+    "LogFile.cpp",
 ]
 EXTRA_SOURCES = []  # appended by later modules (batched engine, surrogate)
 
@@ -76,7 +77,24 @@ def build(verbose=False):
     if r.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}{r.stderr}")
     os.replace(tmp, LIB)
+    build_examples()
     return LIB
+
+
+EXAMPLE = os.path.join(HERE, "examples", "dem_main")
+
+
+def build_examples():
+    """The driver program of examples/ (the reference's main.cpp loop on the
+    C ABI), built with the host compiler against include/ and the library."""
+    src = os.path.join(HERE, "examples", "dem_main.cpp")
+    cmd = [shutil.which("g++") or "g++", "-O2", "-std=c++17", "-Wall", src,
+           f"-I{os.path.join(ROOT, 'include')}", f"-L{HERE}", "-lgpucode_amd",
+           "-Wl,-rpath,$ORIGIN/..", "-o", EXAMPLE]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"example build failed: {' '.join(cmd)}\n{r.stdout}{r.stderr}")
+    return EXAMPLE
 
 
 if __name__ == "__main__":

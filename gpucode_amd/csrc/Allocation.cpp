@@ -195,6 +195,8 @@ static void free_all(dem_ctx *c) {
     (void)hipFree(c->sort_tmp); (void)hipFree(c->nmask);
     for (int q = 0; q < 7; ++q) (void)hipFree(c->F[q]);
     (void)hipFree(c->stop_at); (void)hipFree(c->overflow); (void)hipFree(c->upd_count);
+    for (hipEvent_t &e : c->rb_ev)
+        if (e) (void)hipEventDestroy(e);
     (void)hipFree(c->stats);
 }
 

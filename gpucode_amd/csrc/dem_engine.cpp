@@ -9,6 +9,7 @@
 #include <limits.h>
 #include <string.h>
 
+#include <algorithm>
 #include <vector>
 
 #include "dem_host.h"
@@ -617,10 +618,17 @@ extern "C" int dem_pair_records_used(dem_ctx *c) {
 // ---------------------------------------------------------------------
 // rebuild pieces (main.cpp:473-527)
 // ---------------------------------------------------------------------
+// event after rebuild stage k (0 = start) while a profile is taken
+static void rb_mark(dem_ctx *c, int k) {
+    if (c->rb_prof) (void)hipEventRecord(c->rb_ev[k], c->stream);
+}
+
 extern "C" int dem_cnt_hstry_save(dem_ctx *c) {
     if (!c) return dem_fail(DEM_ERR_ARG, "null context");
+    rb_mark(c, 0);
     int rc = dem_pair_unpack(c);
     if (rc) return rc;
+    rb_mark(c, 1);
     if (c->has_lists) {
         c->hcur = 1 - c->hcur;
         c->hl = 1 - c->hl;
@@ -636,17 +644,25 @@ extern "C" int dem_bld_nebor_list(dem_ctx *c) {
     c->old_total = c->saved ? c->old_total : 0;
     StateBuf a = c->buf(c->cur, c->Lidx), b = c->buf(1 - c->cur, 1 - c->Lidx);
     launch_calc_hash(a, c->level[c->pcur], c->id[c->pcur], c->P, c->keys, c->vals, n, s);
+    rb_mark(c, 2);
     launch_radix_sort(c->keys, c->vals, c->keys2, c->vals2, n, c->key_bits, c->sort_tmp, s);
+    rb_mark(c, 3);
     launch_permute(c->vals, a, b, c->level[c->pcur], c->level[1 - c->pcur], c->id[c->pcur],
                    c->id[1 - c->pcur], n, s);
+    rb_mark(c, 4);
     c->cur = 1 - c->cur;
     c->Lidx = 1 - c->Lidx;
     c->pcur = 1 - c->pcur;
     launch_cell_range(c->keys, c->buf(c->cur, c->Lidx), c->level[c->pcur], c->P, c->cstart,
                       c->cend, c->S.sysNCells, n, s);
+    rb_mark(c, 5);
     launch_fill_i32(c->overflow, 0, 1, s);
-    launch_neighbor_find(c->buf(c->cur, c->Lidx), c->level[c->pcur], c->P, c->cstart, c->cend,
-                         c->nbr[c->hcur], c->nnb[c->hcur], c->overflow, c->cap, n, s);
+    // the tile list of the LDS-staged search lives in vals2 / keys2, free
+    // between the sort and the compact-history layout
+    launch_neighbor_find(c->buf(c->cur, c->Lidx), c->level[c->pcur], c->P, c->keys, c->cstart,
+                         c->cend, c->nbr[c->hcur], c->nnb[c->hcur], c->overflow, (int *)c->vals2,
+                         (int *)c->keys2, c->cap, n, s);
+    rb_mark(c, 6);
     launch_active_slots(c->P4[c->cur], c->nbr[c->hcur], c->nnb[c->hcur], c->amask[c->hl],
                         c->actoff[c->hl], c->nmask, near_gap_of(c->P), c->keys2, c->sort_tmp,
                         c->cap, n, s);
@@ -656,8 +672,10 @@ extern "C" int dem_bld_nebor_list(dem_ctx *c) {
 
 extern "C" int dem_bld_cir_wall(dem_ctx *c) {
     if (!c) return dem_fail(DEM_ERR_ARG, "null context");
+    rb_mark(c, 7);
     launch_wall_list(c->buf(c->cur, c->Lidx), c->P, c->Tw, c->wl[c->hcur], c->nwl[c->hcur], c->cap,
                      c->total, c->stream);
+    rb_mark(c, 8);
     HIPCHK(hipGetLastError());
     return DEM_OK;
 }
@@ -670,6 +688,7 @@ extern "C" int dem_hstry_copy(dem_ctx *c) {
                       c->actoff[ol], c->hp[c->hl], c->hp[ol], c->wl[c->hcur],
                       c->nwl[c->hcur], c->wl[o], c->nwl[o], c->hw[c->hcur], c->hw[o],
                       c->cap, (long)c->K * c->cap, c->total, c->saved ? c->old_total : 0, c->stream);
+    rb_mark(c, 9);
     HIPCHK(hipGetLastError());
     c->saved = false;
     c->has_lists = true;
@@ -718,6 +737,35 @@ extern "C" int dem_rebuild(dem_ctx *c) {
     int rc = dem_do_rebuild(c);
     if (rc) return rc;
     return check_overflow(c);
+}
+
+extern "C" int dem_key_bits(dem_ctx *c) { return c ? c->key_bits : 0; }
+
+extern "C" int dem_scale_velocities(dem_ctx *c, double factor) {
+    if (!c || !(factor >= 0.0)) return dem_fail(DEM_ERR_ARG, "bad argument");
+    launch_scale_velocities(c->buf(c->cur, c->Lidx), factor, c->total, c->stream);
+    HIPCHK(hipGetLastError());
+    return DEM_OK;
+}
+
+extern "C" int dem_profile_rebuild(dem_ctx *c, int reps, double *stage_ms) {
+    if (!c || reps < 1 || !stage_ms) return dem_fail(DEM_ERR_ARG, "bad argument");
+    if (!c->has_lists || c->total <= 0) return dem_fail(DEM_ERR_STATE, "no lists to rebuild");
+    for (int k = 0; k <= dem_ctx::kRbStages; ++k)
+        if (!c->rb_ev[k]) HIPCHK(hipEventCreate(&c->rb_ev[k]));
+    for (int k = 0; k < dem_ctx::kRbStages; ++k) stage_ms[k] = 0.0;
+    int rc = DEM_OK;
+    for (int r = 0; r < reps && rc == DEM_OK; ++r) {
+        c->rb_prof = true;
+        rc = dem_rebuild(c);  // synchronises (overflow check)
+        c->rb_prof = false;
+        for (int k = 0; rc == DEM_OK && k < dem_ctx::kRbStages; ++k) {
+            float ms = 0.f;
+            HIPCHK(hipEventElapsedTime(&ms, c->rb_ev[k], c->rb_ev[k + 1]));
+            stage_ms[k] += ms / reps;
+        }
+    }
+    return rc;
 }
 
 // ---------------------------------------------------------------------
@@ -984,17 +1032,23 @@ extern "C" int dem_stream_copy(long long nbytes, int iters, double *gbps) {
             rc = dem_fail(DEM_ERR_HIP, "stream copy setup: %s", hipGetErrorString(e));
             break;
         }
-        launch_stream_copy(a, b, nbytes, s);  // warm
-        (void)hipEventRecord(e0, s);
-        for (int k = 0; k < iters; ++k) launch_stream_copy(a, b, nbytes, s);
-        (void)hipEventRecord(e1, s);
-        if ((e = hipStreamSynchronize(s)) != hipSuccess) {
+        // the best of a few launch shapes (blocks per CU) is the attainable rate
+        double best = 0.0;
+        for (int bpc : {2, 4, 8, 16}) {
+            launch_stream_copy(a, b, nbytes, bpc, s);  // warm
+            (void)hipEventRecord(e0, s);
+            for (int k = 0; k < iters; ++k) launch_stream_copy(a, b, nbytes, bpc, s);
+            (void)hipEventRecord(e1, s);
+            if ((e = hipStreamSynchronize(s)) != hipSuccess) break;
+            float ms = 0.f;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            best = std::max(best, 2.0 * (double)nbytes * iters / (ms * 1e-3) / 1e9);
+        }
+        if (e != hipSuccess) {
             rc = dem_fail(DEM_ERR_HIP, "stream copy: %s", hipGetErrorString(e));
             break;
         }
-        float ms = 0.f;
-        (void)hipEventElapsedTime(&ms, e0, e1);
-        *gbps = 2.0 * (double)nbytes * iters / (ms * 1e-3) / 1e9;
+        *gbps = best;
     } while (0);
     if (a) (void)hipFree(a);
     if (b) (void)hipFree(b);

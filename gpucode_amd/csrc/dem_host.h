@@ -102,6 +102,11 @@ struct dem_ctx {
 
     // timing of the fused path
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // rebuild stage profile (dem_profile_rebuild): when on, an event after
+    // each stage of a rebuild
+    static const int kRbStages = 9;
+    hipEvent_t rb_ev[kRbStages + 1] = {};
+    bool rb_prof = false;
     double fused_ms = 0.0, rebuild_ms = 0.0;
     long long fused_steps = 0, fused_launches = 0;
     long long bytes = 0;

@@ -441,12 +441,205 @@ __global__ void __launch_bounds__(128) k_neighbor_find(const double4 *p, const i
     neighbor_find_one(p, level, P, cstart, cend, nbr, nnb, overflow, cap, i);
 }
 
-void launch_neighbor_find(const StateBuf &s, const int *level, const DevInst &P, const int *cstart,
-                          const int *cend, int *nbr, int *nnb, int *overflow, int cap, int total,
-                          hipStream_t st) {
+// ---------------------------------------------------------------------
+// The same search with LDS-staged cell tiles, for one size level in Morton
+// storage order.  The particles of an aligned tile of 4 x 4 x 4 cells (the
+// cells whose Morton codes share all but their low 6 bits) are contiguous in
+// storage.  A block takes one tile at a time, copies the positions of the
+// 6 x 6 x 6 cells around it (the tile and its one-cell rim, the cells its
+// particles scan) into LDS once, and each particle scans its 27 cells from
+// LDS in the reference's order: every candidate position is read from HBM
+// or L2 once per tile instead of once per particle that scans it.  A tile
+// whose rim holds more candidates than LDS takes scans from global memory.
+// ---------------------------------------------------------------------
+#define NF_BS 128
+#define NF_EXT 6                         // tile edge + rim, in cells
+#define NF_CELLS (NF_EXT * NF_EXT * NF_EXT)
+#define NF_MAXP 768                      // staged candidate positions
+
+// tile starts (storage positions where the tile prefix of the key changes),
+// appended in any order; one atomic per wave
+__global__ void __launch_bounds__(256) k_tile_starts(const unsigned *keys, int total, int *tiles,
+                                                     int *ntiles) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool st = k < total && (k == 0 || (keys[k] >> 6) != (keys[k - 1] >> 6));
+    const unsigned long long m = __ballot(st);
+    if (m == 0) return;
+    const int lane = threadIdx.x & 63;
+    const int leader = __ffsll((long long)m) - 1;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(ntiles, __popcll(m));
+    base = __shfl(base, leader);
+    if (st) tiles[base + __popcll(m & ((1ull << lane) - 1))] = k;
+}
+
+// every third bit of v from bit 0, packed (the inverse of spread3)
+DEV unsigned compact3(unsigned v) {
+    v &= 0x09249249u;
+    v = (v | (v >> 2)) & 0x030c30c3u;
+    v = (v | (v >> 4)) & 0x0300f00fu;
+    v = (v | (v >> 8)) & 0x030000ffu;
+    v = (v | (v >> 16)) & 0x000003ffu;
+    return v;
+}
+
+__global__ void __launch_bounds__(NF_BS) k_neighbor_tiles(const double4 *p, const int *level,
+                                                          DevInst P, const unsigned *keys,
+                                                          const int *cstart, const int *cend,
+                                                          const int *tiles, const int *ntiles,
+                                                          int *nbr, int *nnb, int *overflow, int cap,
+                                                          int total) {
+    __shared__ double4 sp[NF_MAXP];
+    __shared__ int scount[256];          // candidates per rim cell, then their LDS offsets
+    __shared__ int sglob[NF_CELLS];      // global start of each rim cell
+    __shared__ int sbound[2];
+    const int tid = threadIdx.x;
+    const int nt = *ntiles;
+    const int gx = P.gx[0], gy = P.gy[0], gz = P.gz[0];
+    const int K = P.levelK[0];
+    const double cs = P.cellSize[0];
+    for (int t = blockIdx.x; t < nt; t += gridDim.x) {
+        if (tid == 0) {
+            const int k0 = tiles[t];
+            const unsigned pre = keys[k0] >> 6;
+            // end of the tile: first key past its prefix (keys are sorted)
+            int lo = k0 + 1, hi = total;
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if ((keys[mid] >> 6) == pre) lo = mid + 1;
+                else hi = mid;
+            }
+            sbound[0] = k0;
+            sbound[1] = lo;
+        }
+        __syncthreads();
+        const int k0 = sbound[0], k1 = sbound[1];
+        const unsigned pre = (keys[k0] - P.mortonOff[0]) >> 6;
+        const int x0 = 4 * (int)compact3(pre) - 1, y0 = 4 * (int)compact3(pre >> 1) - 1,
+                  z0 = 4 * (int)compact3(pre >> 2) - 1;
+        // candidates of each rim cell (cells outside the grid hold none)
+        for (int c = tid; c < 256; c += NF_BS) {
+            int cnt = 0;
+            if (c < NF_CELLS) {
+                const int x = x0 + c % NF_EXT, y = y0 + (c / NF_EXT) % NF_EXT,
+                          z = z0 + c / (NF_EXT * NF_EXT);
+                int g0 = 0;
+                if (x >= 0 && x < gx && y >= 0 && y < gy && z >= 0 && z < gz) {
+                    const int h = P.cellOff[0] + z * gy * gx + y * gx + x;
+                    const int s0 = cstart[h];
+                    if (s0 != DEM_EMPTY) {
+                        g0 = s0;
+                        cnt = cend[h] - s0;
+                    }
+                }
+                sglob[c] = g0;
+            }
+            scount[c] = cnt;
+        }
+        __syncthreads();
+        // exclusive prefix sum of the 256 counts (two per thread)
+        {
+            const int a = scount[2 * tid], b = scount[2 * tid + 1];
+            int v = a + b;
+            __syncthreads();
+            scount[2 * tid + 1] = v;  // pair sums at odd slots, scanned below
+            __syncthreads();
+            for (int d = 1; d < NF_BS; d <<= 1) {
+                int u = tid >= d ? scount[2 * (tid - d) + 1] : 0;
+                __syncthreads();
+                if (tid >= d) scount[2 * tid + 1] += u;
+                __syncthreads();
+            }
+            const int incl = scount[2 * tid + 1];
+            __syncthreads();
+            scount[2 * tid] = incl - v;       // exclusive start of cell 2 tid
+            scount[2 * tid + 1] = incl - b;   // exclusive start of cell 2 tid + 1
+            if (tid == NF_BS - 1) sbound[0] = incl;
+            __syncthreads();
+        }
+        const int ncand = sbound[0];
+        const bool staged = ncand <= NF_MAXP;
+        if (staged) {
+            // copy: candidate q belongs to the last cell whose start is <= q
+            for (int q = tid; q < ncand; q += NF_BS) {
+                int lo = 0, hi = NF_CELLS - 1;
+                while (lo < hi) {
+                    const int mid = (lo + hi + 1) >> 1;
+                    if (scount[mid] <= q) lo = mid;
+                    else hi = mid - 1;
+                }
+                sp[q] = p[sglob[lo] + (q - scount[lo])];
+            }
+        }
+        __syncthreads();
+        for (int i = k0 + tid; i < k1; i += NF_BS) {
+            if (!staged) {
+                neighbor_find_one(p, level, P, cstart, cend, nbr, nnb, overflow, cap, i);
+                continue;
+            }
+            const double4 o = p[i];
+            const double ri = o.w;
+            const int cx = cell_coord(o.x, P.origin[0], cs, gx);
+            const int cy = cell_coord(o.y, P.origin[1], cs, gy);
+            const int cz = cell_coord(o.z, P.origin[2], cs, gz);
+            int num = 0;
+            for (int dz = -1; dz <= 1; ++dz) {
+                const int zz = cz + dz;
+                if (zz < 0 || zz > gz - 1) continue;
+                for (int dy = -1; dy <= 1; ++dy) {
+                    const int yy = cy + dy;
+                    if (yy < 0 || yy > gy - 1) continue;
+                    int sx = cx - 1, ex = cx + 1;
+                    if (sx < 0) sx = 0;
+                    if (ex > gx - 1) ex = gx - 1;
+                    const int rowc = (zz - z0) * NF_EXT * NF_EXT + (yy - y0) * NF_EXT - x0;
+                    for (int x = sx; x <= ex; ++x) {
+                        const int c = rowc + x;
+                        const int q0 = scount[c], q1 = scount[c + 1];
+                        const int g0 = sglob[c] - q0;
+                        for (int q = q0; q < q1; ++q) {
+                            const int j = g0 + q;
+                            if (j == i) continue;
+                            const double4 b = sp[q];
+                            double dx = o.x - b.x, dy2 = o.y - b.y, dz2 = o.z - b.z;
+                            double len = sqrt(dx * dx + dy2 * dy2 + dz2 * dz2);
+                            if (len < b.w + ri + P.searchGap) {
+                                if (num < K) nbr[(long)num * cap + i] = j;
+                                ++num;
+                            }
+                        }
+                    }
+                }
+            }
+            if (num > K) {
+                atomicMax(overflow, num);
+                num = K;
+            }
+            nnb[i] = num;
+        }
+        __syncthreads();
+    }
+}
+
+// tiles, ntiles: scratch of total + 1 ints (free while the lists are built)
+void launch_neighbor_find(const StateBuf &s, const int *level, const DevInst &P, const unsigned *keys,
+                          const int *cstart, const int *cend, int *nbr, int *nnb, int *overflow,
+                          int *tiles, int *ntiles, int cap, int total, hipStream_t st) {
     if (total <= 0) return;
-    hipLaunchKernelGGL(k_neighbor_find, dim3(blocks_for(total, 128)), dim3(128), 0, st, s.p, level,
-                       P, cstart, cend, nbr, nnb, overflow, cap, total);
+    const bool tiled = P.morton && P.nLevels == 1 && (P.mortonOff[0] & 63u) == 0 &&
+                       P.gx[0] <= 1024 && P.gy[0] <= 1024 && P.gz[0] <= 1024;
+    if (!tiled) {
+        hipLaunchKernelGGL(k_neighbor_find, dim3(blocks_for(total, 128)), dim3(128), 0, st, s.p,
+                           level, P, cstart, cend, nbr, nnb, overflow, cap, total);
+        return;
+    }
+    launch_fill_i32(ntiles, 0, 1, st);
+    hipLaunchKernelGGL(k_tile_starts, dim3(blocks_for(total, 256)), dim3(256), 0, st, keys, total,
+                       tiles, ntiles);
+    // a few resident blocks per CU walk the tiles
+    const int grid = std::min(2048, std::max(1, total / 64));
+    hipLaunchKernelGGL(k_neighbor_tiles, dim3(grid), dim3(NF_BS), 0, st, s.p, level, P, keys, cstart,
+                       cend, tiles, ntiles, nbr, nnb, overflow, cap, total);
 }
 
 // ---------------------------------------------------------------------
@@ -1046,32 +1239,47 @@ DEV double normal_law(const DevLaw &c, HistRec &h, double d, double R) {
     return fN;
 }
 
-// tangential spring (kernel.cu:172-217 p-p with 1e-7 tolerances,
-// 306-350 p-w with 1e-10)
+// tangential spring update (kernel.cu:172-203 p-p with 1e-7 tolerances,
+// 306-337 p-w with 1e-10): rotate the stored spring into the tangent plane
+// keeping its length, add v_t dt, cap it at xi_max.  Returns xi_max; t is
+// the new spring (also stored in h) and u = 1 - |t|/xi_max where the spring
+// sticks below the cap with xi_max above the tolerance (the Mindlin ratio is
+// then 1 - u^1.5), else -1 (ratio 1).
 template <bool PP>
-DEV V3 tangential_law(const DevLaw &c, HistRec &h, double fN, double d, double R, V3 n, V3 vt,
-                      double dt) {
+DEV double spring_update(const DevLaw &c, HistRec &h, double fN, double d, double R, V3 n, V3 vt,
+                         double dt, V3 &t, double &u) {
     const double tol = PP ? 1.0e-7 : 1.0e-10;
     V3 old = v3(h.dx, h.dy, h.dz);
-    V3 t = cross(smulv(-1.0, n), cross(n, old));
+    t = cross(smulv(-1.0, n), cross(n, old));
     double lt = len(t);
     if (lt > tol)
         t = add(divv(mulv(t, len(old)), lt), mulv(vt, dt));
     else
         t = add(t, mulv(vt, dt));
     double dsmx = c.dsmxA * fN / (c.dsmxB * sqrt(R * d));
-    double ratio = 1.0;
     double l2 = len(t);
+    u = -1.0;
     if (l2 < dsmx) {
-        if (dsmx > tol) ratio = 1.0 - pow15(1.0 - l2 / dsmx);
+        if (dsmx > tol) u = 1.0 - l2 / dsmx;
     } else {
-        ratio = 1.0;
         if (l2 > tol)
             t = divv(smulv(dsmx, t), l2);
         else
             t = smulv(dsmx, t);
     }
     h.dx = t.x; h.dy = t.y; h.dz = t.z;
+    return dsmx;
+}
+
+// tangential force (kernel.cu:205-217 p-p, 339-350 p-w): along the spring
+// while it is below the cap, against the sliding velocity at the cap
+template <bool PP>
+DEV V3 tangential_law(const DevLaw &c, HistRec &h, double fN, double d, double R, V3 n, V3 vt,
+                      double dt) {
+    V3 t;
+    double u;
+    const double dsmx = spring_update<PP>(c, h, fN, d, R, n, vt, dt, t, u);
+    const double ratio = u >= 0.0 ? 1.0 - pow15(u) : 1.0;
     V3 ft = v3(0.0, 0.0, 0.0);
     double l3 = len(t);
     if (l3 < dsmx) {
@@ -1101,89 +1309,48 @@ DEV void contact(const DevLaw &c, double rolling, double dt, HistRec &h, double 
     Mr = Mr + rolling * fN * ri;
 }
 
-// The same contact in mixed precision.  FP64: the overlap d and every
-// difference of history values (the load/unload verdict, d - plstDfm, the
-// new plstDfm), the normal n, and the tangential spring update (rotation
-// into the tangent plane, rescale, + v_t dt), so nothing that accumulates
-// over steps is rounded to FP32.  FP32: the force magnitudes and their
-// square roots and powers (Hertz, plastic radius, damping, xi_max, Mindlin
-// ratio) and the force/moment vectors of this pair.
+// The same contact in mixed precision.  Everything that is stored or that
+// decides a branch is computed in FP64 exactly as in contact(): the normal
+// law with its load/unload verdict and plastic history, the tangential
+// spring with its rotation, rescale and Coulomb cap, xi_max and the
+// stick/slip test.  The reference's friction force turns from the spring
+// direction to the sliding direction on a last-bit comparison at the cap
+// (kernel.cu:190-217), so a verdict taken from FP32 values would flip
+// against FP64 for every contact resting at the cap.  FP32: the Mindlin
+// ratio's power, the damping magnitude and the force, moment and rolling
+// terms of the pair, from the FP64 values above.
 template <bool PP>
 DEV void contact_mixed(const DevLaw &c, double rolling, double dt, HistRec &h, double d, V3 n,
                        double R, double ri, V3 cpv, V3 &F, V3 &M, double &Mr) {
-    const float tol = PP ? 1.0e-7f : 1.0e-10f;
-    int stage;
-    if (d < h.pdfm) stage = 1;
-    else if (d >= h.pdfm && d < h.nd) stage = 3;
-    else if (d >= h.nd && d < h.ndm) stage = 4;
-    else stage = 2;
-    const float c43E = (float)c.c43E;
-    const float sRd = sqrtf((float)(R * d));
-    float fN = 0.0f;
-    if (stage == 2) {
-        bool elastic = PP ? (d <= c.ydsp_t * R) : (d < (c.ydsp_t * R));
-        if (elastic) {
-            fN = c43E * sRd * (float)d;
-            h.prad = R;
-            h.pdfm = 0.0;
-        } else {
-            double fp = (c.yforce_t * (R * R)) + (c.ystiff_t * R) * (d - (c.ydsp_t * R));
-            fN = (float)fp;
-            float fH = c43E * sRd * (float)d;
-            h.prad = R * (double)(fH / fN);
-            float q = fN / (c43E * sqrtf((float)R));
-            h.pdfm = d - (double)cbrtf(q * q);
-        }
-        h.ndm = d;
-    } else if (stage != 1) {
-        float x = (float)(d - h.pdfm);
-        fN = c43E * sqrtf((float)h.prad) * (x * sqrtf(x));
-    }
-    h.nd = d;
-    h.flags = (h.flags & ~0xff) | stage;
-
+    const double fN = normal_law<PP>(c, h, d, R);
     const double cpvn = dot(cpv, n);
-    const float fD = c.damp_on ? (float)c.damp * sRd * (float)cpvn : 0.0f;
     const V3 vt = sub(cpv, smulv(cpvn, n));
-
-    // spring: project onto the tangent plane, keep its length, add v_t dt
-    V3 old = v3(h.dx, h.dy, h.dz);
-    V3 t = sub(old, smulv(dot(n, old), n));
-    const double lt2 = dot(t, t);
-    if (lt2 > (double)tol * (double)tol)
-        t = add(mulv(t, sqrt(dot(old, old) / lt2)), mulv(vt, dt));
-    else
-        t = add(t, mulv(vt, dt));
-    const float dsmx = (float)c.dsmxA * fN / ((float)c.dsmxB * sRd);
-    const float lt = sqrtf((float)dot(t, t));
+    V3 t;
+    double u;
+    const double dsmx = spring_update<PP>(c, h, fN, d, R, n, vt, dt, t, u);
     float ratio = 1.0f;
-    if (lt < dsmx) {
-        if (dsmx > tol) {
-            float u = 1.0f - lt / dsmx;
-            ratio = 1.0f - u * sqrtf(u);
-        }
-    } else if (lt > tol) {
-        t = mulv(t, (double)(dsmx / lt));
-    } else {
-        t = mulv(t, (double)dsmx);
+    if (u >= 0.0) {
+        const float uf = (float)u;
+        ratio = 1.0f - uf * sqrtf(uf);
     }
-    h.dx = t.x; h.dy = t.y; h.dz = t.z;
+    const float fNf = (float)fN;
+    const float sf = -(float)c.sfrc * fNf * ratio;
     float fx = 0.f, fy = 0.f, fz = 0.f;
-    const float sf = -(float)c.sfrc * fN * ratio;
-    const float l3 = sqrtf((float)dot(t, t));
+    const double l3 = len(t);
     if (l3 < dsmx) {
-        if (l3 > 1.0e-10f) {
-            float k = sf / l3;
+        if (l3 > 1.0e-10) {
+            const float k = sf / (float)l3;
             fx = k * (float)t.x; fy = k * (float)t.y; fz = k * (float)t.z;
         }
     } else {
-        const float lv = sqrtf((float)dot(vt, vt));
-        if (lv > 1.0e-10f) {
-            float k = sf / lv;
+        const double lv = len(vt);
+        if (lv > 1.0e-10) {
+            const float k = sf / (float)lv;
             fx = k * (float)vt.x; fy = k * (float)vt.y; fz = k * (float)vt.z;
         }
     }
-    const float fn = fN + fD;
+    const float fD = c.damp_on ? (float)c.damp * sqrtf((float)(R * d)) * (float)cpvn : 0.0f;
+    const float fn = fNf + fD;
     const float nx = (float)n.x, ny = (float)n.y, nz = (float)n.z;
     F.x += (double)(fn * nx + fx);
     F.y += (double)(fn * ny + fy);
@@ -1193,7 +1360,7 @@ DEV void contact_mixed(const DevLaw &c, double rolling, double dt, HistRec &h, d
     M.x += (double)(ay * fz - az * fy);
     M.y += (double)(az * fx - ax * fz);
     M.z += (double)(ax * fy - ay * fx);
-    Mr += (double)((float)rolling * fN * (float)ri);
+    Mr += (double)((float)rolling * fNf * (float)ri);
 }
 
 // Willett bridge force (kernel.cu:65-81) at the instance's contact angle
@@ -1251,19 +1418,8 @@ DEV void pp_particle(const DevInst &P, const StepArgs &A, int i, const double4 p
         const V3 jp = v3(pj.x, pj.y, pj.z);
         const double rj = pj.w;
         const V3 dv = sub(ip, jp);
-        if (MIXED && !WET) {
-            // dry: pairs apart need no square root
-            const double rs = ri + rj;
-            if (!(dot(dv, dv) < rs * rs)) continue;
-        }
         const double l = len(dv);
-        V3 nv;
-        if (MIXED) {
-            const double il = 1.0 / l;
-            nv = mulv(dv, il);
-        } else {
-            nv = divv(dv, l);
-        }
+        const V3 nv = divv(dv, l);
         const double gap = l - ri - rj;
         HistRec h;
         bool loaded = false;
@@ -1489,22 +1645,15 @@ void launch_step_fused(const DevInst &P, const StepArgs &A, bool wet, bool mixed
 // ---------------------------------------------------------------------
 #define CP_BS 256
 
-template <bool MIXED>
+// overlap test, gap and normal of a pair, in FP64 in both precisions (the
+// normal and the overlap feed the stored history)
 DEV bool pair_geometry(const double4 pi, const double4 pj, V3 &nv, double &gap) {
     const V3 dv = v3(pi.x - pj.x, pi.y - pj.y, pi.z - pj.z);
-    if (MIXED) {
-        const double rs = pi.w + pj.w;
-        if (!(dot(dv, dv) < rs * rs)) return false;
-        const double l = len(dv);
-        nv = mulv(dv, 1.0 / l);
-        gap = l - pi.w - pj.w;
-    } else {
-        const double l = len(dv);
-        gap = l - pi.w - pj.w;
-        if (!(gap < 0.0)) return false;
-        nv = divv(dv, l);
-    }
-    return gap < 0.0;
+    const double l = len(dv);
+    gap = l - pi.w - pj.w;
+    if (!(gap < 0.0)) return false;
+    nv = divv(dv, l);
+    return true;
 }
 
 template <bool MIXED>
@@ -1534,7 +1683,7 @@ __global__ void __launch_bounds__(CP_BS) k_step_compact(DevInst P, StepArgs A) {
             const int j = A.nbr[(long)k * A.cap + i];
             V3 nv;
             double gap;
-            if (pair_geometry<MIXED>(pi, A.in.p[j], nv, gap)) mask |= 1ull << k;
+            if (pair_geometry(pi, A.in.p[j], nv, gap)) mask |= 1ull << k;
         }
     }
     // phase B: pack (particle, slot) pairs in slot order
@@ -1571,7 +1720,7 @@ __global__ void __launch_bounds__(CP_BS) k_step_compact(DevInst P, StepArgs A) {
             const double4 qi = A.in.p[ii], qj = A.in.p[j];
             V3 nv;
             double gap;
-            pair_geometry<MIXED>(qi, qj, nv, gap);
+            pair_geometry(qi, qj, nv, gap);
             const V3 iv = vel_of(A.in.v, ii), iw = spin_of(A.in.v, ii);
             const V3 jv = vel_of(A.in.v, j), jw = spin_of(A.in.v, j);
             const long hi = hist_idx(A.amask[ii], A.actoff[ii], k, ii, A.cap, (long)A.K * A.cap);
@@ -1698,7 +1847,7 @@ DEV void pair_compute(const DevInst &P, HistRec &h, const double4 qi, const doub
     const V3 jv = v3(aj.x, aj.y, aj.z), jw = v3(aj.w, bj.x, bj.y);
     V3 nv;
     double gap;
-    pair_geometry<MIXED>(qi, qj, nv, gap);
+    pair_geometry(qi, qj, nv, gap);
     const double ri = qi.w, rj = qj.w;
     const double R = ri * rj / (ri + rj);
     const V3 cpv = sub(sub(iv, jv), cross(add(mulv(iw, ri), mulv(jw, rj)), nv));
@@ -1886,7 +2035,7 @@ DEV void step_wave_block(const DevInst &P, const StepArgs &A, unsigned blk, doub
                 for (int u = 0; u < WV_NB; ++u) {
                     V3 nv;
                     double gap;
-                    if (r0 + u < nn && pair_geometry<MIXED>(pi, q[u], nv, gap)) {
+                    if (r0 + u < nn && pair_geometry(pi, q[u], nv, gap)) {
                         mask |= 1ull << (r0 + u);
                         if (owns_record(i, jj[u], pi.w, q[u].w)) ownm |= 1ull << (r0 + u);
                     }
@@ -1912,7 +2061,7 @@ DEV void step_wave_block(const DevInst &P, const StepArgs &A, unsigned blk, doub
                 for (int u = 0; u < WV_NB; ++u) {
                     V3 nv;
                     double gap;
-                    if (ks[u] >= 0 && pair_geometry<MIXED>(pi, q[u], nv, gap)) mask |= 1ull << ks[u];
+                    if (ks[u] >= 0 && pair_geometry(pi, q[u], nv, gap)) mask |= 1ull << ks[u];
                 }
             }
         }
@@ -2336,7 +2485,7 @@ __global__ void __launch_bounds__(256) k_analyze(const DevInst *insts, const Ana
         for (int k = 0; k < n; ++k) {
             V3 nv;
             double gap;
-            if (pair_geometry<false>(pi, G.p[G.nbr[(long)k * cap + i]], nv, gap)) ++c;
+            if (pair_geometry(pi, G.p[G.nbr[(long)k * cap + i]], nv, gap)) ++c;
         }
         bool touch = c > 0;
         for (int k = 0; k < nw; ++k) {
@@ -2714,18 +2863,50 @@ void launch_scatter_state(const double *pos, const double *vel, const double *an
 // plain device copy: the attainable HBM bandwidth, and the known-size
 // kernel that calibrates the FETCH_SIZE / WRITE_SIZE counters
 // ---------------------------------------------------------------------
+// Each lane issues U independent 16-byte loads (a wave reads U contiguous
+// 1 KiB blocks, one grid stride apart) before it stores them, so enough
+// reads are in flight per CU to reach the HBM bandwidth.
+template <int U>
 __global__ void __launch_bounds__(256) k_stream_copy(const double2 *__restrict__ a,
                                                      double2 *__restrict__ b, long n) {
-    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    const long stride = (long)gridDim.x * blockDim.x;
-    for (; i < n; i += stride) b[i] = a[i];
+    const long T = (long)gridDim.x * blockDim.x;
+    for (long i0 = (long)blockIdx.x * blockDim.x + threadIdx.x; i0 < n; i0 += T * U) {
+        double2 v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (i0 + u * T < n) v[u] = a[i0 + u * T];
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (i0 + u * T < n) b[i0 + u * T] = v[u];
+    }
 }
 
-void launch_stream_copy(const void *a, void *b, long nbytes, hipStream_t s) {
+// blocks_per_cu x 256 CUs of 256 threads
+void launch_stream_copy(const void *a, void *b, long nbytes, int blocks_per_cu, hipStream_t s) {
     long n = nbytes / 16;
     if (n > 0)
-        hipLaunchKernelGGL(k_stream_copy, dim3(256 * 8 * 4), dim3(256), 0, s, (const double2 *)a,
-                           (double2 *)b, n);
+        hipLaunchKernelGGL(k_stream_copy<8>, dim3(256 * blocks_per_cu), dim3(256), 0, s,
+                           (const double2 *)a, (double2 *)b, n);
+}
+
+// ---------------------------------------------------------------------
+// velocity and spin scaling (a settling aid: a quench to rest keeps the
+// contact history)
+// ---------------------------------------------------------------------
+__global__ void k_scale_velocities(double4 *v, double f, int total) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    double4 a = v[2 * i], b = v[2 * i + 1];
+    a.x *= f; a.y *= f; a.z *= f; a.w *= f;  // vx vy vz wx
+    b.x *= f; b.y *= f;                      // wy wz (path length and tag kept)
+    v[2 * i] = a;
+    v[2 * i + 1] = b;
+}
+
+void launch_scale_velocities(const StateBuf &s, double f, int total, hipStream_t st) {
+    if (total <= 0) return;
+    hipLaunchKernelGGL(k_scale_velocities, dim3(blocks_for(total, 256)), dim3(256), 0, st, s.v, f,
+                       total);
 }
 
 // ---------------------------------------------------------------------

@@ -187,6 +187,22 @@ class Engine:
         return dict(dispt=d, nDsp=f[0], nDsp_mx=f[1], plstRad=f[2], plstDfm=f[3], stage=s,
                     lqdBrg=b)
 
+    def set_contacts(self, num, lst, hp, wnum, wlst, hw):
+        """dem_set_contacts: restart lists and history, as nebor(), hist_pp(),
+        walls() and hist_pw() return them."""
+        i32, f64 = np.int32, np.float64
+        a = [np.ascontiguousarray(x, t) for x, t in (
+            (num, i32), (lst, i32), (hp["dispt"], f64), (hp["nDsp"], f64), (hp["nDsp_mx"], f64),
+            (hp["plstRad"], f64), (hp["plstDfm"], f64), (hp["stage"], i32), (hp["lqdBrg"], i32),
+            (wnum, i32), (wlst, i32), (hw["dispt"], f64), (hw["nDsp"], f64), (hw["nDsp_mx"], f64),
+            (hw["plstRad"], f64), (hw["plstDfm"], f64), (hw["stage"], i32), (hw["lqdBrg"], i32))]
+        check(A.lib().dem_set_contacts(self._ctx, a[1].shape[1],
+                                       *[iptr(x) if x.dtype == i32 else dptr(x) for x in a]))
+
+    def set_lqdcnt(self, a):
+        a = np.ascontiguousarray(a, np.int32)
+        check(A.lib().dem_set_lqdcnt(self._ctx, iptr(a)))
+
     def hash_index(self):
         n = self.total
         h, i = np.zeros(n, np.int32), np.zeros(n, np.int32)
@@ -297,8 +313,24 @@ class Engine:
         return A.lib().dem_step_kernel_name(self._ctx).decode()
 
     # --- restart --------------------------------------------------------------
-    def dump(self, path, nBatch=1):
-        check(A.lib().dem_dump_write(self._ctx, str(path).encode(), int(nBatch)))
+    REBUILD_STAGES = ("save", "hash", "sort", "permute", "cell_range", "neighbor_find",
+                      "active_slots", "wall_list", "hist_remap")
+
+    def scale_velocities(self, factor):
+        check(A.lib().dem_scale_velocities(self._ctx, float(factor)))
+
+    def key_bits(self):
+        return A.lib().dem_key_bits(self._ctx)
+
+    def profile_rebuild(self, reps=3):
+        """Mean ms per rebuild stage (dem_profile_rebuild), by stage name."""
+        ms = np.zeros(len(self.REBUILD_STAGES))
+        check(A.lib().dem_profile_rebuild(self._ctx, int(reps), dptr(ms)))
+        return dict(zip(self.REBUILD_STAGES, ms.tolist()))
+
+    def dump(self, path, nBatch=1, binary=False):
+        f = A.lib().dem_dump_write_binary if binary else A.lib().dem_dump_write
+        check(f(self._ctx, str(path).encode(), int(nBatch)))
 
     def load(self, path):
         nb = C.c_int()

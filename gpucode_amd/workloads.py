@@ -91,12 +91,14 @@ def lognormal_diameters(n, seed, median=0.5e-3, sigma=0.25, lo=0.25e-3, hi=1.0e-
 
 
 def wet_poly(n=200_000, seed=0xC2, R=25e-3, H=60e-3, steps=50_000, theta_deg=20.0,
-             reference_law=False, nLevels=3, dt=None):
+             reference_law=False, nLevels=3, dt=None, E=70e9):
     """Wet polydisperse packing; reference_law selects theta = 0 with the
-    reference's rupture rule (kernel.cu:522)."""
+    reference's rupture rule (kernel.cu:522).  A smaller E gives soft
+    spheres whose time step lets the pour settle in tens of thousands of
+    steps."""
     dia = lognormal_diameters(n, seed)
     th = 0.0 if reference_law else math.radians(theta_deg)
-    inp = make_input(n, float(dia.max()), R, H, 0.3, cutGap=0.1, nLevels=nLevels, forceCap=1,
+    inp = make_input(n, float(dia.max()), R, H, 0.3, E=E, cutGap=0.1, nLevels=nLevels, forceCap=1,
                      liquid=0.01, theta=th, rupture_mode=0 if reference_law else 1, dt=dt)
     return dict(name="wet_poly", inp=inp, dia=dia, pos=feed_positions(inp, dia, n, seed + 1),
                 steps=steps)
@@ -212,6 +214,43 @@ def packed_bed(n=2_000_000, d=0.3e-3, R=30e-3, H=30e-3, mu=0.5, skin=0.1, seed=0
     if pos[:, 2].max() + 0.5 > H / d:
         raise ValueError("bed taller than the cylinder")
     return dict(name="packed_bed", inp=inp, dia=None, pos=pos, steps=steps)
+
+
+def pour_lattice(n, R, a=1.5, jitter=0.1, seed=0, z0=0.0):
+    """n spheres of diameter 1 (reduced units) on a jittered square lattice of
+    spacing a inside a cylinder of radius R, layers from the floor up: a
+    loose cloud to be dropped."""
+    rng = np.random.default_rng(seed)
+    m = int(R / a) + 1
+    g = np.arange(-m, m + 1) * a
+    x, y = np.meshgrid(g, g, indexing="ij")
+    x, y = x.ravel(), y.ravel()
+    # the x and y jitter together move a site up to sqrt(2) a jitter outward
+    keep = x * x + y * y <= (R - 0.5 - 1.5 * a * jitter) ** 2
+    x, y = x[keep], y[keep]
+    per = len(x)
+    layers = (n + per - 1) // per
+    out = []
+    for L in range(layers):
+        j = rng.uniform(-jitter * a, jitter * a, (per, 3))
+        out.append(np.stack([x + j[:, 0], y + j[:, 1], z0 + 0.5 + a * (L + 0.5) + j[:, 2]], 1))
+    return np.concatenate(out)[:n]
+
+
+def pour_2M(n=2_000_000, d=0.3e-3, R=30e-3, H=75e-3, mu=0.5, skin=0.1, v0=2.0, seed=0xC3,
+            steps=10_000):
+    """The throughput bed's spheres and material as a falling cloud: a loose
+    lattice (spacing 1.5 d) dropped at v0 [m/s] onto the floor, so the lists
+    are rebuilt every few hundred steps."""
+    inp = make_input(n, d, R, H, mu, cutGap=skin)
+    pos = pour_lattice(n, R / d, seed=seed)
+    if pos[:, 2].max() + 0.5 > H / d:
+        raise ValueError("cloud taller than the cylinder")
+    S, _, _, _ = derive(inp, None)
+    rng = np.random.default_rng(seed + 1)
+    vel = rng.normal(size=(n, 3)) * 0.05 * v0 / S.rvunit
+    vel[:, 2] -= v0 / S.rvunit
+    return dict(name="pour_2M", inp=inp, dia=None, pos=pos, vel=vel, steps=steps)
 
 
 def wet_bed(n=3000, R_d=8.0, poly=False, theta_deg=0.0, reference_law=True, nLevels=None,

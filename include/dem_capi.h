@@ -269,6 +269,20 @@ int dem_contact_stats(dem_ctx *ctx, dem_stats *out);
  * executed, kernels launched, rebuild time, rebuild count. */
 int dem_get_timing(dem_ctx *ctx, double *fused_ms, long long *fused_steps,
                    long long *fused_launches, double *rebuild_ms, int *n_rebuild);
+/* Measurement aid with no reference counterpart: reps rebuilds of the
+ * current state (main.cpp:473-527 with the lists that result unchanged), each
+ * stage timed with HIP events on the context's stream; stage_ms[9] receives
+ * the mean time of: 0 history save (pair-record write-back), 1 hash,
+ * 2 radix sort, 3 permutation into cell order, 4 cell ranges, 5 neighbour
+ * search, 6 compact-history layout, 7 wall lists, 8 history remap. */
+int dem_profile_rebuild(dem_ctx *ctx, int reps, double *stage_ms);
+/* Multiplies every velocity and spin by factor >= 0, keeping positions,
+ * lists and contact history: a settling aid (factor 0 quenches the packing
+ * to rest) with no reference counterpart. */
+int dem_scale_velocities(dem_ctx *ctx, double factor);
+/* Bits of the cell key the radix sort orders (8 per pass); a measurement
+ * aid with no reference counterpart. */
+int dem_key_bits(dem_ctx *ctx);
 /* Bytes of device memory held by the context. */
 long long dem_device_bytes(dem_ctx *ctx);
 /* Name of the step kernel dem_step launches for this context's packing
@@ -297,9 +311,23 @@ int dem_write_tec(const char *path, int outs, double time_s, int total, const do
 /* LoadPara (DumpLoad.cpp:7-18) and a full restart dump in the same style. */
 int dem_load_para(const char *path, int *NP, int *nBatch, int *TotalParticle);
 int dem_dump_write(dem_ctx *ctx, const char *path, int nBatch);
+/* The same restart dump with the records after the text header written as
+ * raw little-endian arrays (only the live list slots): a tenth of the size
+ * and written at disk speed, for multi-million-particle restarts. */
+int dem_dump_write_binary(dem_ctx *ctx, const char *path, int nBatch);
+/* Reads either form (DumpLoad.cpp:7-18 reads the same SYSTEM Info header);
+ * the next step rebuilds and remaps the history (RESTART, main.cpp:473). */
 int dem_dump_read(dem_ctx *ctx, const char *path, int *nBatch);
 int dem_set_lqdcnt(dem_ctx *ctx, const int *lqdCnt);
 int dem_set_clock(dem_ctx *ctx, int iter, double ctime, double Tw);
+/* The run log <genfile>_Log.dat: its header (main.cpp:93-139; restart 0 =
+ * a new run), one record per output (main.cpp:591-618; the particle count
+ * is written while total < NP) and the run's total (main.cpp:631-634). */
+int dem_log_header(const char *path, const char *genfile, int restart, int NP, int nBatch,
+                   int total);
+int dem_log_output(const char *path, int outs, double runtime_s, double elapsed_ms, int nrebuild,
+                   int total, int NP, int iter);
+int dem_log_total(const char *path, double seconds);
 
 #ifdef __cplusplus
 }

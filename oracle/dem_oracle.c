@@ -699,6 +699,53 @@ int orc_get_hist_pp(const orc_state *st, int maxK, double *dispt, double *nDsp,
     return 0;
 }
 
+/* RESTART (main.cpp:57-85, 473-519): lists and history saved by the getters
+ * (lists as original ids in slot order, num[total], list[total][maxK]; wall
+ * lists [total][3]) become the current ones, and the next step rebuilds and
+ * moves every surviving pair's history to its new slot.  Returns -1 if a
+ * list is longer than its size level allows. */
+int orc_set_contacts(orc_state *st, int maxK, const int *num, const int *list,
+                     const double *dispt, const double *nDsp, const double *nDsp_mx,
+                     const double *plstRad, const double *plstDfm, const int *stage,
+                     const int *lqdBrg, const int *wnum, const int *wlist,
+                     const double *wdispt, const double *wnDsp, const double *wnDsp_mx,
+                     const double *wplstRad, const double *wplstDfm, const int *wstage,
+                     const int *wlqdBrg) {
+    for (int i = 0; i < st->total; ++i)
+        if (num[i] > st->S.levelNNebors[st->pLevel[i]] || num[i] > maxK || wnum[i] > ORC_WALLS)
+            return -1;
+    for (int i = 0; i < st->total; ++i) {
+        st->numNebor[i] = num[i];
+        for (int k = 0; k < num[i]; ++k) {
+            size_t o = (size_t)i * maxK + k, sl = slot_of(st, i, k);
+            orc_hist1 h;
+            h.dispt[0] = dispt[3 * o]; h.dispt[1] = dispt[3 * o + 1]; h.dispt[2] = dispt[3 * o + 2];
+            h.nDsp = nDsp[o]; h.nDsp_mx = nDsp_mx[o]; h.plstRad = plstRad[o];
+            h.plstDfm = plstDfm[o]; h.stage = stage[o]; h.lqdBrg = lqdBrg[o];
+            st->nebor[sl] = list[o];
+            hist_store(&st->pp, sl, &h);
+        }
+        st->numWall[i] = wnum[i];
+        for (int k = 0; k < wnum[i]; ++k) {
+            size_t o = (size_t)i * ORC_WALLS + k;
+            orc_hist1 h;
+            h.dispt[0] = wdispt[3 * o]; h.dispt[1] = wdispt[3 * o + 1]; h.dispt[2] = wdispt[3 * o + 2];
+            h.nDsp = wnDsp[o]; h.nDsp_mx = wnDsp_mx[o]; h.plstRad = wplstRad[o];
+            h.plstDfm = wplstDfm[o]; h.stage = wstage[o]; h.lqdBrg = wlqdBrg[o];
+            st->wallList[o] = wlist[o];
+            hist_store(&st->pw, (size_t)k * st->NP + i, &h);
+        }
+    }
+    st->restart = 1;
+    return 0;
+}
+
+/* the bridge counts of the last step (lqdCnt, which VarInit makes
+ * lqdCnt_Old), as a restart restores them */
+void orc_set_lqdcnt(orc_state *st, const int *lqdCnt) {
+    memcpy(st->lqdCnt, lqdCnt, (size_t)st->total * sizeof(int));
+}
+
 void orc_get_walls(const orc_state *st, int *num, int *list) {
     for (int i = 0; i < st->total; ++i) {
         num[i] = st->numWall[i];

@@ -166,6 +166,14 @@ int orc_get_hist_pp(const orc_state *st, int maxK, double *dispt, double *nDsp,
                     double *nDsp_mx, double *plstRad, double *plstDfm,
                     int *stage, int *lqdBrg);
 void orc_get_walls(const orc_state *st, int *num, int *list /* [total][3] */);
+int orc_set_contacts(orc_state *st, int maxK, const int *num, const int *list,
+                     const double *dispt, const double *nDsp, const double *nDsp_mx,
+                     const double *plstRad, const double *plstDfm, const int *stage,
+                     const int *lqdBrg, const int *wnum, const int *wlist,
+                     const double *wdispt, const double *wnDsp, const double *wnDsp_mx,
+                     const double *wplstRad, const double *wplstDfm, const int *wstage,
+                     const int *wlqdBrg);
+void orc_set_lqdcnt(orc_state *st, const int *lqdCnt);
 void orc_get_hist_pw(const orc_state *st, double *dispt, double *nDsp,
                      double *nDsp_mx, double *plstRad, double *plstDfm,
                      int *stage, int *lqdBrg); /* each [total][3] */

@@ -102,6 +102,9 @@ _SIGS = {
     "orc_get_hist_pw": (None, [P, D, D, D, D, D, I, I]),
     "orc_get_hash": (None, [P, I, I]),
     "orc_get_cells": (None, [P, I, I]),
+    "orc_set_contacts": (C.c_int, [P, C.c_int, I, I, D, D, D, D, D, I, I, I, I, D, D, D, D, D,
+                                   I, I]),
+    "orc_set_lqdcnt": (None, [P, I]),
     "orc_rebuild": (C.c_int, [P]),
     "orc_step": (C.c_int, [P]),
     "orc_run": (C.c_int, [P, C.c_int]),
@@ -139,6 +142,17 @@ def _d(a):
 
 def _i(a):
     return None if a is None else a.ctypes.data_as(I)
+
+
+def _contact_args(num, lst, hp, wnum, wlst, hw):
+    f = lambda a, t: np.ascontiguousarray(a, t)  # noqa: E731
+    i32, f64 = np.int32, np.float64
+    return [f(num, i32), f(lst, i32), f(hp["dispt"], f64), f(hp["nDsp"], f64),
+            f(hp["nDsp_mx"], f64), f(hp["plstRad"], f64), f(hp["plstDfm"], f64),
+            f(hp["stage"], i32), f(hp["lqdBrg"], i32), f(wnum, i32), f(wlst, i32),
+            f(hw["dispt"], f64), f(hw["nDsp"], f64), f(hw["nDsp_mx"], f64),
+            f(hw["plstRad"], f64), f(hw["plstDfm"], f64), f(hw["stage"], i32),
+            f(hw["lqdBrg"], i32)]
 
 
 def as_input(inp):
@@ -288,6 +302,17 @@ class Oracle:
         lib().orc_get_hist_pw(self.st, _d(d), *[_d(x) for x in f], _i(s), _i(b))
         return dict(dispt=d, nDsp=f[0], nDsp_mx=f[1], plstRad=f[2], plstDfm=f[3], stage=s,
                     lqdBrg=b)
+
+    def set_contacts(self, num, lst, hp, wnum, wlst, hw):
+        """Restart state (lists, history) as the getters return it."""
+        c = _contact_args(num, lst, hp, wnum, wlst, hw)
+        if lib().orc_set_contacts(self.st, lst.shape[1], *[_i(a) if a.dtype == np.int32 else _d(a)
+                                                           for a in c]) != 0:
+            raise ValueError("a list is longer than its size level allows")
+
+    def set_lqdcnt(self, a):
+        a = np.ascontiguousarray(a, np.int32)
+        lib().orc_set_lqdcnt(self.st, _i(a))
 
     def hash_index(self):
         n = self.total

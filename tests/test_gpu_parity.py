@@ -398,13 +398,14 @@ def test_overflow_is_reported_not_written():
     assert e.iter == it + 2
 
 
-def test_dump_load_round_trip(tmp_path):
+@pytest.mark.parametrize("binary", [False, True])
+def test_dump_load_round_trip(tmp_path, binary):
     w = W.small_bed(n=2000)
     a = G.Engine(w["inp"])
     a.set_particles(w["pos"])
     a.step(200)
     path = tmp_path / "heyi_dump.dat"
-    a.dump(path, nBatch=4)
+    a.dump(path, nBatch=4, binary=binary)
     assert G.load_para(path) == (2000, 4, 2000)
     b = G.Engine(w["inp"])
     assert b.load(path) == 4
@@ -416,6 +417,18 @@ def test_dump_load_round_trip(tmp_path):
     b.step(100)
     for u, v in zip(a.particles(), b.particles()):
         assert np.array_equal(u, v)
+    # and the oracle restarted from the same dump's state agrees bit for bit
+    if binary:
+        o = O.Oracle(w["inp"])
+        c = G.Engine(w["inp"])
+        c.load(path)
+        o.set_particles(*c.particles())
+        o.set_contacts(*c.nebor(), c.hist_pp(), *c.walls(), c.hist_pw())
+        o.set_lqdcnt(c.lqdcnt())
+        c.step(100)
+        o.step(100)
+        for u, v in zip(c.particles(), o.particles()):
+            assert np.array_equal(u, v)
 
 
 def test_contact_stats_match_oracle_count():

@@ -200,6 +200,28 @@ No
         G.read_deck(bad)
 
 
+def test_run_log_format(tmp_path):
+    # main.cpp:93-139, 591-618, 631-634
+    from gpucode_amd import _capi as A
+    path = str(tmp_path / "heyi_Log.dat").encode()
+    L = A.lib()
+    assert L.dem_log_header(path, b"heyi", 0, 2000, 1, 500) == 0
+    assert L.dem_log_output(path, 0, 0.0125, 812.5, 3, 500, 2000, 12500) == 0
+    assert L.dem_log_output(path, 1, 0.025, 1600.25, 1, 2000, 2000, 25000) == 0
+    assert L.dem_log_total(path, 1.6) == 0
+    t = open(path.decode()).read()
+    assert t.startswith("==================|*--- PROJECT---*|===================\n<File>: new\n<Time>: ")
+    assert "<Name>: heyi\n" in t and "<Particle>: 2000 \n" in t
+    assert ("0    \truntime:0.0125 \telapsed:812.50000           ms\trebuild:3\t"
+            "particle:500\tIterate:12500\n") in t
+    assert ("1    \truntime:0.0250 \telapsed:1600.25000          ms\trebuild:1\t"
+            "Iterate:25000\n") in t
+    assert t.endswith("\nGPU DEM Time used: 1.60         s \n\n")
+    assert L.dem_log_header(path, b"heyi", 1, 2000, 3, 1500) == 0
+    t = open(path.decode()).read()
+    assert "<File>: restarted" in t and "<Feed batch>: 3 \t<Current particle>: 1500 \n" in t
+
+
 def test_calibration_round_trip():
     from gpucode_amd.calibrate import collision_restitution, restitution_to_dmpn
     d = restitution_to_dmpn(0.5, 1e-3, 2500, 70e9, 0.25, v=0.5)
