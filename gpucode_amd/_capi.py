@@ -177,6 +177,7 @@ _SIGS = {
     "sur_get_params": (C.c_int, [P, F]),
     "sur_set_params": (C.c_int, [P, F]),
     "sur_forward": (C.c_int, [P, C.c_int, F, F]),
+    "sur_forward_noalloc": (C.c_int, [P, C.c_int, C.c_void_p, C.c_void_p]),
     "sur_loss_grad": (C.c_int, [P, C.c_int, F, F, F, F]),
     "sur_train": (C.c_int, [P, C.c_int, F, F, C.c_int, C.c_int, C.c_float, C.c_ulonglong, F]),
 }

@@ -13,6 +13,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <mutex>
 #include <vector>
 
 #include "../../include/dem_capi.h"
@@ -164,7 +165,40 @@ struct sur_mlp {
     int *idx = nullptr;
     int tcap = 0;
     long step = 0;
+    unsigned char *scratch = nullptr;  // this GPU's copy of `scratch` (below)
 };
+
+// Activations of sur_forward_noalloc: a fixed array in device memory, one
+// copy per GPU the library runs on, held until the process ends.  Rows of
+// at most SUR_NOALLOC_MAX_ROWS inputs: the BF16 inputs (16 B per row), then
+// two BF16 activation buffers that the hidden layers use in turn (256 B per
+// row each).  Every element is written before it is read, so nothing is
+// cleared; every call on a GPU uses the same array, so calls that overlap on
+// one GPU race.
+static const size_t kNoAllocXb = (size_t)SUR_NOALLOC_MAX_ROWS * SUR_IN * sizeof(bf16);
+static const size_t kNoAllocH = (size_t)SUR_NOALLOC_MAX_ROWS * SUR_HIDDEN * sizeof(bf16);
+static __device__ __align__(256) unsigned char scratch[kNoAllocXb + 2 * kNoAllocH];
+
+// Device address of `scratch` on each GPU, looked up once per device with
+// that device current (the array's name is a host-side stand-in, not a
+// device address).
+static const int kMaxDevices = 64;
+static unsigned char *g_scratch_dev[kMaxDevices];
+static std::mutex g_scratch_mu;
+
+static int scratch_on_current_device(unsigned char **out) {
+    int dev = -1;
+    SCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return dem_fail(DEM_ERR_ARG, "device %d out of range", dev);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void *p = nullptr;
+        SCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_scratch_dev[dev] = (unsigned char *)p;
+    }
+    *out = g_scratch_dev[dev];
+    return DEM_OK;
+}
 
 static void free_rows(sur_mlp *s) {
     (void)hipFree(s->X); (void)hipFree(s->T); (void)hipFree(s->Y); (void)hipFree(s->dY);
@@ -198,11 +232,21 @@ static int ensure_rows(sur_mlp *s, int n) {
     return DEM_OK;
 }
 
-// forward pass of n rows already in s->X (FP32): Xb, Z[l], Hb[l], Y
-static void forward(sur_mlp *s, int n) {
-    hipLaunchKernelGGL(k_to_bf16, dim3(blocks((long)n * SUR_IN, 256)), dim3(256), 0, s->stream, s->X,
-                       s->Xb, (long)n * SUR_IN);
-    const bf16 *in = s->Xb;
+// Buffers of one forward pass: input rows X (FP32), their BF16 copy Xb,
+// the pre-activations Z[l] (FP32, may be null when no backward pass
+// follows), the SiLU activations Hb[l] (BF16) and the outputs Y.
+struct FwdBufs {
+    const float *X;
+    bf16 *Xb;
+    float *Z[3];
+    bf16 *Hb[3];
+    float *Y;
+};
+
+static void forward_into(const sur_mlp *s, const FwdBufs &b, int n, hipStream_t st) {
+    hipLaunchKernelGGL(k_to_bf16, dim3(blocks((long)n * SUR_IN, 256)), dim3(256), 0, st, b.X, b.Xb,
+                       (long)n * SUR_IN);
+    const bf16 *in = b.Xb;
     int K = SUR_IN;
     const int wo[3] = {kW1, kW2, kW3}, bo[3] = {kB1, kB2, kB3};
     for (int l = 0; l < 3; ++l) {
@@ -210,18 +254,24 @@ static void forward(sur_mlp *s, int n) {
         g.A = in; g.as0 = K; g.as1 = 1;
         g.B = s->pb + wo[l]; g.bs0 = 1; g.bs1 = K;  // B(k,j) = W[j][k]
         g.M = n; g.N = H; g.K = K;
-        g.C = s->Z[l]; g.cs0 = H; g.bias = s->p + bo[l];
-        g.Hs = s->Hb[l]; g.hs0 = H;
-        gemm(g, s->stream);
-        in = s->Hb[l];
+        g.C = b.Z[l]; g.cs0 = H; g.bias = s->p + bo[l];
+        g.Hs = b.Hb[l]; g.hs0 = H;
+        gemm(g, st);
+        in = b.Hb[l];
         K = H;
     }
     GemmArgs g = {};
     g.A = in; g.as0 = H; g.as1 = 1;
     g.B = s->pb + kW4; g.bs0 = 1; g.bs1 = H;
     g.M = n; g.N = SUR_OUT; g.K = H;
-    g.C = s->Y; g.cs0 = SUR_OUT; g.bias = s->p + kB4;
-    gemm(g, s->stream);
+    g.C = b.Y; g.cs0 = SUR_OUT; g.bias = s->p + kB4;
+    gemm(g, st);
+}
+
+// forward pass of n rows already in s->X (FP32): Xb, Z[l], Hb[l], Y
+static void forward(sur_mlp *s, int n) {
+    FwdBufs b = {s->X, s->Xb, {s->Z[0], s->Z[1], s->Z[2]}, {s->Hb[0], s->Hb[1], s->Hb[2]}, s->Y};
+    forward_into(s, b, n, s->stream);
 }
 
 // backward pass after forward(n) with targets in s->T: gradients into s->g
@@ -319,6 +369,12 @@ extern "C" int sur_create(unsigned long long seed, sur_mlp **out) {
     if (!out) return dem_fail(DEM_ERR_ARG, "null argument");
     *out = nullptr;
     sur_mlp *s = new sur_mlp();
+    // the network lives on the current device; so does its scratch array
+    int src = scratch_on_current_device(&s->scratch);
+    if (src) {
+        delete s;
+        return src;
+    }
     hipError_t e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipMalloc((void **)&s->p, kNP * sizeof(float));
     if (e == hipSuccess) e = hipMalloc((void **)&s->g, kNP * sizeof(float));
@@ -355,6 +411,23 @@ extern "C" int sur_forward(sur_mlp *s, int n, const float *x, float *y) {
     forward(s, n);
     SCHK(hipGetLastError());
     SCHK(hipMemcpyAsync(y, s->Y, (size_t)n * SUR_OUT * sizeof(float), hipMemcpyDeviceToHost, s->stream));
+    SCHK(hipStreamSynchronize(s->stream));
+    return DEM_OK;
+}
+
+extern "C" int sur_forward_noalloc(sur_mlp *s, int n, const float *x, float *y) {
+    if (!s || n < 0 || (n > 0 && (!x || !y))) return dem_fail(DEM_ERR_ARG, "bad argument");
+    if (n > SUR_NOALLOC_MAX_ROWS)
+        return dem_fail(DEM_ERR_ARG, "sur_forward_noalloc: %d rows, at most %d per call", n,
+                        SUR_NOALLOC_MAX_ROWS);
+    if (n == 0) return DEM_OK;
+    // the device address found at sur_create, never the array's name
+    unsigned char *base = s->scratch;
+    bf16 *Xb = (bf16 *)base;
+    bf16 *H0 = (bf16 *)(base + kNoAllocXb), *H1 = (bf16 *)(base + kNoAllocXb + kNoAllocH);
+    FwdBufs b = {x, Xb, {nullptr, nullptr, nullptr}, {H0, H1, H0}, y};
+    forward_into(s, b, n, s->stream);
+    SCHK(hipGetLastError());
     SCHK(hipStreamSynchronize(s->stream));
     return DEM_OK;
 }

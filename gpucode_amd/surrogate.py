@@ -48,6 +48,11 @@ class MLP:
         check(A.lib().sur_forward(self._m, len(x), fptr(x), fptr(y)))
         return y
 
+    def forward_noalloc(self, n, x_dev, y_dev):
+        """sur_forward_noalloc: x_dev (n x 8 FP32) and y_dev (n x 2 FP32) are
+        device addresses (ints) on the network's GPU."""
+        check(A.lib().sur_forward_noalloc(self._m, int(n), C.c_void_p(x_dev), C.c_void_p(y_dev)))
+
     def loss_grad(self, x, t):
         x = np.ascontiguousarray(x, np.float32)
         t = np.ascontiguousarray(t, np.float32)

@@ -39,6 +39,18 @@ int sur_get_params(sur_mlp *m, float *params);
 int sur_set_params(sur_mlp *m, const float *params);
 /* y[n][2] = f(x[n][8]) with the BF16 weights. */
 int sur_forward(sur_mlp *m, int n, const float *x, float *y);
+
+/* Allocation-free sur_forward, for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops of a sweep's
+ * screening).  No counterpart in the reference.  x[n][8] and y[n][2] are
+ * DEVICE pointers on the network's GPU; y equals sur_forward's output bit
+ * for bit.  At most SUR_NOALLOC_MAX_ROWS rows per call (larger n gives
+ * DEM_ERR_ARG).  The activations live in a fixed array of the library in
+ * device memory (4.3 MB per GPU, held until the process ends) that every
+ * call on that GPU shares: calls that overlap on one GPU race.  Runs on the
+ * network's stream and returns once y is written. */
+#define SUR_NOALLOC_MAX_ROWS 8192
+int sur_forward_noalloc(sur_mlp *m, int n, const float *x, float *y);
 /* Mean squared error over the n x 2 outputs against t[n][2], and its
  * gradient with respect to the FP32 parameters (grad may be NULL). */
 int sur_loss_grad(sur_mlp *m, int n, const float *x, const float *t, float *loss, float *grad);
