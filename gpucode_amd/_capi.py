@@ -151,6 +151,8 @@ _SIGS = {
                                  C.c_int, C.c_int]),
     "dem_log_total": (C.c_int, [C.c_char_p, C.c_double]),
     "dem_key_bits": (C.c_int, [P]),
+    "dem_set_neighbor_search": (C.c_int, [P, C.c_int]),
+    "dem_neighbor_search_used": (C.c_int, [P]),
     "dem_scale_velocities": (C.c_int, [P, C.c_double]),
     "dem_dump_write": (C.c_int, [P, C.c_char_p, C.c_int]),
     "dem_dump_write_binary": (C.c_int, [P, C.c_char_p, C.c_int]),
@@ -164,6 +166,8 @@ _SIGS = {
     "dem_batch_size": (C.c_int, [P]),
     "dem_batch_instance": (P, [P, C.c_int]),
     "dem_batch_step": (C.c_int, [P, C.c_int]),
+    "dem_batch_set_schedule": (C.c_int, [P, C.c_int, C.c_int]),
+    "dem_batch_schedule_stats": (C.c_int, [P, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]),
     "dem_batch_timing": (C.c_int, [P, D, C.POINTER(C.c_longlong), D, C.POINTER(C.c_longlong)]),
     "dem_batch_analyze": (C.c_int, [P, D, D, D]),
     "dem_analyze": (C.c_int, [P, D, D, D]),

@@ -197,6 +197,8 @@ static void free_all(dem_ctx *c) {
     (void)hipFree(c->stop_at); (void)hipFree(c->overflow); (void)hipFree(c->upd_count);
     for (hipEvent_t &e : c->rb_ev)
         if (e) (void)hipEventDestroy(e);
+    for (hipEvent_t &e : c->nf_ev)
+        if (e) (void)hipEventDestroy(e);
     (void)hipFree(c->stats);
 }
 

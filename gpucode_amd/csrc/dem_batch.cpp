@@ -2,14 +2,22 @@
 //
 // The reference runs one packing per process: its constants are process
 // globals (kernel.cu:15-19, heydem.h:196-204) and main.cpp:425-623 steps
-// that one packing.  A batch holds n ordinary contexts on one stream.  Per
-// chunk of steps it uploads one table entry per instance (buffers, lists,
-// step limit, parities) and launches one step kernel per step for all dry
-// instances (per-wave compaction) and one for all others (one thread per
-// particle); the blocks of each launch are tasks (instance, block) listed
-// largest instance first.  Rebuild requests stay per instance: an instance
-// whose particles passed the path-length threshold skips the rest of the
-// chunk and is rebuilt before the next one, exactly as dem_step does.
+// that one packing.  A batch holds n ordinary contexts on one stream and
+// launches one step kernel per step for all dry instances (per-wave
+// compaction) and one for all others (one thread per particle); the blocks
+// of each launch are tasks (instance, block) listed largest instance first.
+// Rebuild requests stay per instance, and each instance takes exactly the
+// steps and rebuilds dem_step would give it.
+//
+// Two schedules.  The device-driven one (the default, for instances of at
+// most 65,536 particles) keeps every instance's state in device tables: the
+// host enqueues blocks of slot_every steps and, after each block, a rebuild
+// slot on a second stream that rebuilds the instances that asked while the
+// next block steps the others; the rebuilt instances resume after that
+// block.  The host reads nothing back until the run ends (apart from a
+// look every few dozen blocks to see whether every instance is through).
+// The host-chunked one runs chunks of steps, reads the rebuild requests
+// back, rebuilds on the host's decision and runs the next chunk.
 #include <limits.h>
 #include <string.h>
 
@@ -46,10 +54,22 @@ struct dem_batch {
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
     double step_ms = 0.0, rebuild_ms = 0.0;
     long long launches = 0, n_rebuild = 0;
+    // device-driven schedule (on by default): steps between rebuild slots
+    bool device_schedule = true;
+    int slot_every = 4;
+    DBatchInst *d_dins = nullptr, *h_dins = nullptr;  // h_dins pinned
+    DBatchCtl *d_ctl = nullptr, *h_ctl = nullptr;     // h_ctl pinned
+    DBatchCtl *d_nxt = nullptr;                       // control records a slot hands to kb_commit
+    hipStream_t rstream = nullptr;                    // rebuild slots, beside the steps
+    hipEvent_t evS[4] = {}, evR[4] = {};              // step-side and slot-side marks, a ring
+    int2 *d_stask = nullptr;                          // static rebuild tasks: particles, then cells
+    int nstask_p = 0, nstask_c = 0;
+    long long slots = 0, host_syncs = 0;
 };
 
 static void batch_free(dem_batch *b) {
     if (b->stream) (void)hipStreamSynchronize(b->stream);
+    if (b->rstream) (void)hipStreamSynchronize(b->rstream);
     for (dem_ctx *c : b->inst) dem_destroy(c);
     (void)hipFree(b->d_inst);
     (void)hipFree(b->d_tab);
@@ -60,6 +80,12 @@ static void batch_free(dem_batch *b) {
     (void)hipFree(b->d_tasks);
     (void)hipFree(b->d_rb);
     (void)hipFree(b->d_rtask);
+    (void)hipFree(b->d_dins);
+    (void)hipFree(b->d_ctl);
+    (void)hipFree(b->d_nxt);
+    (void)hipFree(b->d_stask);
+    (void)hipHostFree(b->h_dins);
+    (void)hipHostFree(b->h_ctl);
     (void)hipHostFree(b->h_rb);
     (void)hipHostFree(b->h_tab);
     (void)hipHostFree(b->h_ana);
@@ -69,6 +95,11 @@ static void batch_free(dem_batch *b) {
     if (b->ev0) (void)hipEventDestroy(b->ev0);
     if (b->ev1) (void)hipEventDestroy(b->ev1);
     if (b->ev2) (void)hipEventDestroy(b->ev2);
+    for (int q = 0; q < 4; ++q) {
+        if (b->evS[q]) (void)hipEventDestroy(b->evS[q]);
+        if (b->evR[q]) (void)hipEventDestroy(b->evR[q]);
+    }
+    if (b->rstream) (void)hipStreamDestroy(b->rstream);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
 }
@@ -105,9 +136,19 @@ extern "C" int dem_batch_create(int n, const dem_input *in, const double *const 
         if (e == hipSuccess) e = hipHostMalloc((void **)&b->h_ovf, un * sizeof(int), 0);
         if (e == hipSuccess) e = hipMalloc((void **)&b->d_rb, un * sizeof(RbInst));
         if (e == hipSuccess) e = hipHostMalloc((void **)&b->h_rb, un * sizeof(RbInst), 0);
+        if (e == hipSuccess) e = hipMalloc((void **)&b->d_dins, un * sizeof(DBatchInst));
+        if (e == hipSuccess) e = hipMalloc((void **)&b->d_ctl, un * sizeof(DBatchCtl));
+        if (e == hipSuccess) e = hipHostMalloc((void **)&b->h_dins, un * sizeof(DBatchInst), 0);
+        if (e == hipSuccess) e = hipHostMalloc((void **)&b->h_ctl, un * sizeof(DBatchCtl), 0);
         if (e == hipSuccess) e = hipEventCreate(&b->ev0);
         if (e == hipSuccess) e = hipEventCreate(&b->ev1);
         if (e == hipSuccess) e = hipEventCreate(&b->ev2);
+        if (e == hipSuccess) e = hipMalloc((void **)&b->d_nxt, un * sizeof(DBatchCtl));
+        if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->rstream, hipStreamNonBlocking);
+        for (int q = 0; q < 4 && e == hipSuccess; ++q) {
+            e = hipEventCreateWithFlags(&b->evS[q], hipEventDisableTiming);
+            if (e == hipSuccess) e = hipEventCreateWithFlags(&b->evR[q], hipEventDisableTiming);
+        }
         if (e != hipSuccess) rc = dem_fail(DEM_ERR_HIP, "batch allocation: %s", hipGetErrorString(e));
     }
     if (rc != DEM_OK) {
@@ -297,6 +338,210 @@ static int batch_rebuilds(dem_batch *b, const std::vector<int> &left) {
     return DEM_OK;
 }
 
+// ---------------------------------------------------------------------
+// Device-driven schedule: after the rebuilds due at the start (host path,
+// one synchronisation), the host enqueues slot_every step launches and one
+// rebuild slot at a time and never waits for them; the kernels keep every
+// instance's segment (buffer parities, steps left, rebuild request).  An
+// instance that asks for a rebuild idles until the next slot, at most
+// slot_every - 1 launches.  Every 64 slots the host reads the instances'
+// state back to see whether all are done.  Results are those of dem_step
+// on each instance: same steps, same rebuild points.
+// ---------------------------------------------------------------------
+static bool device_schedule_applies(dem_batch *b) {
+    if (!b->device_schedule) return false;
+    for (dem_ctx *c : b->inst)
+        if (c->total < 1 || c->total > kBatchRebuildMax) return false;
+    return true;
+}
+
+static int static_rebuild_tasks(dem_batch *b) {
+    std::vector<int2> t;
+    for (int k = 0; k < (int)b->inst.size(); ++k)
+        for (int q = 0; q < (b->inst[k]->total + 255) / 256; ++q) t.push_back(make_int2(k, q));
+    const int np = (int)t.size();
+    for (int k = 0; k < (int)b->inst.size(); ++k)
+        for (int q = 0; q < (b->inst[k]->S.sysNCells + 255) / 256; ++q) t.push_back(make_int2(k, q));
+    (void)hipFree(b->d_stask);
+    b->d_stask = nullptr;
+    HIPCHK(hipMalloc((void **)&b->d_stask, t.size() * sizeof(int2)));
+    HIPCHK(hipMemcpyAsync(b->d_stask, t.data(), t.size() * sizeof(int2), hipMemcpyHostToDevice,
+                          b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));  // t goes out of scope
+    b->nstask_p = np;
+    b->nstask_c = (int)t.size() - np;
+    return DEM_OK;
+}
+
+static int batch_step_device(dem_batch *b, int nsteps) {
+    const int n = (int)b->inst.size();
+    const bool mixed = b->precision == DEM_PREC_MIXED;
+    int rc;
+    std::vector<int> left(n, nsteps);
+    if ((rc = batch_rebuilds(b, left)) || (rc = build_tasks(b)) || (rc = static_rebuild_tasks(b)))
+        return rc;
+    for (int k = 0; k < n; ++k) {
+        dem_ctx *c = b->inst[k];
+        DBatchInst &D = b->h_dins[k];
+        memset(&D, 0, sizeof(D));
+        for (int q = 0; q < 2; ++q) {
+            D.P4[q] = c->P4[q]; D.VW[q] = c->VW[q]; D.lq[q] = c->lq[q];
+            D.level[q] = c->level[q]; D.id[q] = c->id[q];
+            D.nbr[q] = c->nbr[q]; D.nnb[q] = c->nnb[q]; D.hp[q] = c->hp[q];
+            D.amask[q] = c->amask[q]; D.actoff[q] = c->actoff[q];
+            D.wl[q] = c->wl[q]; D.nwl[q] = c->nwl[q]; D.hw[q] = c->hw[q];
+        }
+        D.keys = c->keys; D.vals = c->vals; D.keys2 = c->keys2; D.vals2 = c->vals2;
+        D.cstart = c->cstart; D.cend = c->cend; D.nmask = c->nmask;
+        D.overflow = c->overflow;
+        D.stop_at = b->d_stop + k;
+        D.cap = c->cap; D.total = c->total; D.K = c->K; D.ncells = c->S.sysNCells;
+        D.passes = (c->key_bits + 7) / 8 < 1 ? 1 : (c->key_bits + 7) / 8;
+        D.dTw = c->S.topv * c->S.dt;
+        DBatchCtl &C = b->h_ctl[k];
+        memset(&C, 0, sizeof(C));
+        C.cur = c->cur; C.Lidx = c->Lidx; C.pcur = c->pcur; C.hcur = c->hcur; C.hl = c->hl;
+        C.base = 0;
+        C.left = nsteps;
+        C.Tw = c->Tw;
+    }
+    HIPCHK(hipMemcpyAsync(b->d_dins, b->h_dins, (size_t)n * sizeof(DBatchInst), hipMemcpyHostToDevice,
+                          b->stream));
+    HIPCHK(hipMemcpyAsync(b->d_ctl, b->h_ctl, (size_t)n * sizeof(DBatchCtl), hipMemcpyHostToDevice,
+                          b->stream));
+    launch_fill_i32(b->d_stop, INT_MAX, n, b->stream);
+    HIPCHK(hipEventRecord(b->ev0, b->stream));
+    // Blocks of R global steps on the step stream.  After block j the
+    // rebuild slot j runs on the rebuild stream while block j + 1 steps the
+    // other instances; then the step stream waits for the slot, kb_commit
+    // hands the rebuilt instances their new control records, and they
+    // resume with block j + 2.  Nothing returns to the host in between; it
+    // looks at the control records only to see whether every instance is
+    // through (every 16 blocks once nsteps global steps have run).
+    const int R = b->slot_every < 1 ? 1 : b->slot_every;
+    // an instance that rebuilds after every step takes 2R global steps per step
+    const long long gmax = ((long long)nsteps + 1) * 2 * R;
+    long long G = 0, slots = 0, blocks = 0, last_check = 0;
+    int pend = -1;
+    bool done = nsteps == 0;
+    while (!done) {
+        for (int r = 0; r < R; ++r, ++G) {
+            launch_step_dbatch(b->d_inst, b->d_dins, b->d_ctl, b->d_tasks, b->ntask_wave, true, mixed,
+                               (int)G, b->stream);
+            launch_step_dbatch(b->d_inst, b->d_dins, b->d_ctl, b->d_tasks + b->ntask_wave,
+                               b->ntask_loop, false, mixed, (int)G, b->stream);
+        }
+        if (pend >= 0) {
+            HIPCHK(hipStreamWaitEvent(b->stream, b->evR[pend], 0));
+            launch_commit_slot(b->d_dins, b->d_ctl, b->d_nxt, n, b->stream);
+            pend = -1;
+        }
+        HIPCHK(hipGetLastError());
+        ++blocks;
+        if ((G >= nsteps && blocks - last_check >= 16) || blocks - last_check >= 256 || G >= gmax) {
+            last_check = blocks;
+            HIPCHK(hipMemcpyAsync(b->h_ctl, b->d_ctl, (size_t)n * sizeof(DBatchCtl),
+                                  hipMemcpyDeviceToHost, b->stream));
+            HIPCHK(hipMemcpyAsync(b->h_stop, b->d_stop, (size_t)n * sizeof(int), hipMemcpyDeviceToHost,
+                                  b->stream));
+            HIPCHK(hipStreamSynchronize(b->stream));
+            ++b->host_syncs;
+            done = true;
+            for (int k = 0; k < n && done; ++k) {
+                const DBatchCtl &C = b->h_ctl[k];
+                const int seen = (int)G - C.base, st = b->h_stop[k];
+                int seg = st < seen ? st : seen;
+                seg = seg < C.left ? seg : C.left;
+                // still running, or rebuilding at the next slot
+                if (!C.failed && seg < C.left) done = false;
+            }
+            if (done) break;
+            if (G >= gmax) return dem_fail(DEM_ERR_STATE, "batched run made no progress");
+        }
+        const int q = (int)(slots & 3);
+        HIPCHK(hipEventRecord(b->evS[q], b->stream));
+        HIPCHK(hipStreamWaitEvent(b->rstream, b->evS[q], 0));
+        launch_rebuild_slot(b->d_inst, b->d_dins, b->d_ctl, b->d_nxt, b->d_rb, n, (int)G - 1,
+                            (int)G + R, b->d_stask, b->nstask_p, b->d_stask + b->nstask_p, b->nstask_c,
+                            b->rstream);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipEventRecord(b->evR[q], b->rstream));
+        pend = q;
+        ++slots;
+    }
+    HIPCHK(hipEventRecord(b->ev1, b->stream));
+    HIPCHK(hipMemcpyAsync(b->h_ctl, b->d_ctl, (size_t)n * sizeof(DBatchCtl), hipMemcpyDeviceToHost,
+                          b->stream));
+    HIPCHK(hipMemcpyAsync(b->h_stop, b->d_stop, (size_t)n * sizeof(int), hipMemcpyDeviceToHost,
+                          b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+    b->step_ms += ms;
+    b->launches += 2LL * G;
+    b->slots += slots;
+    int first_bad = -1;
+    for (int k = 0; k < n; ++k) {
+        dem_ctx *c = b->inst[k];
+        const DBatchCtl &C = b->h_ctl[k];
+        int seen = (int)G - C.base;
+        seen = seen < 0 ? 0 : seen;
+        const int st = b->h_stop[k];
+        int seg = st < seen ? st : seen;
+        seg = seg < C.left ? seg : C.left;
+        if (C.failed) seg = 0;  // halted at the rebuild that overflowed
+        const int ex = C.executed + seg;
+        c->cur = (C.cur + seg) & 1;
+        c->Lidx = (C.Lidx + seg) & 1;
+        c->pcur = C.pcur;
+        c->hcur = C.hcur;
+        c->hl = C.hl;
+        double tw = C.Tw;
+        for (int q = 0; q < seg; ++q) tw = tw + c->S.topv * c->S.dt;
+        c->Tw = tw;
+        for (int q = 0; q < ex; ++q) c->ctime = c->ctime + c->S.dt;
+        c->iter += ex;
+        c->fused_steps += ex;
+        if (C.nreb > 0) {
+            c->n_rebuild += C.nreb;
+            b->n_rebuild += C.nreb;
+            c->rebuild_interval = C.seg_steps;
+            c->iter_at_rebuild = c->iter - seg;
+            c->saved = false;
+            c->has_lists = true;
+            c->old_total = c->total;
+            c->layout_age = seg;
+        } else {
+            c->layout_age += ex;
+        }
+        c->forces_valid = false;
+        c->rebuild_pending = st <= seen;  // asked in its last segment, no step left to rebuild for
+        if (C.failed > 0) {
+            c->lists_overflow = C.failed;
+            if (first_bad < 0) first_bad = k;
+        }
+    }
+    if (first_bad >= 0)
+        return dem_fail(DEM_ERR_OVERFLOW,
+                        "instance %d: a neighbour list needed %d slots, more than its size level "
+                        "allows", first_bad, b->inst[first_bad]->lists_overflow);
+    return DEM_OK;
+}
+
+extern "C" int dem_batch_set_schedule(dem_batch *b, int device, int slot_every) {
+    if (!b || slot_every < 1) return dem_fail(DEM_ERR_ARG, "bad argument");
+    b->device_schedule = device != 0;
+    b->slot_every = slot_every;
+    return DEM_OK;
+}
+
+extern "C" int dem_batch_schedule_stats(dem_batch *b, long long *slots, long long *host_syncs) {
+    if (!b) return dem_fail(DEM_ERR_ARG, "null batch");
+    if (slots) *slots = b->slots;
+    if (host_syncs) *host_syncs = b->host_syncs;
+    return DEM_OK;
+}
+
 extern "C" int dem_batch_step(dem_batch *b, int nsteps) {
     if (!b || nsteps < 0) return dem_fail(DEM_ERR_ARG, "bad argument");
     const int n = (int)b->inst.size();
@@ -311,6 +556,7 @@ extern "C" int dem_batch_step(dem_batch *b, int nsteps) {
         if ((rc = dem_pair_unpack(c))) return rc;
     }
     if ((rc = upload_insts(b))) return rc;
+    if (device_schedule_applies(b)) return batch_step_device(b, nsteps);
     std::vector<int> left(n, nsteps);
     const bool mixed = b->precision == DEM_PREC_MIXED;
     // chunk length: an instance that requested a rebuild idles for the rest

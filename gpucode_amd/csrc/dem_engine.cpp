@@ -54,6 +54,16 @@ static int check_overflow(dem_ctx *c) {
     int ov = 0;
     HIPCHK(hipMemcpyAsync(&ov, c->overflow, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
+    if (c->nf_pending) {  // a timed neighbour search of this rebuild
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, c->nf_ev[0], c->nf_ev[1]));
+        c->nf_ms[c->nf_timed] = ms;
+        c->nf_pending = false;
+        if (++c->nf_trials >= 2) {
+            c->nf_choice = c->nf_ms[1] <= c->nf_ms[0] ? 1 : 0;
+            c->nf_since = 0;
+        }
+    }
     c->lists_overflow = ov > 0 ? ov : 0;
     return dem_lists_ok(c);
 }
@@ -657,11 +667,35 @@ extern "C" int dem_bld_nebor_list(dem_ctx *c) {
                       c->cend, c->S.sysNCells, n, s);
     rb_mark(c, 5);
     launch_fill_i32(c->overflow, 0, 1, s);
-    // the tile list of the LDS-staged search lives in vals2 / keys2, free
-    // between the sort and the compact-history layout
+    // the LDS-staged (tiled) search pays where tiles hold many particles:
+    // under AUTO each context times both forms and keeps the faster
+    bool tiled = false;
+    const bool can_tile = neighbor_tiles_apply(c->P);
+    bool timing = false;
+    if (can_tile) {
+        if (c->nf_mode == DEM_NSEARCH_TILED) {
+            tiled = true;
+        } else if (c->nf_mode == DEM_NSEARCH_AUTO) {
+            if (c->nf_choice >= 0 && ++c->nf_since >= 256) {
+                c->nf_choice = -1;  // conditions change (a pour settles): time both again
+                c->nf_trials = 0;
+            }
+            tiled = c->nf_choice >= 0 ? c->nf_choice == 1 : (c->nf_trials & 1) == 0;
+            timing = c->nf_choice < 0;
+        }
+    }
+    if (timing) {
+        for (hipEvent_t &e : c->nf_ev)
+            if (!e) HIPCHK(hipEventCreate(&e));
+        HIPCHK(hipEventRecord(c->nf_ev[0], s));
+    }
     launch_neighbor_find(c->buf(c->cur, c->Lidx), c->level[c->pcur], c->P, c->keys, c->cstart,
-                         c->cend, c->nbr[c->hcur], c->nnb[c->hcur], c->overflow, (int *)c->vals2,
-                         (int *)c->keys2, c->cap, n, s);
+                         c->cend, c->nbr[c->hcur], c->nnb[c->hcur], c->overflow, tiled, c->cap, n, s);
+    if (timing) {
+        HIPCHK(hipEventRecord(c->nf_ev[1], s));
+        c->nf_timed = tiled ? 1 : 0;
+        c->nf_pending = true;
+    }
     rb_mark(c, 6);
     launch_active_slots(c->P4[c->cur], c->nbr[c->hcur], c->nnb[c->hcur], c->amask[c->hl],
                         c->actoff[c->hl], c->nmask, near_gap_of(c->P), c->keys2, c->sort_tmp,
@@ -740,6 +774,23 @@ extern "C" int dem_rebuild(dem_ctx *c) {
 }
 
 extern "C" int dem_key_bits(dem_ctx *c) { return c ? c->key_bits : 0; }
+
+extern "C" int dem_set_neighbor_search(dem_ctx *c, int mode) {
+    if (!c || mode < DEM_NSEARCH_AUTO || mode > DEM_NSEARCH_TILED)
+        return dem_fail(DEM_ERR_ARG, "bad argument");
+    c->nf_mode = mode;
+    c->nf_choice = -1;
+    c->nf_trials = 0;
+    c->nf_pending = false;
+    return DEM_OK;
+}
+
+extern "C" int dem_neighbor_search_used(dem_ctx *c) {
+    if (!c || !neighbor_tiles_apply(c->P) || c->nf_mode == DEM_NSEARCH_PLAIN) return DEM_NSEARCH_PLAIN;
+    if (c->nf_mode == DEM_NSEARCH_TILED) return DEM_NSEARCH_TILED;
+    return c->nf_choice == 1 ? DEM_NSEARCH_TILED : c->nf_choice == 0 ? DEM_NSEARCH_PLAIN
+                                                                      : DEM_NSEARCH_AUTO;
+}
 
 extern "C" int dem_scale_velocities(dem_ctx *c, double factor) {
     if (!c || !(factor >= 0.0)) return dem_fail(DEM_ERR_ARG, "bad argument");
@@ -1034,7 +1085,7 @@ extern "C" int dem_stream_copy(long long nbytes, int iters, double *gbps) {
         }
         // the best of a few launch shapes (blocks per CU) is the attainable rate
         double best = 0.0;
-        for (int bpc : {2, 4, 8, 16}) {
+        for (int bpc : {16, 32, 64, 128}) {
             launch_stream_copy(a, b, nbytes, bpc, s);  // warm
             (void)hipEventRecord(e0, s);
             for (int k = 0; k < iters; ++k) launch_stream_copy(a, b, nbytes, bpc, s);

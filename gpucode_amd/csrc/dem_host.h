@@ -107,6 +107,13 @@ struct dem_ctx {
     static const int kRbStages = 9;
     hipEvent_t rb_ev[kRbStages + 1] = {};
     bool rb_prof = false;
+    // neighbour search form: nf_mode DEM_NSEARCH_*; under AUTO the first two
+    // rebuilds time the tiled and the plain form (nf_ms) and nf_choice keeps
+    // the faster (-1 while undecided), for 256 rebuilds
+    int nf_mode = 0, nf_choice = -1, nf_trials = 0, nf_timed = 0, nf_since = 0;
+    bool nf_pending = false;
+    float nf_ms[2] = {0.f, 0.f};
+    hipEvent_t nf_ev[2] = {};
     double fused_ms = 0.0, rebuild_ms = 0.0;
     long long fused_steps = 0, fused_launches = 0;
     long long bytes = 0;

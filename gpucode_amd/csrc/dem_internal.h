@@ -163,6 +163,42 @@ struct BatchInst {
     double Tw0, dTw;
 };
 
+// Device-driven batched stepping (dem_batch_step with every instance small
+// enough for the batched rebuild): the host enqueues steps and rebuild slots
+// without reading anything back, and the kernels keep each instance's
+// state.  DBatchInst holds both copies of every buffer of an instance;
+// DBatchCtl is its state at the start of its current segment (the steps
+// since its last rebuild), which the rebuild slot kernels update.
+struct DBatchInst {
+    double4 *P4[2], *VW[2];
+    int *lq[2], *level[2], *id[2];
+    int *nbr[2], *nnb[2];
+    HistRec *hp[2];
+    unsigned long long *amask[2];
+    int *actoff[2];
+    int *wl[2], *nwl[2];
+    HistRec *hw[2];
+    unsigned *keys, *vals, *keys2, *vals2;
+    int *cstart, *cend;
+    unsigned long long *nmask;
+    int *overflow;
+    int *stop_at;              // the step kernels' rebuild request, local step + 1
+    int cap, total, K, ncells, passes, pad;
+    double dTw;                // lid increment per step
+};
+
+struct DBatchCtl {
+    int cur, Lidx, pcur, hcur, hl;  // buffer parities at the segment's start
+    int base;                  // global step index at which the segment began
+    int left;                  // steps the instance may still take from base
+    int armed;                 // rebuilt in the current slot
+    int failed;                // > 0: a list overflowed (slots needed); halted
+    int nreb;                  // rebuilds in this run
+    int executed;              // steps before the current segment, this run
+    int seg_steps;             // steps of the segment the last rebuild ended
+    double Tw;                 // lid height at the segment's start
+};
+
 // Inputs of the packing analysis of one instance (current records).
 struct AnaArgs {
     const double4 *p;

@@ -432,6 +432,126 @@ DEV void neighbor_find_one(const double4 *p, const int *level, const DevInst &P,
     nnb[i] = num;
 }
 
+// The same search by a group of 16 lanes per particle, for the batched
+// rebuild of small instances, where one thread per particle leaves the chip
+// nearly empty and the search time is the latency of one particle's walk
+// over its rows.  The rows of the scan (level, z row, y row in the order
+// above) are dealt out 16 at a time, one per lane; a lane walks its row's
+// cells in x order and candidates in storage order, so the list is the
+// concatenation of the rows' hits in row order: a 16-lane prefix sum of the
+// hit counts places each lane's hits.  Up to NFG_KEEP hits of a row stay
+// in registers from the counting pass; a row with more is walked again.
+#define NFG_KEEP 8
+
+DEV int nfg_row(const double4 *p, const DevInst &P, const int *cstart, const int *cend, int i,
+                const double4 o, double ri, int rowb, int sx, int ex, int *keep, bool write, int *nbr,
+                int cap, int num0, int K) {
+    int c = 0;
+    for (int x = sx; x <= ex; ++x) {
+        const int s0 = cstart[rowb + x];
+        if (s0 == DEM_EMPTY) continue;
+        const int e0 = cend[rowb + x];
+        for (int j = s0; j < e0; ++j) {
+            if (j == i) continue;
+            const double4 q = p[j];
+            double dx = o.x - q.x, dy2 = o.y - q.y, dz2 = o.z - q.z;
+            double len = sqrt(dx * dx + dy2 * dy2 + dz2 * dz2);
+            if (len < q.w + ri + P.searchGap) {
+                if (write) {
+                    if (num0 + c < K) nbr[(long)(num0 + c) * cap + i] = j;
+                } else {
+#pragma unroll
+                    for (int t = 0; t < NFG_KEEP; ++t)
+                        if (t == c) keep[t] = j;
+                }
+                ++c;
+            }
+        }
+    }
+    return c;
+}
+
+// g = lane within the particle's group of 16; every lane of the group calls
+// this with the same i (valid = false for a group past the end)
+DEV void neighbor_find_group16(const double4 *p, const int *level, const DevInst &P,
+                               const int *cstart, const int *cend, int *nbr, int *nnb, int *overflow,
+                               int cap, int i, int g, bool valid) {
+    double4 o = make_double4(0.0, 0.0, 0.0, 0.0);
+    int iL = 0;
+    if (valid) {
+        o = p[i];
+        iL = level[i];
+    }
+    const double ri = o.w;
+    const int K = P.levelK[iL];
+    // rows per level and their total
+    int nrows = 0;
+    for (int L = 0; L < P.nLevels; ++L) {
+        const int off = (int)ceil(P.levelSize[iL] / P.levelSize[L]);
+        nrows += (2 * off + 1) * (2 * off + 1);
+    }
+    if (!valid) nrows = 0;
+    int num = 0;
+    const int gbase = (threadIdx.x & 63) & ~15;
+    for (int r0 = 0; r0 < nrows; r0 += 16) {
+        // this lane's row: level, then (dz, dy)
+        int r = r0 + g, L = 0, rowb = 0, sx = 0, ex = -1;
+        bool have = r < nrows;
+        if (have) {
+            int off = 0, w = 0;
+            for (L = 0; L < P.nLevels; ++L) {
+                off = (int)ceil(P.levelSize[iL] / P.levelSize[L]);
+                w = 2 * off + 1;
+                if (r < w * w) break;
+                r -= w * w;
+            }
+            const double cs = P.cellSize[L];
+            const int gx = P.gx[L], gy = P.gy[L], gz = P.gz[L];
+            const int cx = cell_coord(o.x, P.origin[0], cs, gx);
+            const int cy = cell_coord(o.y, P.origin[1], cs, gy);
+            const int cz = cell_coord(o.z, P.origin[2], cs, gz);
+            const int zz = cz + (r / w - off), yy = cy + (r % w - off);
+            if (zz < 0 || zz > gz - 1 || yy < 0 || yy > gy - 1) {
+                have = false;
+            } else {
+                sx = cx - off;
+                ex = cx + off;
+                if (sx < 0) sx = 0;
+                if (ex > gx - 1) ex = gx - 1;
+                rowb = P.cellOff[L] + zz * gy * gx + yy * gx;
+            }
+        }
+        int keep[NFG_KEEP];
+        const int c = have ? nfg_row(p, P, cstart, cend, i, o, ri, rowb, sx, ex, keep, false, nbr, cap,
+                                     0, K)
+                           : 0;
+        // inclusive scan of c over the 16 lanes of the group
+        int inc = c;
+#pragma unroll
+        for (int d = 1; d < 16; d <<= 1) {
+            const int t = __shfl_up(inc, d, 16);
+            if (g >= d) inc += t;
+        }
+        const int tot = __shfl(inc, gbase + 15, 64);
+        const int at = num + inc - c;
+        if (c > NFG_KEEP) {
+            (void)nfg_row(p, P, cstart, cend, i, o, ri, rowb, sx, ex, keep, true, nbr, cap, at, K);
+        } else {
+#pragma unroll
+            for (int t = 0; t < NFG_KEEP; ++t)
+                if (t < c && at + t < K) nbr[(long)(at + t) * cap + i] = keep[t];
+        }
+        num += tot;
+    }
+    if (valid && g == 0) {
+        if (num > K) {
+            atomicMax(overflow, num);
+            num = K;
+        }
+        nnb[i] = num;
+    }
+}
+
 __global__ void __launch_bounds__(128) k_neighbor_find(const double4 *p, const int *level,
                                                        DevInst P, const int *cstart,
                                                        const int *cend, int *nbr, int *nnb,
@@ -452,26 +572,10 @@ __global__ void __launch_bounds__(128) k_neighbor_find(const double4 *p, const i
 // or L2 once per tile instead of once per particle that scans it.  A tile
 // whose rim holds more candidates than LDS takes scans from global memory.
 // ---------------------------------------------------------------------
-#define NF_BS 128
+#define NF_BS 256
 #define NF_EXT 6                         // tile edge + rim, in cells
 #define NF_CELLS (NF_EXT * NF_EXT * NF_EXT)
 #define NF_MAXP 768                      // staged candidate positions
-
-// tile starts (storage positions where the tile prefix of the key changes),
-// appended in any order; one atomic per wave
-__global__ void __launch_bounds__(256) k_tile_starts(const unsigned *keys, int total, int *tiles,
-                                                     int *ntiles) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool st = k < total && (k == 0 || (keys[k] >> 6) != (keys[k - 1] >> 6));
-    const unsigned long long m = __ballot(st);
-    if (m == 0) return;
-    const int lane = threadIdx.x & 63;
-    const int leader = __ffsll((long long)m) - 1;
-    int base = 0;
-    if (lane == leader) base = atomicAdd(ntiles, __popcll(m));
-    base = __shfl(base, leader);
-    if (st) tiles[base + __popcll(m & ((1ull << lane) - 1))] = k;
-}
 
 // every third bit of v from bit 0, packed (the inverse of spread3)
 DEV unsigned compact3(unsigned v) {
@@ -483,92 +587,99 @@ DEV unsigned compact3(unsigned v) {
     return v;
 }
 
+// Block b takes the tiles that start among storage positions
+// [256 b, 256 b + 256), one after another (a tile may run past the range).
 __global__ void __launch_bounds__(NF_BS) k_neighbor_tiles(const double4 *p, const int *level,
                                                           DevInst P, const unsigned *keys,
                                                           const int *cstart, const int *cend,
-                                                          const int *tiles, const int *ntiles,
                                                           int *nbr, int *nnb, int *overflow, int cap,
                                                           int total) {
     __shared__ double4 sp[NF_MAXP];
-    __shared__ int scount[256];          // candidates per rim cell, then their LDS offsets
-    __shared__ int sglob[NF_CELLS];      // global start of each rim cell
-    __shared__ int sbound[2];
-    const int tid = threadIdx.x;
-    const int nt = *ntiles;
+    __shared__ int soff[NF_CELLS + 1];   // LDS offset of each rim cell's candidates, and the total
+    __shared__ int sglob[NF_CELLS];      // storage index of each rim cell's first particle
+    __shared__ int sstart[NF_BS];        // the tiles starting in this block's range
+    __shared__ int swave[NF_BS / 64];    // per-wave totals of a scan
+    __shared__ int sval[2];              // tile count; tile end
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int gx = P.gx[0], gy = P.gy[0], gz = P.gz[0];
     const int K = P.levelK[0];
     const double cs = P.cellSize[0];
-    for (int t = blockIdx.x; t < nt; t += gridDim.x) {
-        if (tid == 0) {
-            const int k0 = tiles[t];
-            const unsigned pre = keys[k0] >> 6;
-            // end of the tile: first key past its prefix (keys are sorted)
-            int lo = k0 + 1, hi = total;
-            while (lo < hi) {
-                const int mid = (lo + hi) >> 1;
-                if ((keys[mid] >> 6) == pre) lo = mid + 1;
-                else hi = mid;
-            }
-            sbound[0] = k0;
-            sbound[1] = lo;
-        }
+    {
+        const int k = blockIdx.x * NF_BS + tid;
+        const bool st = k < total && (k == 0 || (keys[k] >> 6) != (keys[k - 1] >> 6));
+        const unsigned long long m = __ballot(st);
+        if (lane == 0) swave[wv] = __popcll(m);
         __syncthreads();
-        const int k0 = sbound[0], k1 = sbound[1];
-        const unsigned pre = (keys[k0] - P.mortonOff[0]) >> 6;
-        const int x0 = 4 * (int)compact3(pre) - 1, y0 = 4 * (int)compact3(pre >> 1) - 1,
-                  z0 = 4 * (int)compact3(pre >> 2) - 1;
-        // candidates of each rim cell (cells outside the grid hold none)
-        for (int c = tid; c < 256; c += NF_BS) {
-            int cnt = 0;
-            if (c < NF_CELLS) {
-                const int x = x0 + c % NF_EXT, y = y0 + (c / NF_EXT) % NF_EXT,
-                          z = z0 + c / (NF_EXT * NF_EXT);
-                int g0 = 0;
-                if (x >= 0 && x < gx && y >= 0 && y < gy && z >= 0 && z < gz) {
-                    const int h = P.cellOff[0] + z * gy * gx + y * gx + x;
-                    const int s0 = cstart[h];
-                    if (s0 != DEM_EMPTY) {
-                        g0 = s0;
-                        cnt = cend[h] - s0;
-                    }
+        int before = 0, all = 0;
+        for (int w = 0; w < NF_BS / 64; ++w) {
+            before += w < wv ? swave[w] : 0;
+            all += swave[w];
+        }
+        if (st) sstart[before + __popcll(m & ((1ull << lane) - 1))] = k;
+        if (tid == 0) sval[0] = all;
+        __syncthreads();
+    }
+    const int nst = sval[0];
+    for (int t = 0; t < nst; ++t) {
+        const int k0 = sstart[t];
+        const unsigned key0 = keys[k0];
+        const unsigned pre = key0 >> 6;
+        // the tile's end: the first later position with another prefix
+        if (tid == 0) sval[1] = total;
+        __syncthreads();
+        int k1 = total;
+        for (int c0 = k0 + 1; c0 < total; c0 += NF_BS) {
+            const int j = c0 + tid;
+            if (j < total && (keys[j] >> 6) != pre) atomicMin(&sval[1], j);
+            __syncthreads();
+            k1 = sval[1];
+            __syncthreads();
+            if (k1 < total) break;
+        }
+        // rim cells and their candidate counts (cells outside the grid hold none)
+        const unsigned tp = (key0 - P.mortonOff[0]) >> 6;
+        const int x0 = 4 * (int)compact3(tp) - 1, y0 = 4 * (int)compact3(tp >> 1) - 1,
+                  z0 = 4 * (int)compact3(tp >> 2) - 1;
+        int cnt = 0;
+        if (tid < NF_CELLS) {
+            const int x = x0 + tid % NF_EXT, y = y0 + (tid / NF_EXT) % NF_EXT,
+                      z = z0 + tid / (NF_EXT * NF_EXT);
+            int g0 = 0;
+            if (x >= 0 && x < gx && y >= 0 && y < gy && z >= 0 && z < gz) {
+                const int h = P.cellOff[0] + z * gy * gx + y * gx + x;
+                const int s0 = cstart[h], e0 = cend[h];
+                if (s0 != DEM_EMPTY) {
+                    g0 = s0;
+                    cnt = e0 - s0;
                 }
-                sglob[c] = g0;
             }
-            scount[c] = cnt;
+            sglob[tid] = g0;
         }
+        // exclusive prefix sum over the block: within each wave by shuffles,
+        // then the totals of the waves before
+        int incl = cnt;
+        for (int d = 1; d < 64; d <<= 1) {
+            const int u = __shfl_up(incl, d);
+            if (lane >= d) incl += u;
+        }
+        if (lane == 63) swave[wv] = incl;
         __syncthreads();
-        // exclusive prefix sum of the 256 counts (two per thread)
-        {
-            const int a = scount[2 * tid], b = scount[2 * tid + 1];
-            int v = a + b;
-            __syncthreads();
-            scount[2 * tid + 1] = v;  // pair sums at odd slots, scanned below
-            __syncthreads();
-            for (int d = 1; d < NF_BS; d <<= 1) {
-                int u = tid >= d ? scount[2 * (tid - d) + 1] : 0;
-                __syncthreads();
-                if (tid >= d) scount[2 * tid + 1] += u;
-                __syncthreads();
-            }
-            const int incl = scount[2 * tid + 1];
-            __syncthreads();
-            scount[2 * tid] = incl - v;       // exclusive start of cell 2 tid
-            scount[2 * tid + 1] = incl - b;   // exclusive start of cell 2 tid + 1
-            if (tid == NF_BS - 1) sbound[0] = incl;
-            __syncthreads();
-        }
-        const int ncand = sbound[0];
+        int woff = 0;
+        for (int w = 0; w < wv; ++w) woff += swave[w];
+        if (tid <= NF_CELLS) soff[tid] = woff + incl - cnt;
+        __syncthreads();
+        const int ncand = soff[NF_CELLS];
         const bool staged = ncand <= NF_MAXP;
         if (staged) {
-            // copy: candidate q belongs to the last cell whose start is <= q
+            // candidate q belongs to the last cell whose offset is <= q
             for (int q = tid; q < ncand; q += NF_BS) {
                 int lo = 0, hi = NF_CELLS - 1;
                 while (lo < hi) {
                     const int mid = (lo + hi + 1) >> 1;
-                    if (scount[mid] <= q) lo = mid;
+                    if (soff[mid] <= q) lo = mid;
                     else hi = mid - 1;
                 }
-                sp[q] = p[sglob[lo] + (q - scount[lo])];
+                sp[q] = p[sglob[lo] + (q - soff[lo])];
             }
         }
         __syncthreads();
@@ -595,7 +706,7 @@ __global__ void __launch_bounds__(NF_BS) k_neighbor_tiles(const double4 *p, cons
                     const int rowc = (zz - z0) * NF_EXT * NF_EXT + (yy - y0) * NF_EXT - x0;
                     for (int x = sx; x <= ex; ++x) {
                         const int c = rowc + x;
-                        const int q0 = scount[c], q1 = scount[c + 1];
+                        const int q0 = soff[c], q1 = soff[c + 1];
                         const int g0 = sglob[c] - q0;
                         for (int q = q0; q < q1; ++q) {
                             const int j = g0 + q;
@@ -617,29 +728,26 @@ __global__ void __launch_bounds__(NF_BS) k_neighbor_tiles(const double4 *p, cons
             }
             nnb[i] = num;
         }
-        __syncthreads();
+        __syncthreads();  // LDS is reused by the next tile
     }
 }
 
-// tiles, ntiles: scratch of total + 1 ints (free while the lists are built)
+bool neighbor_tiles_apply(const DevInst &P) {
+    return P.morton && P.nLevels == 1 && (P.mortonOff[0] & 63u) == 0 && P.gx[0] <= 1024 &&
+           P.gy[0] <= 1024 && P.gz[0] <= 1024;
+}
+
+// tiled: the LDS-staged form (only where neighbor_tiles_apply holds)
 void launch_neighbor_find(const StateBuf &s, const int *level, const DevInst &P, const unsigned *keys,
                           const int *cstart, const int *cend, int *nbr, int *nnb, int *overflow,
-                          int *tiles, int *ntiles, int cap, int total, hipStream_t st) {
+                          bool tiled, int cap, int total, hipStream_t st) {
     if (total <= 0) return;
-    const bool tiled = P.morton && P.nLevels == 1 && (P.mortonOff[0] & 63u) == 0 &&
-                       P.gx[0] <= 1024 && P.gy[0] <= 1024 && P.gz[0] <= 1024;
-    if (!tiled) {
+    if (tiled && neighbor_tiles_apply(P))
+        hipLaunchKernelGGL(k_neighbor_tiles, dim3(blocks_for(total, NF_BS)), dim3(NF_BS), 0, st, s.p,
+                           level, P, keys, cstart, cend, nbr, nnb, overflow, cap, total);
+    else
         hipLaunchKernelGGL(k_neighbor_find, dim3(blocks_for(total, 128)), dim3(128), 0, st, s.p,
                            level, P, cstart, cend, nbr, nnb, overflow, cap, total);
-        return;
-    }
-    launch_fill_i32(ntiles, 0, 1, st);
-    hipLaunchKernelGGL(k_tile_starts, dim3(blocks_for(total, 256)), dim3(256), 0, st, keys, total,
-                       tiles, ntiles);
-    // a few resident blocks per CU walk the tiles
-    const int grid = std::min(2048, std::max(1, total / 64));
-    hipLaunchKernelGGL(k_neighbor_tiles, dim3(grid), dim3(NF_BS), 0, st, s.p, level, P, keys, cstart,
-                       cend, tiles, ntiles, nbr, nnb, overflow, cap, total);
 }
 
 // ---------------------------------------------------------------------
@@ -788,11 +896,21 @@ DEV void hist_remap_one(const unsigned *perm, const int *id_new, const int *id_o
     const unsigned long long amn = am_new[i];
     const int ofn = off_new[i];
     int n = nnb[i];
+    // the old partners' ids, gathered once (a short old list stays in
+    // registers); the last old slot with the partner's id wins, as in
+    // kernel.cu:895-905 (partners are unique, so there is one)
+    constexpr int kReg = 32;
+    int oid[kReg];
+#pragma unroll
+    for (int b = 0; b < kReg; ++b) oid[b] = b < nold ? id_old[nbr_old[(long)b * cap + p]] : -1;
     for (int a = 0; a < n; ++a) {
         long sa = (long)a * cap + i;
         int jid = id_new[nbr[sa]];
         int hit = -1;
-        for (int b = 0; b < nold; ++b)
+#pragma unroll
+        for (int b = 0; b < kReg; ++b)
+            if (oid[b] == jid) hit = b;
+        for (int b = kReg; b < nold; ++b)
             if (id_old[nbr_old[(long)b * cap + p]] == jid) hit = b;
         h[hist_idx(amn, ofn, a, i, cap, Kcap)] =
             hit >= 0 ? h_old[hist_idx(amo, ofo, hit, p, cap, Kcap)] : hist_zero();
@@ -2354,7 +2472,8 @@ DEV bool batch_step_args(const BatchInst &B, int step, StepArgs &A, int &stop) {
     A.out.p = B.P4[1 - b]; A.out.v = B.VW[1 - b]; A.out.lqd = B.lq[1 - l];
     A.step = step;
     double tw = B.Tw0;
-    for (int s = 0; s <= step; ++s) tw = tw + B.dTw;
+    if (B.dTw != 0.0)  // a fixed lid (topv = 0) leaves Tw as it is
+        for (int s = 0; s <= step; ++s) tw = tw + B.dTw;
     A.Tw = tw;
     return true;
 }
@@ -2404,6 +2523,191 @@ void launch_step_batch(const DevInst *insts, const BatchInst *tab, const int2 *t
             hipLaunchKernelGGL((k_step_batch_loop<false>), dim3(ntasks), dim3(256), 0, s, insts, tab,
                                tasks, step);
     }
+}
+
+// ---------------------------------------------------------------------
+// Device-driven batched stepping.  Global step g of the run is local step
+// s = g - base of an instance's current segment; the instance skips it once
+// it has taken its steps or asked for a rebuild at an earlier step (stop_at
+// = local step + 1), exactly as a chunk of single-instance steps.  Every few
+// steps a rebuild slot runs: kb_arm fills the batched rebuild table for
+// the instances that asked (zero particles and cells for the others, so the
+// stage kernels skip them), the stage kernels run over task lists of every
+// instance, and kb_finish records overflow.
+// ---------------------------------------------------------------------
+DEV bool dbatch_step_args(const DBatchInst &D, const DBatchCtl &C, int g, StepArgs &A, int &stop) {
+    const int s = g - C.base;
+    if (C.failed || s < 0 || s >= C.left) return false;
+    stop = __hip_atomic_load(D.stop_at, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const int b = (C.cur + s) & 1, l = (C.Lidx + s) & 1;
+    StepArgs a = {};
+    a.in.p = D.P4[b]; a.in.v = D.VW[b]; a.in.lqd = D.lq[l];
+    a.out.p = D.P4[1 - b]; a.out.v = D.VW[1 - b]; a.out.lqd = D.lq[1 - l];
+    a.nbr = D.nbr[C.hcur]; a.nnb = D.nnb[C.hcur];
+    a.h = D.hp[C.hl]; a.amask = D.amask[C.hl]; a.actoff = D.actoff[C.hl];
+    a.nmask = D.nmask;
+    a.wl = D.wl[C.hcur]; a.nwl = D.nwl[C.hcur]; a.hw = D.hw[C.hcur];
+    a.stop_at = D.stop_at;
+    a.cap = D.cap; a.K = D.K; a.total = D.total;
+    a.step = s;
+    double tw = C.Tw;
+    if (D.dTw != 0.0)  // a fixed lid (topv = 0) leaves Tw as it is
+        for (int q = 0; q <= s; ++q) tw = tw + D.dTw;
+    a.Tw = tw;
+    A = a;
+    return true;
+}
+
+template <bool MIXED>
+__global__ void __launch_bounds__(WV_BS) k_step_dbatch_wave(const DevInst *insts, const DBatchInst *dins,
+                                                            const DBatchCtl *ctl, const int2 *tasks,
+                                                            int g) {
+    const int2 tk = tasks[xcd_block(gridDim.x)];
+    StepArgs A;
+    int stop;
+    if (!dbatch_step_args(dins[tk.x], ctl[tk.x], g, A, stop)) return;
+    __shared__ double res_all[WV_BS / 64][7][64];
+    __shared__ WvStash stash[WV_BS / 64];
+    step_wave_block<MIXED, false>(insts[tk.x], A, (unsigned)tk.y, res_all[threadIdx.x >> 6],
+                                  &stash[threadIdx.x >> 6], 0, 64, stop);
+}
+
+template <bool MIXED>
+__global__ void __launch_bounds__(256) k_step_dbatch_loop(const DevInst *insts, const DBatchInst *dins,
+                                                          const DBatchCtl *ctl, const int2 *tasks,
+                                                          int g) {
+    const int2 tk = tasks[xcd_block(gridDim.x)];
+    StepArgs A;
+    int stop;
+    if (!dbatch_step_args(dins[tk.x], ctl[tk.x], g, A, stop)) return;
+    const DevInst &P = insts[tk.x];
+    if (P.forceCap == 1)
+        step_loop_block<true, MIXED>(P, A, (unsigned)tk.y, stop);
+    else
+        step_loop_block<false, MIXED>(P, A, (unsigned)tk.y, stop);
+}
+
+void launch_step_dbatch(const DevInst *insts, const DBatchInst *dins, const DBatchCtl *ctl,
+                        const int2 *tasks, int ntasks, bool wave, bool mixed, int g, hipStream_t s) {
+    if (ntasks <= 0) return;
+    if (wave) {
+        if (mixed)
+            hipLaunchKernelGGL((k_step_dbatch_wave<true>), dim3(ntasks), dim3(WV_BS), 0, s, insts, dins,
+                               ctl, tasks, g);
+        else
+            hipLaunchKernelGGL((k_step_dbatch_wave<false>), dim3(ntasks), dim3(WV_BS), 0, s, insts,
+                               dins, ctl, tasks, g);
+    } else {
+        if (mixed)
+            hipLaunchKernelGGL((k_step_dbatch_loop<true>), dim3(ntasks), dim3(256), 0, s, insts, dins,
+                               ctl, tasks, g);
+        else
+            hipLaunchKernelGGL((k_step_dbatch_loop<false>), dim3(ntasks), dim3(256), 0, s, insts, dins,
+                               ctl, tasks, g);
+    }
+}
+
+// one thread per instance, after global steps up to g_end: the rebuild
+// table entry of each instance that asked for a rebuild and has steps left
+// (the buffer flips of a rebuild as dem_do_rebuild makes them), and in
+// nxt the control record it resumes with at global step resume.  The steps
+// of the next R global steps run concurrently on the other stream and read
+// ctl, so ctl is left alone here: kb_commit installs nxt once the slot is
+// done and those steps have run.  An armed instance idles until then (its
+// stop_at stays at or below the steps it has seen).
+__global__ void kb_arm(const DBatchInst *dins, const DBatchCtl *ctl, DBatchCtl *nxt, RbInst *tab,
+                       int n, int g_end, int resume) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const DBatchInst &D = dins[k];
+    DBatchCtl C = ctl[k];
+    RbInst R;
+    memset(&R, 0, sizeof(R));
+    R.inst = k;
+    R.overflow = D.overflow;
+    R.keys = D.keys; R.vals = D.vals; R.keys2 = D.keys2; R.vals2 = D.vals2;
+    R.cap = D.cap;
+    R.K = D.K;
+    const int seen = g_end + 1 - C.base;
+    // steps running now may lower stop_at, never to seen or below
+    const int stop = __hip_atomic_load(D.stop_at, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const int done = stop < seen ? stop : seen;
+    const int executed = done < C.left ? done : C.left;
+    const bool need = !C.failed && stop <= seen && executed < C.left;
+    if (!need) {  // R.total = R.ncells = 0: every stage skips this entry
+        tab[k] = R;
+        nxt[k].armed = 0;
+        return;
+    }
+    const int cur = (C.cur + executed) & 1, lx = (C.Lidx + executed) & 1;
+    const int hc = 1 - C.hcur, hl = 1 - C.hl;  // the lists become the old ones
+    R.pa = D.P4[cur]; R.va = D.VW[cur]; R.lqa = D.lq[lx];
+    R.pb = D.P4[1 - cur]; R.vb = D.VW[1 - cur]; R.lqb = D.lq[1 - lx];
+    R.lvla = D.level[C.pcur]; R.ida = D.id[C.pcur];
+    R.lvlb = D.level[1 - C.pcur]; R.idb = D.id[1 - C.pcur];
+    R.passes = D.passes;
+    R.cstart = D.cstart; R.cend = D.cend; R.ncells = D.ncells;
+    R.nbr = D.nbr[hc]; R.nnb = D.nnb[hc]; R.nbr_old = D.nbr[1 - hc]; R.nnb_old = D.nnb[1 - hc];
+    R.hp = D.hp[hl]; R.hp_old = D.hp[1 - hl];
+    R.amask = D.amask[hl]; R.amask_old = D.amask[1 - hl];
+    R.actoff = D.actoff[hl]; R.actoff_old = D.actoff[1 - hl];
+    R.nmask = D.nmask;
+    R.wl = D.wl[hc]; R.nwl = D.nwl[hc]; R.wl_old = D.wl[1 - hc]; R.nwl_old = D.nwl[1 - hc];
+    R.hw = D.hw[hc]; R.hw_old = D.hw[1 - hc];
+    R.total = D.total;
+    R.old_total = D.total;
+    double tw = C.Tw;
+    if (D.dTw != 0.0)
+        for (int q = 0; q < executed; ++q) tw = tw + D.dTw;
+    R.Tw = tw;
+    tab[k] = R;
+    C.cur = 1 - cur;
+    C.Lidx = 1 - lx;
+    C.pcur = 1 - C.pcur;
+    C.hcur = hc;
+    C.hl = hl;
+    C.base = resume;
+    C.left -= executed;
+    C.executed += executed;
+    C.seg_steps = executed;
+    C.Tw = tw;
+    C.nreb += 1;
+    C.armed = 1;
+    nxt[k] = C;
+}
+
+// after the slot's stages: an instance whose lists overflowed halts
+__global__ void kb_finish(const DBatchInst *dins, DBatchCtl *nxt, int n) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n || !nxt[k].armed) return;
+    const int ov = *dins[k].overflow;
+    if (ov > 0) nxt[k].failed = ov;
+}
+
+// on the step stream, after the slot and the steps that ran beside it:
+// the rebuilt instances take their new control records and clear their
+// rebuild requests
+__global__ void kb_commit(const DBatchInst *dins, DBatchCtl *ctl, const DBatchCtl *nxt, int n) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n || !nxt[k].armed) return;
+    DBatchCtl C = nxt[k];
+    C.armed = 0;
+    ctl[k] = C;
+    *dins[k].stop_at = INT_MAX;
+}
+
+void launch_rebuild_slot(const DevInst *insts, const DBatchInst *dins, const DBatchCtl *ctl,
+                         DBatchCtl *nxt, RbInst *tab, int n, int g_end, int resume, const int2 *ptasks,
+                         int nptasks, const int2 *ctasks, int nctasks, hipStream_t s) {
+    hipLaunchKernelGGL(kb_arm, dim3((n + 63) / 64), dim3(64), 0, s, dins, ctl, nxt, tab, n, g_end,
+                       resume);
+    launch_rebuild_batch(insts, tab, n, ptasks, nptasks, ctasks, nctasks, s);
+    hipLaunchKernelGGL(kb_finish, dim3((n + 63) / 64), dim3(64), 0, s, dins, nxt, n);
+}
+
+void launch_commit_slot(const DBatchInst *dins, DBatchCtl *ctl, const DBatchCtl *nxt, int n,
+                        hipStream_t s) {
+    hipLaunchKernelGGL(kb_commit, dim3((n + 63) / 64), dim3(64), 0, s, dins, ctl, nxt, n);
 }
 
 // ---------------------------------------------------------------------
@@ -2628,7 +2932,7 @@ __global__ void __launch_bounds__(RB_BS) kb_fill_cells(const RbInst *tab, const 
         R.cstart[c] = DEM_EMPTY;
         R.cend[c] = DEM_EMPTY;
     }
-    if (tk.y == 0 && threadIdx.x == 0) *R.overflow = 0;
+    if (tk.y == 0 && threadIdx.x == 0 && R.total > 0) *R.overflow = 0;
 }
 
 __global__ void __launch_bounds__(RB_BS) kb_cell_range(const DevInst *insts, const RbInst *tab,
@@ -2639,15 +2943,20 @@ __global__ void __launch_bounds__(RB_BS) kb_cell_range(const DevInst *insts, con
     if (k < R.total) cell_range_one(R.keys, R.pb, R.lvlb, insts[R.inst], R.cstart, R.cend, R.total, k);
 }
 
-__global__ void __launch_bounds__(RB_BS) kb_lists(const DevInst *insts, const RbInst *tab,
-                                                  const int2 *tasks) {
-    const int2 tk = tasks[blockIdx.x];
+// four blocks of 1024 lanes per task of 256 particles, 16 lanes per particle
+#define KBL_BS 1024
+__global__ void __launch_bounds__(KBL_BS) kb_lists(const DevInst *insts, const RbInst *tab,
+                                                   const int2 *tasks) {
+    const int2 tk = tasks[blockIdx.x >> 2];
     const RbInst &R = tab[tk.x];
-    const int i = tk.y * RB_BS + threadIdx.x;
-    if (i >= R.total) return;
+    const int first = tk.y * RB_BS + (blockIdx.x & 3) * (KBL_BS / 16);
+    if (first >= R.total) return;  // the whole block: no lane is needed for a shuffle
+    const int i = first + (int)(threadIdx.x >> 4), g = threadIdx.x & 15;
+    const bool valid = i < R.total;
     const DevInst &P = insts[R.inst];
-    neighbor_find_one(R.pb, R.lvlb, P, R.cstart, R.cend, R.nbr, R.nnb, R.overflow, R.cap, i);
-    wall_list_one(R.pb, P, R.Tw, R.wl, R.nwl, R.cap, i);
+    neighbor_find_group16(R.pb, R.lvlb, P, R.cstart, R.cend, R.nbr, R.nnb, R.overflow, R.cap, i, g,
+                          valid);
+    if (valid && g == 1) wall_list_one(R.pb, P, R.Tw, R.wl, R.nwl, R.cap, i);
 }
 
 __global__ void __launch_bounds__(RB_BS) kb_active(const DevInst *insts, const RbInst *tab,
@@ -2661,17 +2970,26 @@ __global__ void __launch_bounds__(RB_BS) kb_active(const DevInst *insts, const R
 }
 
 // actoff = exclusive prefix sum of the per-particle counts, one block per
-// instance, tile by tile
+// instance, tiles of SCAN_ITEMS consecutive counts per lane
 __global__ void __launch_bounds__(SCAN_BS) kb_actoff(const RbInst *tab) {
     const RbInst &R = tab[blockIdx.x];
     __shared__ unsigned lds[4];
     unsigned carry = 0;
-    for (int t0 = 0; t0 < R.total; t0 += SCAN_BS) {
-        const int e = t0 + threadIdx.x;
-        const unsigned v = e < R.total ? R.keys2[e] : 0u;
+    for (int t0 = 0; t0 < R.total; t0 += SCAN_TILE) {
+        const int e = t0 + threadIdx.x * SCAN_ITEMS;
+        unsigned v[SCAN_ITEMS], sum = 0;
+#pragma unroll
+        for (int q = 0; q < SCAN_ITEMS; ++q) {
+            v[q] = e + q < R.total ? R.keys2[e + q] : 0u;
+            sum += v[q];
+        }
         unsigned tot;
-        const unsigned ex = block_excl_scan(v, &tot, lds);
-        if (e < R.total) R.actoff[e] = (int)(carry + ex);
+        unsigned ex = carry + block_excl_scan(sum, &tot, lds);
+#pragma unroll
+        for (int q = 0; q < SCAN_ITEMS; ++q) {
+            if (e + q < R.total) R.actoff[e + q] = (int)ex;
+            ex += v[q];
+        }
         carry += tot;
     }
 }
@@ -2696,7 +3014,7 @@ void launch_rebuild_batch(const DevInst *insts, const RbInst *tab, int ninst, co
     if (nctasks > 0) hipLaunchKernelGGL(kb_fill_cells, dim3(nctasks), dim3(RB_BS), 0, s, tab, ctasks);
     if (nptasks > 0) {
         hipLaunchKernelGGL(kb_cell_range, dim3(nptasks), dim3(RB_BS), 0, s, insts, tab, ptasks);
-        hipLaunchKernelGGL(kb_lists, dim3(nptasks), dim3(RB_BS), 0, s, insts, tab, ptasks);
+        hipLaunchKernelGGL(kb_lists, dim3(4 * nptasks), dim3(KBL_BS), 0, s, insts, tab, ptasks);
         hipLaunchKernelGGL(kb_active, dim3(nptasks), dim3(RB_BS), 0, s, insts, tab, ptasks);
         hipLaunchKernelGGL(kb_actoff, dim3(ninst), dim3(SCAN_BS), 0, s, tab);
         hipLaunchKernelGGL(kb_hist_remap, dim3(nptasks), dim3(RB_BS), 0, s, tab, ptasks);
@@ -2863,21 +3181,31 @@ void launch_scatter_state(const double *pos, const double *vel, const double *an
 // plain device copy: the attainable HBM bandwidth, and the known-size
 // kernel that calibrates the FETCH_SIZE / WRITE_SIZE counters
 // ---------------------------------------------------------------------
-// Each lane issues U independent 16-byte loads (a wave reads U contiguous
-// 1 KiB blocks, one grid stride apart) before it stores them, so enough
-// reads are in flight per CU to reach the HBM bandwidth.
-template <int U>
+// Each block copies tiles of 4 x 256 consecutive 16-byte elements: four
+// independent loads per lane, then four stores, all non-temporal (the data
+// is touched once).  Of the forms timed on an MI355X (grid-stride single
+// loads, 2-16 loads per lane, contiguous or strided, temporal or not) this
+// one moved the most: 5.86 TB/s of read + write at 64 blocks per CU.
 __global__ void __launch_bounds__(256) k_stream_copy(const double2 *__restrict__ a,
                                                      double2 *__restrict__ b, long n) {
-    const long T = (long)gridDim.x * blockDim.x;
-    for (long i0 = (long)blockIdx.x * blockDim.x + threadIdx.x; i0 < n; i0 += T * U) {
-        double2 v[U];
+    for (long base = (long)blockIdx.x * 1024; base < n; base += (long)gridDim.x * 1024) {
+        double2 v[4];
 #pragma unroll
-        for (int u = 0; u < U; ++u)
-            if (i0 + u * T < n) v[u] = a[i0 + u * T];
+        for (int u = 0; u < 4; ++u) {
+            const long i = base + u * 256 + threadIdx.x;
+            if (i < n) {
+                v[u].x = __builtin_nontemporal_load(&a[i].x);
+                v[u].y = __builtin_nontemporal_load(&a[i].y);
+            }
+        }
 #pragma unroll
-        for (int u = 0; u < U; ++u)
-            if (i0 + u * T < n) b[i0 + u * T] = v[u];
+        for (int u = 0; u < 4; ++u) {
+            const long i = base + u * 256 + threadIdx.x;
+            if (i < n) {
+                __builtin_nontemporal_store(v[u].x, &b[i].x);
+                __builtin_nontemporal_store(v[u].y, &b[i].y);
+            }
+        }
     }
 }
 
@@ -2885,7 +3213,7 @@ __global__ void __launch_bounds__(256) k_stream_copy(const double2 *__restrict__
 void launch_stream_copy(const void *a, void *b, long nbytes, int blocks_per_cu, hipStream_t s) {
     long n = nbytes / 16;
     if (n > 0)
-        hipLaunchKernelGGL(k_stream_copy<8>, dim3(256 * blocks_per_cu), dim3(256), 0, s,
+        hipLaunchKernelGGL(k_stream_copy, dim3(256 * blocks_per_cu), dim3(256), 0, s,
                            (const double2 *)a, (double2 *)b, n);
 }
 

@@ -19,9 +19,10 @@ void launch_permute(const unsigned *perm, const StateBuf &a, const StateBuf &b, 
                     int *lvl_b, const int *id_a, int *id_b, int total, hipStream_t s);
 void launch_cell_range(const unsigned *keys, const StateBuf &s, const int *level, const DevInst &P,
                        int *cstart, int *cend, int ncells, int total, hipStream_t st);
+bool neighbor_tiles_apply(const DevInst &P);
 void launch_neighbor_find(const StateBuf &s, const int *level, const DevInst &P, const unsigned *keys,
                           const int *cstart, const int *cend, int *nbr, int *nnb, int *overflow,
-                          int *tiles, int *ntiles, int cap, int total, hipStream_t st);
+                          bool tiled, int cap, int total, hipStream_t st);
 void launch_wall_list(const StateBuf &s, const DevInst &P, double Tw, int *wl, int *nwl, int cap,
                       int total, hipStream_t st);
 void launch_hist_remap(const unsigned *perm, const int *id_new, const int *id_old, const int *nbr,
@@ -81,5 +82,12 @@ void launch_gather1i(const int *id, const int *a, int *out, int total, hipStream
 void launch_scatter_state(const double *pos, const double *vel, const double *angv,
                           const double *rad, const StateBuf &s, int n, int off, hipStream_t st);
 void launch_stats(const DevInst &P, const StepArgs &A, unsigned long long *out, hipStream_t s);
+void launch_step_dbatch(const DevInst *insts, const DBatchInst *dins, const DBatchCtl *ctl,
+                        const int2 *tasks, int ntasks, bool wave, bool mixed, int g, hipStream_t s);
+void launch_rebuild_slot(const DevInst *insts, const DBatchInst *dins, const DBatchCtl *ctl,
+                         DBatchCtl *nxt, RbInst *tab, int n, int g_end, int resume, const int2 *ptasks,
+                         int nptasks, const int2 *ctasks, int nctasks, hipStream_t s);
+void launch_commit_slot(const DBatchInst *dins, DBatchCtl *ctl, const DBatchCtl *nxt, int n,
+                        hipStream_t s);
 void launch_scale_velocities(const StateBuf &s, double f, int total, hipStream_t st);
 void launch_stream_copy(const void *a, void *b, long nbytes, int blocks_per_cu, hipStream_t s);

@@ -319,6 +319,14 @@ class Engine:
     def scale_velocities(self, factor):
         check(A.lib().dem_scale_velocities(self._ctx, float(factor)))
 
+    NSEARCH_AUTO, NSEARCH_PLAIN, NSEARCH_TILED = 0, 1, 2
+
+    def set_neighbor_search(self, mode):
+        check(A.lib().dem_set_neighbor_search(self._ctx, int(mode)))
+
+    def neighbor_search_used(self):
+        return A.lib().dem_neighbor_search_used(self._ctx)
+
     def key_bits(self):
         return A.lib().dem_key_bits(self._ctx)
 
@@ -383,6 +391,14 @@ class Batch:
 
     def __exit__(self, *a):
         self.close()
+
+    def set_schedule(self, device=True, slot_every=4):
+        check(A.lib().dem_batch_set_schedule(self._b, int(bool(device)), int(slot_every)))
+
+    def schedule_stats(self):
+        a, b = C.c_longlong(), C.c_longlong()
+        check(A.lib().dem_batch_schedule_stats(self._b, C.byref(a), C.byref(b)))
+        return {"slots": a.value, "host_syncs": b.value}
 
     def step(self, n):
         check(A.lib().dem_batch_step(self._b, int(n)))

@@ -34,6 +34,19 @@ dem_ctx *dem_batch_instance(dem_batch *b, int k);
 /* Advances every instance by nsteps fused steps, rebuilding each instance
  * when its own trigger fires; one launch per step covers all instances. */
 int dem_batch_step(dem_batch *b, int nsteps);
+/* Schedule of dem_batch_step (no counterpart in the reference, which runs
+ * one packing per process).  device != 0, the default: per-instance state
+ * in device tables; after every slot_every global steps (default 4) a
+ * rebuild slot on a second stream rebuilds the instances that asked, while
+ * the next slot_every steps run for the others, and they resume after
+ * that; the host reads nothing back per rebuild (applies when every
+ * instance has 1 to 65,536 particles).  device = 0: chunks of steps with
+ * the rebuild requests read back by the host.  Either way every instance
+ * takes exactly the steps and rebuilds it takes alone. */
+int dem_batch_set_schedule(dem_batch *b, int device, int slot_every);
+/* Rebuild slots run and host synchronisations of the device-driven
+ * schedule so far (no counterpart in the reference). */
+int dem_batch_schedule_stats(dem_batch *b, long long *slots, long long *host_syncs);
 /* Event timings of dem_batch_step: step-kernel time, launches, rebuild
  * time, instance rebuilds. */
 int dem_batch_timing(dem_batch *b, double *step_ms, long long *launches, double *rebuild_ms,

@@ -234,6 +234,16 @@ int dem_sync(dem_ctx *ctx);
  * and lists longer than 64 slots.  AUTO takes WAVE where it applies, else
  * LOOP. */
 int dem_set_kernel(dem_ctx *ctx, int variant);
+/* Neighbour search form.  TILED stages the positions of each aligned tile
+ * of 4 x 4 x 4 cells and its one-cell rim in LDS and scans them from there
+ * (one size level in Morton order only); PLAIN has one thread per particle
+ * scan its 27 cells in memory; AUTO (default) times both at the context's
+ * first rebuilds and keeps the faster, again every 256 rebuilds.  The lists
+ * are identical. */
+enum { DEM_NSEARCH_AUTO = 0, DEM_NSEARCH_PLAIN = 1, DEM_NSEARCH_TILED = 2 };
+int dem_set_neighbor_search(dem_ctx *ctx, int mode);
+/* The form the next rebuild uses (AUTO while still timing). */
+int dem_neighbor_search_used(dem_ctx *ctx);
 /* Storage order of the particles between rebuilds: 1 (default) = Morton
  * order of the cells, 0 = the reference's row-major cell hash.  Lists,
  * forces and the exported hash/cell tables are identical either way. */

@@ -110,10 +110,16 @@ def _instances():
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("schedule", ["device", "host"])
 @pytest.mark.parametrize("precision", [G.PREC_FP64, G.PREC_MIXED])
-def test_batch_equals_separate_runs(precision):
+def test_batch_equals_separate_runs(precision, schedule):
+    # device: rebuild requests acted on by the kernels at rebuild slots
+    # every 2 steps, no host round trip per rebuild; host: chunks of steps
+    # with the requests read back.  Either gives every instance exactly its
+    # own single-instance sequence of steps and rebuilds.
     cases = _instances()
     b = G.Batch([w["inp"] for w, _ in cases], [w["dia"] for w, _ in cases], precision)
+    b.set_schedule(device=schedule == "device", slot_every=2)
     singles = []
     for k, (w, vel) in enumerate(cases):
         b[k].set_particles(w["pos"], vel)
@@ -140,6 +146,12 @@ def test_batch_equals_separate_runs(precision):
     assert sum(e.n_rebuild for e in singles) > 2 * len(singles)
     t = b.timing()
     assert t["launches"] >= steps and t["n_rebuild"] >= len(singles)
+    st = b.schedule_stats()
+    if schedule == "device":
+        # one rebuild slot per 2 steps; the host looks every 64 slots
+        assert st["slots"] >= steps // 2 and st["host_syncs"] <= st["slots"] // 64 + 1
+    else:
+        assert st["slots"] == 0
     # a second call continues from where the first stopped
     b.step(50)
     for e in singles:
@@ -169,6 +181,79 @@ def test_batch_reports_the_overflowing_instance():
     with pytest.raises(G.DemError) as err:
         b.step(1)
     assert err.value.code == -3 and "instance 1:" in str(err.value)
+    b.close()
+
+
+def converging_cluster(n=150, rad=3.9, steps_to_centre=20.0, seed=1):
+    """n spheres in a ball of radius rad (sphere diameters) in the middle of
+    small_bed's container, each moving to the centre in steps_to_centre
+    steps: the lists are rebuilt every step and, about 14 steps in, a
+    rebuild finds more neighbours than the lists hold (the oracle run in
+    test_converging_cluster_overflows_in_the_oracle shows where)."""
+    from oracle import pyoracle as O
+
+    w = W.small_bed(n=300, seed=3)
+    S = O.derive(w["inp"])[0]
+    rng = np.random.default_rng(seed)
+    c = np.array([S.cirX, S.cirY, S.Bw + 0.5 * (S.Tw - S.Bw)])
+    pts = []
+    while len(pts) < n:
+        p = c + rng.uniform(-rad, rad, 3)
+        if np.linalg.norm(p - c) <= rad and all(np.linalg.norm(p - q) > 1.05 for q in pts):
+            pts.append(p)
+    pos = np.array(pts)
+    return w, pos, -(pos - c) / (steps_to_centre * S.dt)
+
+
+def test_converging_cluster_overflows_in_the_oracle():
+    from oracle import pyoracle as O
+
+    w, pos, vel = converging_cluster()
+    o = O.Oracle(w["inp"])
+    o.set_particles(pos, vel)
+    with pytest.raises(ValueError, match="overflow"):
+        o.step(40)
+    assert 5 <= o.iter < 40 and o.n_rebuild == o.iter + 1
+    o.close()
+
+
+@pytest.mark.gpu
+def test_device_schedule_reports_overflow_at_a_rebuild_slot():
+    # instance 1's lists overflow at a rebuild the kernels make in the middle
+    # of the run (its first rebuild, made by the host, is fine): the step
+    # reports that instance, which halts at that rebuild, as the single
+    # instance does; the other instances run all their steps unharmed
+    wc, pc, vc = converging_cluster()
+    cases = [(W.small_bed(n=400, seed=4), None), (wc, (pc, vc)),
+             (W.wet_bed(n=600, poly=True, reference_law=False, theta_deg=20.0, liquid=0.2), None)]
+    b = G.Batch([w["inp"] for w, _ in cases], [w["dia"] for w, _ in cases])
+    b.set_schedule(device=True, slot_every=2)
+    singles = []
+    for k, (w, pv) in enumerate(cases):
+        pos, vel = pv if pv is not None else (w["pos"], None)
+        b[k].set_particles(pos, vel)
+        e = G.Engine(w["inp"], w["dia"])
+        e.set_particles(pos, vel)
+        singles.append(e)
+    with pytest.raises(G.DemError) as err:
+        b.step(40)
+    assert err.value.code == -3 and "instance 1:" in str(err.value)
+    with pytest.raises(G.DemError) as err1:
+        singles[1].step(40)
+    assert err1.value.code == -3
+    assert 5 <= b[1].iter == singles[1].iter < 40
+    assert b[1].n_rebuild == singles[1].n_rebuild
+    for k in (0, 2):
+        singles[k].step(40)
+        assert b[k].iter == singles[k].iter == 40
+        for u, v in zip(b[k].particles(), singles[k].particles()):
+            assert np.array_equal(u, v), k
+    assert b.schedule_stats()["slots"] >= 15
+    with pytest.raises(G.DemError) as err:
+        b.step(1)
+    assert err.value.code == -3 and "instance 1:" in str(err.value)
+    for e in singles:
+        e.close()
     b.close()
 
 

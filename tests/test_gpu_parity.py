@@ -93,6 +93,47 @@ def test_neighbour_and_wall_lists_bit_exact():
     assert np.array_equal(w1, w2) and np.array_equal(wl1, wl2)
 
 
+@pytest.mark.parametrize("make", [
+    lambda: W.small_bed(n=3000),
+    lambda: W.pour_2M(n=20000, R=10e-3, H=30e-3),
+], ids=["dense_bed", "sparse_pour"])
+def test_tiled_and_plain_neighbor_search_equal(make):
+    # the LDS-staged tile search and the one-thread search give the same
+    # lists, hence the same steps, rebuilds included
+    w = make()
+    out = {}
+    for mode in (G.Engine.NSEARCH_PLAIN, G.Engine.NSEARCH_TILED):
+        e = G.Engine(w["inp"])
+        e.set_neighbor_search(mode)
+        e.set_particles(w["pos"], w.get("vel"))
+        e.step(1000)
+        assert e.neighbor_search_used() == mode
+        out[mode] = (e.nebor(), e.particles(), e.n_rebuild)
+        e.close()
+    a, b = out[G.Engine.NSEARCH_PLAIN], out[G.Engine.NSEARCH_TILED]
+    assert a[2] == b[2] and a[2] >= (2 if w["name"] == "pour_2M" else 1)
+    for u, v in zip(a[0] + a[1], b[0] + b[1]):
+        assert np.array_equal(u, v)
+
+
+def test_tiled_search_crowded_tiles_fall_back():
+    # 2,000 spheres crowded into a cube of 6.6 d (about ten per cell): a
+    # tile's rim then holds more candidates than its LDS stage, those tiles
+    # scan global memory, and the lists overflow their 35 slots: reported,
+    # as with the one-thread search
+    w = W.small_bed(n=2000)
+    rng = np.random.default_rng(3)
+    pos = rng.uniform(0.0, 6.6, (2000, 3)) + np.array([-3.3, -3.3, 0.6])
+    for mode in (G.Engine.NSEARCH_PLAIN, G.Engine.NSEARCH_TILED):
+        e = G.Engine(w["inp"])
+        e.set_neighbor_search(mode)
+        e.set_particles(pos)
+        with pytest.raises(G.DemError) as err:
+            e.step(1)
+        assert err.value.code == -3
+        e.close()
+
+
 def test_radix_sort_edge_sizes():
     # tile boundaries of the sort (2048 keys per block) and a single particle
     for n in (1, 2, 2047, 2048, 2049, 5000):
