@@ -158,6 +158,152 @@ def sweep_main(args):
     print(json.dumps(out), flush=True)
 
 
+def c4_cpu_baseline(instances, threads, budget_s=12.0):
+    """The FP64 C oracle on a bounded sample of the batched set: whole steps
+    (rebuilds included) of the first instances from their fed states, about
+    budget_s of host time; particle-steps over time."""
+    from oracle import pyoracle as O
+
+    O.threads(threads)
+    total_ps, total_t, parts = 0, 0.0, []
+    for w in instances:
+        if total_t >= budget_s:
+            break
+        o = O.Oracle(w["inp"], w.get("dia"))
+        o.set_particles(w["pos"], w.get("vel"))
+        t0 = time.perf_counter()
+        o.step(10)
+        t10 = time.perf_counter() - t0
+        steps = int(max(10, min(3000, (budget_s / 3) / max(t10 / 10, 1e-6))))
+        t0 = time.perf_counter()
+        o.step(steps)
+        t = time.perf_counter() - t0
+        o.close()
+        n = len(w["pos"])
+        total_ps += n * steps
+        total_t += t
+        parts.append(f"{w['name']} ({n} spheres, {steps} steps)")
+    return dict(value=total_ps / total_t, unit="particle-steps/s", cores=threads, kind="port",
+                sample="FP64 C oracle (OpenMP, gather form): whole steps, rebuilds included, from "
+                       "the fed state of " + ", ".join(parts))
+
+
+def c4_main(args):
+    """Batched uneven instances in one launch per step: instances from
+    workloads.c4_instance, each autotuned (skin x block shape x storage
+    order), then `steps` steps of all of them from the feed (pour and
+    settle), timed whole.  Prints its own JSON line."""
+    import numpy as np
+    import gpucode_amd as G
+    from gpucode_amd import workloads as W
+
+    # default: FP32 pair forces with FP64 state
+    prec = G.PREC_FP64 if args.precision == "fp64" else G.PREC_MIXED
+    steps = args.steps if args.steps is not None else 30000
+    t0 = time.time()
+    ws = [W.c4_instance(k) for k in range(args.c4_instances)]
+    t_gen = time.time() - t0
+    n_tot = sum(len(w["pos"]) for w in ws)
+    log(f"{len(ws)} instances, {n_tot} spheres, fed in {t_gen:.1f}s")
+    t0 = time.time()
+    choices = None
+    if args.c4_tune_steps > 0:
+        choices = G.tune_instances(ws, prec, steps=args.c4_tune_steps, log=log)
+    t_tune = time.time() - t0
+    b = G.Batch.from_instances(ws, choices, prec)
+    b.set_schedule(device=True, slot_every=args.slot_every)
+    log(f"tuned in {t_tune:.1f}s; batch up")
+    n_k = np.array([len(w["pos"]) for w in ws], float)
+
+    def bytes_per_ps():
+        out = np.zeros(len(ws))
+        for k in range(len(ws)):
+            st = b[k].stats()
+            B, _, _, _ = algorithmic_bytes(st, st["n_particles"])
+            if ws[k]["inp"].forceCap == 1:
+                # lqdCnt old/new; lqdBrg r/w of bridged slots that do not touch
+                B += 8.0 + 8.0 * max(st["n_bridge"] - st["n_contact"], 0) / st["n_particles"]
+            out[k] = B
+        return out
+
+    nseg = max(1, steps // args.c4_segment)
+    seg = [steps // nseg + (1 if i < steps % nseg else 0) for i in range(nseg)]
+    B_prev = bytes_per_ps()
+    el = bytes_alg = 0.0
+    phases = []
+    tm0 = b.timing()
+    for i, k in enumerate(seg):
+        t0 = time.perf_counter()
+        b.step(k)
+        dt = time.perf_counter() - t0
+        el += dt
+        B_now = bytes_per_ps()
+        seg_bytes = float(np.sum(n_k * k * 0.5 * (B_prev + B_now)))
+        bytes_alg += seg_bytes
+        phases.append(dict(steps=k, seconds=dt, particle_steps_per_s=n_tot * k / dt,
+                           gbs_algorithmic=seg_bytes / dt / 1e9,
+                           bytes_per_particle_step=seg_bytes / (n_tot * k)))
+        log(f"segment {i + 1}/{nseg}: {k} steps in {dt:.2f}s, {n_tot * k / dt:.3e} "
+            f"particle-steps/s, {seg_bytes / (n_tot * k):.1f} B/particle-step")
+        B_prev = B_now
+    tm = b.timing()
+    sch = b.schedule_stats()
+    phi, Z, _ = b.analyze()
+    value = n_tot * steps / el
+    achieved = bytes_alg / el / 1e9
+    cpu = None
+    if not args.no_cpu_baseline:
+        threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+        cpu = c4_cpu_baseline(ws, threads)
+        cpu["nproc"] = os.cpu_count()
+        log(f"cpu baseline {cpu['value']:.3e} particle-steps/s on {threads} threads")
+    kinds = {}
+    if choices:
+        wave = sum(c["kernel"] == G.Engine.KERNEL_WAVE for c in choices)
+        morton = sum(c["morton"] for c in choices)
+        gaps = np.array([c["cutGap"] for c in choices])
+        kinds = {"wave": wave, "loop": len(choices) - wave, "morton": morton,
+                 "row_major": len(choices) - morton,
+                 "cutGap_quartiles": [float(q) for q in np.percentile(gaps, [0, 25, 50, 75, 100])]}
+    out = {
+        "metric": "batched particle-steps/s (uneven instances, one launch per step)",
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": 0,
+        "ms_per_step": el / steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if prec == G.PREC_FP64 else "f32-pair/f64-state",
+        "data": "synthetic",
+        "config": {"workload": f"{len(ws)} cylinder packings, N~U[1000,20000] "
+                               f"(total {n_tot}), mu U[0.1,0.8], liquid 0 (p=1/4) or "
+                               f"U(0,3%], theta U[0,60] deg, lognormal sigma U[0,0.4]; "
+                               f"{steps} steps from the feed, rebuilds included",
+                   "n_instances": len(ws), "n_particles": n_tot,
+                   "slot_every": args.slot_every, "parallelism": "single-gpu"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "note": "algorithmic bytes of every instance's steps (per-instance "
+                             "counters sampled between segments) over the whole timed run, "
+                             "rebuild slots included"},
+        "cpu_baseline": cpu,
+        "tuning": {"seconds": t_tune, "steps_per_candidate": args.c4_tune_steps,
+                   "choices": kinds,
+                   "ms_per_step_default_sum": sum(c["ms_default"] for c in choices) if choices else None,
+                   "ms_per_step_tuned_sum": sum(c["ms_per_step"] for c in choices) if choices else None},
+        "schedule": {"global_launches": tm["launches"] - tm0["launches"], "slots": sch["slots"],
+                     "host_syncs": sch["host_syncs"], "rebuilds": tm["n_rebuild"] - tm0["n_rebuild"]},
+        "phases": phases,
+        "packing": {"phi_mean": float(np.mean(phi)), "phi_min": float(np.min(phi)),
+                    "phi_max": float(np.max(phi)), "Z_mean": float(np.mean(Z))},
+    }
+    b.close()
+    print(json.dumps(out), flush=True)
+
+
 REBUILD_BYTES = {
     # algorithmic bytes per particle of each rebuild stage (n_nb neighbours,
     # n_c contacts, c_per_p cells per particle, passes of the radix sort)
@@ -176,7 +322,7 @@ REBUILD_BYTES = {
 def prepare(args, G, W):
     """The workload's engine, ready for the timed region, and its labels."""
     t0 = time.time()
-    prec = G.PREC_FP64 if args.precision == "fp64" else G.PREC_MIXED
+    prec = G.PREC_MIXED if args.precision == "mixed" else G.PREC_FP64
     kern = {"auto": G.Engine.KERNEL_AUTO, "wave": G.Engine.KERNEL_WAVE,
             "compact": G.Engine.KERNEL_COMPACT, "loop": G.Engine.KERNEL_LOOP}[args.kernel]
     info = {}
@@ -322,7 +468,9 @@ def main():
                          "many steps (0: never)")
     ap.add_argument("--rebuild-reps", type=int, default=3,
                     help="rebuilds of the final state timed stage by stage")
-    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
+    ap.add_argument("--precision", choices=["fp64", "mixed"], default=None,
+                    help="fp64, or mixed (FP32 pair forces, FP64 state); default fp64 for the "
+                         "single packings, mixed for --c4")
     ap.add_argument("--kernel", choices=["auto", "wave", "compact", "loop"], default="auto",
                     help="fused step kernel: per-wave contact compaction (auto, wave), "
                          "block-wide compaction (compact) or the one-thread loop (loop)")
@@ -335,6 +483,16 @@ def main():
     ap.add_argument("--sweep-steps", type=int, default=15000)
     ap.add_argument("--sweep-nmax", type=int, default=4000)
     ap.add_argument("--sweep-epochs", type=int, default=400)
+    ap.add_argument("--c4", action="store_true",
+                    help="instead of the 2M-sphere step: the batched uneven instances, "
+                         "autotuned, stepped together (prints its own JSON line)")
+    ap.add_argument("--c4-instances", type=int, default=256)
+    ap.add_argument("--c4-tune-steps", type=int, default=200,
+                    help="steps per autotuner candidate (0: no tuning)")
+    ap.add_argument("--c4-segment", type=int, default=5000,
+                    help="steps per timed segment (counters are sampled between segments)")
+    ap.add_argument("--slot-every", type=int, default=4,
+                    help="global steps between rebuild slots of the batched schedule")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -348,6 +506,8 @@ def main():
 
     if args.sweep:
         return sweep_main(args)
+    if args.c4:
+        return c4_main(args)
 
     P = prepare(args, G, W)
     e, w, desc, metric, tag, nseg = P["e"], P["w"], P["desc"], P["metric"], P["tag"], P["nseg"]

@@ -7,8 +7,8 @@ generators (workloads) and parameter calibration (calibrate).
 """
 from ._capi import (DemError, DemInput, DemSetup, DemStats, PREC_FP64, PREC_MIXED,  # noqa: F401
                     declared_symbols, lib)
-from .engine import Batch, Engine, autotune, derive, load_para, particle_feed, read_deck, write_tec  # noqa: F401
+from .engine import Batch, Engine, autotune, derive, load_para, particle_feed, read_deck, tune_instances, write_tec  # noqa: F401
 
 __all__ = ["Engine", "Batch", "autotune", "DemInput", "DemSetup", "DemStats", "DemError", "PREC_FP64", "PREC_MIXED",
            "derive", "read_deck", "particle_feed", "write_tec", "load_para", "lib",
-           "declared_symbols"]
+           "declared_symbols", "tune_instances"]

@@ -443,29 +443,49 @@ DEV void neighbor_find_one(const double4 *p, const int *level, const DevInst &P,
 // in registers from the counting pass; a row with more is walked again.
 #define NFG_KEEP 8
 
+DEV void nfg_hit(int j, int i, const double4 q, const double4 o, double ri, double gap, int *keep,
+                 bool write, int *nbr, int cap, int num0, int K, int &c) {
+    if (j == i) return;
+    double dx = o.x - q.x, dy2 = o.y - q.y, dz2 = o.z - q.z;
+    double len = sqrt(dx * dx + dy2 * dy2 + dz2 * dz2);
+    if (len < q.w + ri + gap) {
+        if (write) {
+            if (num0 + c < K) nbr[(long)(num0 + c) * cap + i] = j;
+        } else {
+#pragma unroll
+            for (int t = 0; t < NFG_KEEP; ++t)
+                if (t == c) keep[t] = j;
+        }
+        ++c;
+    }
+}
+
+// one row: the cell ranges of four cells at a time are loaded together,
+// candidates two at a time (the order of the hits is the scan order)
 DEV int nfg_row(const double4 *p, const DevInst &P, const int *cstart, const int *cend, int i,
                 const double4 o, double ri, int rowb, int sx, int ex, int *keep, bool write, int *nbr,
                 int cap, int num0, int K) {
     int c = 0;
-    for (int x = sx; x <= ex; ++x) {
-        const int s0 = cstart[rowb + x];
-        if (s0 == DEM_EMPTY) continue;
-        const int e0 = cend[rowb + x];
-        for (int j = s0; j < e0; ++j) {
-            if (j == i) continue;
-            const double4 q = p[j];
-            double dx = o.x - q.x, dy2 = o.y - q.y, dz2 = o.z - q.z;
-            double len = sqrt(dx * dx + dy2 * dy2 + dz2 * dz2);
-            if (len < q.w + ri + P.searchGap) {
-                if (write) {
-                    if (num0 + c < K) nbr[(long)(num0 + c) * cap + i] = j;
-                } else {
+    const double gap = P.searchGap;
+    for (int x0 = sx; x0 <= ex; x0 += 4) {
+        int s4[4], e4[4];
 #pragma unroll
-                    for (int t = 0; t < NFG_KEEP; ++t)
-                        if (t == c) keep[t] = j;
-                }
-                ++c;
+        for (int u = 0; u < 4; ++u) {
+            const bool in = x0 + u <= ex;
+            s4[u] = in ? cstart[rowb + x0 + u] : DEM_EMPTY;
+            e4[u] = in ? cend[rowb + x0 + u] : 0;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            if (s4[u] == DEM_EMPTY) continue;
+            const int e0 = e4[u];
+            int j = s4[u];
+            for (; j + 1 < e0; j += 2) {
+                const double4 q0 = p[j], q1 = p[j + 1];
+                nfg_hit(j, i, q0, o, ri, gap, keep, write, nbr, cap, num0, K, c);
+                nfg_hit(j + 1, i, q1, o, ri, gap, keep, write, nbr, cap, num0, K, c);
             }
+            if (j < e0) nfg_hit(j, i, p[j], o, ri, gap, keep, write, nbr, cap, num0, K, c);
         }
     }
     return c;

@@ -70,6 +70,37 @@ def autotune(inp, pos, dia=None, vel=None, steps=200, cut_gaps=None, precision=A
     return {f: getattr(r, f) for f, _ in A.DemTuneResult._fields_}, table
 
 
+def tune_instances(instances, precision=A.PREC_MIXED, steps=200, cut_gaps=(0.1, 0.2, 0.3, 0.5),
+                   log=None):
+    """dem_autotune on each instance's own packing: every skin of cut_gaps
+    (which sets the cell size and the list length) x block shape (the LOOP
+    kernel, one thread per particle, or the WAVE kernel, a wave per chunk of
+    64 with contact compaction) x storage order, timed for `steps` steps
+    from the instance's start.  Returns one dict per instance: cutGap,
+    kernel, morton, ms_per_step (the choice), ms_default (the instance's
+    own configuration)."""
+    from .workloads import min_cut_gap
+
+    out = []
+    for k, w in enumerate(instances):
+        # a wet packing's skins start at the smallest that reaches its
+        # bridges' rupture distance
+        g0 = min_cut_gap(w["inp"], w.get("dia"))
+        gaps = sorted({float(g) for g in cut_gaps if g >= g0} | ({g0, 1.5 * g0} if g0 > 0 else set()))
+        best, table = autotune(w["inp"], w["pos"], w.get("dia"), w.get("vel"), steps=steps,
+                               cut_gaps=gaps, precision=precision)
+        # a batch runs the LOOP and WAVE kernels; the COMPACT kernel (block
+        # compaction) is a single-instance form, so it is not a candidate
+        cand = [(table[s, kq, mo], s, kq, mo) for s in range(len(gaps)) for kq in (0, 2)
+                for mo in (0, 1) if table[s, kq, mo] >= 0]
+        ms, s, kq, mo = min(cand)
+        out.append(dict(cutGap=gaps[s], kernel=(Engine.KERNEL_LOOP, None, Engine.KERNEL_WAVE)[kq],
+                        morton=mo, ms_per_step=float(ms), ms_default=float(best["ms_default"])))
+        if log and (k + 1) % 32 == 0:
+            log(f"tuned {k + 1} of {len(instances)}")
+    return out
+
+
 def load_para(path):
     v = [C.c_int(), C.c_int(), C.c_int()]
     check(A.lib().dem_load_para(str(path).encode(), *[C.byref(x) for x in v]))
@@ -366,6 +397,25 @@ class Batch:
         check(A.lib().dem_batch_create(n, arr, dptrs, int(precision), C.byref(self._b)))
         self.inst = [Engine(inputs[k], None, precision, _ctx=C.c_void_p(
             A.lib().dem_batch_instance(self._b, k))) for k in range(n)]
+
+    @classmethod
+    def from_instances(cls, instances, choices=None, precision=A.PREC_FP64):
+        """A batch of workload instances (dicts with inp, dia, pos and
+        optionally vel), each with its tuned skin, step kernel and storage
+        order when choices (from tune_instances) are given, particles set."""
+        inputs = []
+        for k, w in enumerate(instances):
+            inp = A.DemInput.from_buffer_copy(w["inp"])
+            if choices is not None:
+                inp.cutGap = choices[k]["cutGap"]
+            inputs.append(inp)
+        b = cls(inputs, [w.get("dia") for w in instances], precision)
+        for k, w in enumerate(instances):
+            if choices is not None:
+                b[k].set_kernel(choices[k]["kernel"])
+                b[k].set_order(choices[k]["morton"])
+            b[k].set_particles(w["pos"], w.get("vel"))
+        return b
 
     def __len__(self):
         return len(self.inst)

@@ -60,6 +60,21 @@ def make_input(NP, d_max, R, H, mu, *, E=70e9, nu=0.25, rho=2500.0, e=0.7, yield
     return inp
 
 
+def min_cut_gap(inp, dia=None, margin=1.02):
+    """Smallest cutGap whose neighbour search reaches the largest capillary
+    rupture distance (two of the largest spheres, each holding the liquid of
+    12 bridges), times margin; 0 for a dry packing.  The search gap is
+    linear in cutGap (searchGap = cutGap x the smallest diameter)."""
+    if inp.forceCap != 1:
+        return 0.0
+    probe = A.DemInput.from_buffer_copy(inp)
+    probe.cutGap = 1.0
+    S = derive(probe, dia)[0]
+    V = 2.0 * S.vol / 12.0
+    rup = (1.0 + 0.5 * S.theta) * V ** (1.0 / 3.0) if S.ruptureMode == 1 else 0.99 * V ** 0.33
+    return margin * rup / S.searchGap
+
+
 def feed_positions(inp, dia, n, seed):
     """Initial positions from the engine's ParticleFeed, called until n
     particles are placed (one call places at most 60,000, as in the
@@ -325,3 +340,76 @@ def sweep_instance(x, n, seed, *, d=1.0e-3, E=1.0e6, bed_height_d=20.0, fill=3.0
                      rupture_mode=1, mu_wall=mu_w)
     pos = feed_positions(inp, dia, n, seed + 1)
     return dict(name="sweep", inp=inp, dia=dia, pos=pos, x=np.asarray(x, float))
+
+
+# --- batched uneven instances (the autotuned multi-instance launch) ----------
+C4_RANGES = {
+    "n": (1000, 20000),          # particles, uniform integers
+    "mu": (0.1, 0.8),            # sliding friction (particle-particle and wall)
+    "liquid": (0.0, 0.03),       # liquid volume per particle volume
+    "theta_deg": (0.0, 60.0),    # contact angle
+    "sigma": (0.0, 0.4),         # lognormal sigma of the diameters
+}
+
+
+def c4_instance(k, seed=0xC4, *, n=None, d_median=0.5e-3, E=1.0e6, e=0.7, tension=0.02, v0=0.5,
+                bed_height_d=25.0, fill=4.0, dry_fraction=0.25):
+    """Instance k of the batched uneven set: N ~ U[1,000, 20,000] spheres
+    and seeded parameters drawn from C4_RANGES.  Liquid fraction: 0 (a dry
+    packing) with probability dry_fraction, else uniform in (0, 3 %], so
+    both step kernels are exercised.  Diameters lognormal with median 0.5
+    mm and the instance's sigma, clipped to 0.25-1.0 mm (sigma = 0:
+    monodisperse); one size level up to a diameter ratio of 1.5, two up to
+    2.5, else three.  Soft spheres (E = 1 MPa, restitution 0.7) give a
+    time step of 2e-6 to 4.5e-6 s.  Surface tension 0.02 N/m (an alcohol's;
+    with soft spheres, water's 0.072 N/m would press bridged spheres into
+    each other by several per cent of d, while 0.02 keeps that below 1 %).
+    The cylinder is sized for a settled bed about bed_height_d mean
+    diameters tall (radius clamped to 5-15 of them) and is fill times that
+    tall (the random sequential feed needs that room); the fed spheres
+    start downward at v0 m/s, so they land within about 0.05 s and the
+    30,000 steps pour and settle.  Capillary bridges rupture at
+    (1 + theta/2) V^(1/3).  n, if given, replaces the drawn particle count
+    (the other parameters stay those of instance k)."""
+    rng = np.random.default_rng([seed, k])
+    lo, hi = C4_RANGES["n"]
+    n_drawn = int(rng.integers(lo, hi + 1))
+    n = n_drawn if n is None else int(n)
+    mu = float(rng.uniform(*C4_RANGES["mu"]))
+    liquid = 0.0 if rng.random() < dry_fraction else float(rng.uniform(1e-4, C4_RANGES["liquid"][1]))
+    theta_deg = float(rng.uniform(*C4_RANGES["theta_deg"]))
+    sigma = float(rng.uniform(*C4_RANGES["sigma"]))
+    dseed = int(rng.integers(1 << 31))
+    if sigma < 1e-3:
+        dia, d = None, d_median
+        ratio = 1.0
+    else:
+        dia = lognormal_diameters(n, dseed, median=d_median, sigma=sigma)
+        d = float(dia.max())
+        ratio = d / float(dia.min())
+    nLevels = 1 if ratio <= 1.5 else (2 if ratio <= 2.5 else 3)
+    # the bed is sized in mean diameters dm (= d when monodisperse)
+    dm = d if dia is None else float(np.mean(dia))
+    rel3 = 1.0 if dia is None else float(np.mean((dia / dm) ** 3))
+    vol_per = (math.pi / 6) / 0.6 * rel3
+    R_m = min(15.0, max(5.0, math.sqrt(n * vol_per / (math.pi * bed_height_d))))
+    R_d = R_m * dm / d
+    H = dm * fill * (n * vol_per / (math.pi * R_m * R_m) + 4)
+    inp = make_input(n, d, R_d * d, H, mu, E=E, e=e, cutGap=0.2, nLevels=nLevels,
+                     forceCap=1 if liquid > 0 else 0, liquid=liquid, tension=tension,
+                     theta=math.radians(theta_deg), rupture_mode=1)
+    if dia is not None:
+        # level bounds as the default rule draws them, given explicitly so
+        # that the top one is the largest diameter itself, not a rounding of it
+        dmin = float(dia.min())
+        inp.sizeLevelInput = 1
+        for l in range(nLevels):
+            inp.levelDia[l] = d if l == nLevels - 1 else dmin + (d - dmin) / nLevels * (l + 1)
+    # the lists must reach every bridge until it ruptures
+    inp.cutGap = max(inp.cutGap, min_cut_gap(inp, dia))
+    pos = feed_positions(inp, dia, n, dseed + 1)
+    vel = np.zeros_like(pos)
+    vel[:, 2] = -v0 / derive(inp, dia)[0].rvunit
+    return dict(name=f"c4_{k}", inp=inp, dia=dia, pos=pos, vel=vel,
+                params=dict(n=n, mu=mu, liquid=liquid, theta_deg=theta_deg, sigma=sigma,
+                            nLevels=nLevels, R_d=R_d))

@@ -340,3 +340,46 @@ def test_autotune_skips_skins_shorter_than_capillary_bridges():
     # wet packings run the one-thread loop only
     assert (table[1, 0] > 0).all() and (table[1, 1:] == -1).all()
     assert best["cutGap"] == 0.5 and best["kernel"] == G.Engine.KERNEL_LOOP
+
+
+@pytest.mark.gpu
+def test_autotuned_batched_instances_equal_the_oracle():
+    # four instances of the batched uneven set at 1,500 spheres each (dry
+    # monodisperse, dry three-level polydisperse, wet at theta 54 deg, wet
+    # polydisperse at liquid 3 %), each with the skin, step kernel and
+    # storage order the autotuner picks, stepped together in FP64 for 1,000
+    # steps of their pour (rebuilds included) by the device-driven schedule:
+    # every instance follows the oracle run with the same input (lists
+    # identical, states within FP64 rounding of the oracle's)
+    from oracle import pyoracle as O
+
+    ks = (30, 16, 34, 5)
+    ws = [W.c4_instance(k, n=1500) for k in ks]
+    assert ws[0]["inp"].forceCap == 0 and ws[1]["inp"].forceCap == 0 and ws[1]["inp"].nLevels == 3
+    assert ws[2]["inp"].forceCap == 1 and ws[3]["inp"].forceCap == 1
+    choices = G.tune_instances(ws, G.PREC_FP64, steps=50)
+    b = G.Batch.from_instances(ws, choices, G.PREC_FP64)
+    b.set_schedule(device=True, slot_every=4)
+    steps = 1000
+    b.step(steps)
+    for k, w in enumerate(ws):
+        inp = G.DemInput.from_buffer_copy(w["inp"])
+        inp.cutGap = choices[k]["cutGap"]
+        o = O.Oracle(inp, w["dia"])
+        o.set_particles(w["pos"], w["vel"])
+        o.step(steps)
+        e = b[k]
+        print(ks[k], choices[k], "rebuilds", e.n_rebuild, o.n_rebuild)
+        assert e.iter == o.iter == steps and e.n_rebuild == o.n_rebuild
+        p1, v1, w1 = e.particles()
+        p2, v2, w2 = o.particles()
+        assert np.abs(p1 - p2).max() <= 1e-9
+        assert np.abs(v1 - v2).max() <= 1e-7 * max(np.abs(v2).max(), 1e-12)
+        assert np.abs(w1 - w2).max() <= 1e-6 * max(np.abs(w2).max(), 1e-12)
+        n1, l1 = e.nebor()
+        n2, l2 = o.nebor()
+        assert np.array_equal(n1, n2) and np.array_equal(l1, l2)
+        if w["inp"].forceCap == 1:
+            assert np.array_equal(e.lqdcnt(), o.lqdcnt())
+        o.close()
+    b.close()
