@@ -122,6 +122,7 @@ _SIGS = {
     "dem_cir_pw_calc": (C.c_int, [P]),
     "dem_update_system": (C.c_int, [P, I]),
     "dem_rebuild": (C.c_int, [P]),
+    "dem_flush_rebuild": (C.c_int, [P]),
     "dem_step_unfused": (C.c_int, [P]),
     "dem_step": (C.c_int, [P, C.c_int]),
     "dem_time_steps": (C.c_int, [P, C.c_int, D, I]),
@@ -167,6 +168,7 @@ _SIGS = {
     "dem_batch_instance": (P, [P, C.c_int]),
     "dem_batch_step": (C.c_int, [P, C.c_int]),
     "dem_batch_set_schedule": (C.c_int, [P, C.c_int, C.c_int]),
+    "dem_batch_flush_rebuilds": (C.c_int, [P]),
     "dem_batch_schedule_stats": (C.c_int, [P, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]),
     "dem_batch_timing": (C.c_int, [P, D, C.POINTER(C.c_longlong), D, C.POINTER(C.c_longlong)]),
     "dem_batch_analyze": (C.c_int, [P, D, D, D]),

@@ -528,6 +528,21 @@ static int batch_step_device(dem_batch *b, int nsteps) {
     return DEM_OK;
 }
 
+extern "C" int dem_batch_flush_rebuilds(dem_batch *b) {
+    if (!b) return dem_fail(DEM_ERR_ARG, "null batch");
+    const int n = (int)b->inst.size();
+    for (int k = 0; k < n; ++k) {
+        dem_ctx *c = b->inst[k];
+        if (c->rebuild_pending) {
+            int rc = dem_pair_unpack(c);
+            if (rc) return rc;
+        }
+    }
+    int rc = upload_insts(b);
+    if (rc) return rc;
+    return batch_rebuilds(b, std::vector<int>(n, 1));
+}
+
 extern "C" int dem_batch_set_schedule(dem_batch *b, int device, int slot_every) {
     if (!b || slot_every < 1) return dem_fail(DEM_ERR_ARG, "bad argument");
     b->device_schedule = device != 0;

@@ -773,6 +773,12 @@ extern "C" int dem_rebuild(dem_ctx *c) {
     return check_overflow(c);
 }
 
+extern "C" int dem_flush_rebuild(dem_ctx *c) {
+    if (!c) return dem_fail(DEM_ERR_ARG, "null context");
+    if (!c->rebuild_pending) return DEM_OK;
+    return dem_rebuild(c);
+}
+
 extern "C" int dem_key_bits(dem_ctx *c) { return c ? c->key_bits : 0; }
 
 extern "C" int dem_set_neighbor_search(dem_ctx *c, int mode) {

@@ -250,6 +250,10 @@ class Engine:
     def rebuild(self):
         check(A.lib().dem_rebuild(self._ctx))
 
+    def flush_rebuild(self):
+        """The rebuild the next step would start with, done now."""
+        check(A.lib().dem_flush_rebuild(self._ctx))
+
     def step(self, n=1):
         check(A.lib().dem_step(self._ctx, int(n)))
 
@@ -441,6 +445,9 @@ class Batch:
 
     def __exit__(self, *a):
         self.close()
+
+    def flush_rebuilds(self):
+        check(A.lib().dem_batch_flush_rebuilds(self._b))
 
     def set_schedule(self, device=True, slot_every=4):
         check(A.lib().dem_batch_set_schedule(self._b, int(bool(device)), int(slot_every)))

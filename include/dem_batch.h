@@ -44,6 +44,8 @@ int dem_batch_step(dem_batch *b, int nsteps);
  * the rebuild requests read back by the host.  Either way every instance
  * takes exactly the steps and rebuilds it takes alone. */
 int dem_batch_set_schedule(dem_batch *b, int device, int slot_every);
+/* dem_flush_rebuild for every instance, as one batched rebuild. */
+int dem_batch_flush_rebuilds(dem_batch *b);
 /* Rebuild slots run and host synchronisations of the device-driven
  * schedule so far (no counterpart in the reference). */
 int dem_batch_schedule_stats(dem_batch *b, long long *slots, long long *host_syncs);

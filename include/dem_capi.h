@@ -293,6 +293,11 @@ int dem_scale_velocities(dem_ctx *ctx, double factor);
 /* Bits of the cell key the radix sort orders (8 per pass); a measurement
  * aid with no reference counterpart. */
 int dem_key_bits(dem_ctx *ctx);
+/* The rebuild the next step would start with, done now (nothing when none
+ * is due): lets a caller end a timed run with the rebuild its last step
+ * asked for.  No counterpart in the reference, whose loop rebuilds at the
+ * top of the next step (main.cpp:470-527). */
+int dem_flush_rebuild(dem_ctx *ctx);
 /* Bytes of device memory held by the context. */
 long long dem_device_bytes(dem_ctx *ctx);
 /* Name of the step kernel dem_step launches for this context's packing
