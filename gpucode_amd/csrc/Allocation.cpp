@@ -96,6 +96,15 @@ void dem_fill_inst(const dem_setup &S, DevInst &P) {
         P.cellOff[l] = l ? S.ncellIdx[l - 1] : 0;
         P.levelK[l] = S.levelNNebors[l];
     }
+    long long tab = 0;
+    for (int l = 0; l < S.nLevels; ++l) {
+        P.bbx[l] = (P.gx[l] + 3) / 4;
+        P.bby[l] = (P.gy[l] + 3) / 4;
+        P.blkOff[l] = (int)tab;
+        tab += (long long)P.bbx[l] * P.bby[l] * ((P.gz[l] + 3) / 4) * 64;
+        if (tab > 0x7fffffffLL) tab = 0x7fffffffLL;  // refused by dem_create
+    }
+    P.ncellTab = (int)tab;
 }
 
 // Sort-key layout: Morton codes of the cells when every level grid fits 10
@@ -166,8 +175,8 @@ static int alloc_all(dem_ctx *c) {
             (rc = dalloc(c, &c->hw[b], cap * DEM_WALLS)))
             return rc;
     }
-    if ((rc = dalloc(c, &c->cstart, (size_t)c->S.sysNCells)) ||
-        (rc = dalloc(c, &c->cend, (size_t)c->S.sysNCells)) || (rc = dalloc(c, &c->keys, cap)) ||
+    if ((rc = dalloc(c, &c->cstart, (size_t)c->P.ncellTab)) ||
+        (rc = dalloc(c, &c->cend, (size_t)c->P.ncellTab)) || (rc = dalloc(c, &c->keys, cap)) ||
         (rc = dalloc(c, &c->vals, cap)) || (rc = dalloc(c, &c->keys2, cap)) ||
         (rc = dalloc(c, &c->vals2, cap)) || (rc = dalloc(c, &c->nmask, cap)) ||
         (rc = dalloc(c, &c->sort_tmp, (size_t)radix_tmp_elems((long)cap))))
@@ -230,6 +239,10 @@ int dem_create_on(const dem_input *in, const double *dia_m, int precision, hipSt
     }
     for (int l = 0; l < c->S.nLevels; ++l) c->K = c->S.levelNNebors[l] > c->K ? c->S.levelNNebors[l] : c->K;
     dem_refresh_inst(c);
+    if (c->P.ncellTab >= 0x7fffffff) {
+        delete c;
+        return dem_fail(DEM_ERR_ARG, "cell grids too large");
+    }
     c->Tw = c->S.Tw;
     if (stream) {
         c->stream = stream;

@@ -236,7 +236,7 @@ static void rebuild_entry(dem_ctx *c, int k, RbInst &R) {
     R.lvlb = c->level[1 - c->pcur]; R.idb = c->id[1 - c->pcur];
     R.keys = c->keys; R.vals = c->vals; R.keys2 = c->keys2; R.vals2 = c->vals2;
     R.passes = (c->key_bits + 7) / 8 < 1 ? 1 : (c->key_bits + 7) / 8;
-    R.cstart = c->cstart; R.cend = c->cend; R.ncells = c->S.sysNCells;
+    R.cstart = c->cstart; R.cend = c->cend; R.ncells = c->P.ncellTab;
     const int o = 1 - c->hcur;
     R.nbr = c->nbr[c->hcur]; R.nnb = c->nnb[c->hcur]; R.nbr_old = c->nbr[o]; R.nnb_old = c->nnb[o];
     const int ol = 1 - c->hl;
@@ -287,7 +287,7 @@ static int batch_rebuilds(dem_batch *b, const std::vector<int> &left) {
         rebuild_entry(c, k, b->h_rb[nr]);
         b->h_rb[nr].overflow = b->d_ovf + nr;
         for (int q = 0; q < (c->total + 255) / 256; ++q) pt.push_back(make_int2(nr, q));
-        for (int q = 0; q < (c->S.sysNCells + 255) / 256; ++q) ct.push_back(make_int2(nr, q));
+        for (int q = 0; q < (c->P.ncellTab + 255) / 256; ++q) ct.push_back(make_int2(nr, q));
         ++nr;
     }
     if (done.empty() && single.empty()) return DEM_OK;
@@ -361,7 +361,7 @@ static int static_rebuild_tasks(dem_batch *b) {
         for (int q = 0; q < (b->inst[k]->total + 255) / 256; ++q) t.push_back(make_int2(k, q));
     const int np = (int)t.size();
     for (int k = 0; k < (int)b->inst.size(); ++k)
-        for (int q = 0; q < (b->inst[k]->S.sysNCells + 255) / 256; ++q) t.push_back(make_int2(k, q));
+        for (int q = 0; q < (b->inst[k]->P.ncellTab + 255) / 256; ++q) t.push_back(make_int2(k, q));
     (void)hipFree(b->d_stask);
     b->d_stask = nullptr;
     HIPCHK(hipMalloc((void **)&b->d_stask, t.size() * sizeof(int2)));
@@ -395,7 +395,7 @@ static int batch_step_device(dem_batch *b, int nsteps) {
         D.cstart = c->cstart; D.cend = c->cend; D.nmask = c->nmask;
         D.overflow = c->overflow;
         D.stop_at = b->d_stop + k;
-        D.cap = c->cap; D.total = c->total; D.K = c->K; D.ncells = c->S.sysNCells;
+        D.cap = c->cap; D.total = c->total; D.K = c->K; D.ncells = c->P.ncellTab;
         D.passes = (c->key_bits + 7) / 8 < 1 ? 1 : (c->key_bits + 7) / 8;
         D.dTw = c->S.topv * c->S.dt;
         DBatchCtl &C = b->h_ctl[k];

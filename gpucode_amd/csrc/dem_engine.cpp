@@ -457,27 +457,38 @@ extern "C" int dem_set_contacts(dem_ctx *c, int maxK, const int *num, const int 
 }
 
 // The device tables give each cell's range in storage order (row-major or
-// Morton); the reference's tables (thrust-sorted (hash, index) pairs and
-// CellStart/CellEnd into them) follow from walking the cells in row-major
-// order, since a cell's particles are stored by original index either way.
+// Morton), in the blocked cell layout (cell_slot); the reference's tables
+// (thrust-sorted (hash, index) pairs and CellStart/CellEnd into them)
+// follow from walking the cells in row-major order, since a cell's
+// particles are stored by original index either way.
 static int ref_tables(dem_ctx *c, int *hash, int *index, int *cellStart, int *cellEnd) {
-    const size_t nc = (size_t)c->S.sysNCells;
-    std::vector<int> cs(nc), ce(nc), ids;
+    const size_t nt = (size_t)c->P.ncellTab, nc = (size_t)c->S.sysNCells;
+    std::vector<int> cs(nt), ce(nt), ids;
     int rc;
-    if ((rc = download(c, cs.data(), c->cstart, nc)) || (rc = download(c, ce.data(), c->cend, nc)) ||
+    if ((rc = download(c, cs.data(), c->cstart, nt)) || (rc = download(c, ce.data(), c->cend, nt)) ||
         (rc = host_ids(c, ids)))
         return rc;
+    // the table entry of every row-major cell, level by level
+    std::vector<int> slot(nc);
+    for (int L = 0; L < c->S.nLevels; ++L) {
+        const int gx = c->P.gx[L], gy = c->P.gy[L], gz = c->P.gz[L];
+        size_t h = (size_t)c->P.cellOff[L];
+        for (int z = 0; z < gz; ++z)
+            for (int y = 0; y < gy; ++y)
+                for (int x = 0; x < gx; ++x) slot[h++] = cell_slot(c->P, L, x, y, z);
+    }
     int acc = 0;
     for (size_t cell = 0; cell < nc; ++cell) {
-        if (cs[cell] == DEM_EMPTY) {
+        const int t = slot[cell];
+        if (cs[t] == DEM_EMPTY) {
             if (cellStart) cellStart[cell] = DEM_EMPTY;
             if (cellEnd) cellEnd[cell] = DEM_EMPTY;
             continue;
         }
-        const int n = ce[cell] - cs[cell];
+        const int n = ce[t] - cs[t];
         if (cellStart) cellStart[cell] = acc;
         if (cellEnd) cellEnd[cell] = acc + n;
-        for (int q = cs[cell]; q < ce[cell]; ++q, ++acc) {
+        for (int q = cs[t]; q < ce[t]; ++q, ++acc) {
             if (hash) hash[acc] = (int)cell;
             if (index) index[acc] = ids[q];
         }
@@ -664,7 +675,7 @@ extern "C" int dem_bld_nebor_list(dem_ctx *c) {
     c->Lidx = 1 - c->Lidx;
     c->pcur = 1 - c->pcur;
     launch_cell_range(c->keys, c->buf(c->cur, c->Lidx), c->level[c->pcur], c->P, c->cstart,
-                      c->cend, c->S.sysNCells, n, s);
+                      c->cend, c->P.ncellTab, n, s);
     rb_mark(c, 5);
     launch_fill_i32(c->overflow, 0, 1, s);
     // the LDS-staged (tiled) search pays where tiles hold many particles:

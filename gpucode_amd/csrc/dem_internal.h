@@ -65,7 +65,27 @@ struct DevInst {
     // cell are stored by original index.
     int morton;
     unsigned mortonOff[DEM_MAXL];  // key offset of each level's Morton range
+    // The CellStart/CellEnd tables are stored in blocks of 4 x 4 x 4 cells:
+    // block (x/4, y/4, z/4) of level L in row-major block order from
+    // blkOff[L], the 64 cells of a block row-major inside it.  The 27 cells
+    // a particle scans then lie in one to eight 256-byte blocks, which the
+    // particles stored next to it (Morton order keeps a block's particles
+    // together) read as well.  bbx, bby: blocks per row and per layer.
+    int bbx[DEM_MAXL], bby[DEM_MAXL], blkOff[DEM_MAXL];
+    int ncellTab;              // entries of each table (>= the reference's cell count)
 };
+
+// table entry of cell (x, y, z) of level L (see DevInst::blkOff)
+__host__ __device__ inline int cell_slot(const DevInst &P, int L, int x, int y, int z) {
+    return P.blkOff[L] + (((z >> 2) * P.bby[L] + (y >> 2)) * P.bbx[L] + (x >> 2)) * 64 +
+           (((z & 3) << 4) | ((y & 3) << 2) | (x & 3));
+}
+// first entry of the cells (x, y, z) of one row, x = 0, 4, 8, ...; cell x
+// is at row_slot + (x >> 2) * 64 + (x & 3)
+__host__ __device__ inline int row_slot(const DevInst &P, int L, int y, int z) {
+    return P.blkOff[L] + ((z >> 2) * P.bby[L] + (y >> 2)) * P.bbx[L] * 64 +
+           (((z & 3) << 4) | ((y & 3) << 2));
+}
 
 // Gap at a rebuild above which a pair cannot touch before the next rebuild:
 // until then each centre stays within rebuildThr of where it was, so a gap

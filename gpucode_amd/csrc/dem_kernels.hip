@@ -343,17 +343,21 @@ void launch_permute(const unsigned *perm, const StateBuf &a, const StateBuf &b, 
 // ---------------------------------------------------------------------
 // cell ranges (kernel.cu:682-724)
 // ---------------------------------------------------------------------
-// CellStart/CellEnd indexed by the row-major cell hash; a cell's particles
-// are contiguous in storage in either order (equal sort keys = same cell)
+// CellStart/CellEnd in the blocked cell layout (cell_slot); a cell's
+// particles are contiguous in storage in either order (equal sort keys =
+// same cell)
 DEV void cell_range_one(const unsigned *keys, const double4 *p, const int *level, const DevInst &P,
                         int *cstart, int *cend, int total, int k) {
     const unsigned key = keys[k];
     const bool first = k == 0 || key != keys[k - 1];
     const bool last = k == total - 1 || key != keys[k + 1];
     if (!first && !last) return;
-    int h;
-    unsigned kk;
-    cell_of(P, p[k], level[k], h, kk);
+    const double4 q = p[k];
+    const int L = level[k];
+    const double cs = P.cellSize[L];
+    const int h = cell_slot(P, L, cell_coord(q.x, P.origin[0], cs, P.gx[L]),
+                            cell_coord(q.y, P.origin[1], cs, P.gy[L]),
+                            cell_coord(q.z, P.origin[2], cs, P.gz[L]));
     if (first) cstart[h] = k;
     if (last) cend[h] = k + 1;
 }
@@ -389,7 +393,7 @@ DEV void neighbor_find_one(const double4 *p, const int *level, const DevInst &P,
     int num = 0;
     for (int L = 0; L < P.nLevels; ++L) {
         const double cs = P.cellSize[L];
-        const int gx = P.gx[L], gy = P.gy[L], gz = P.gz[L], coff = P.cellOff[L];
+        const int gx = P.gx[L], gy = P.gy[L], gz = P.gz[L];
         const int cx = cell_coord(o.x, P.origin[0], cs, gx);
         const int cy = cell_coord(o.y, P.origin[1], cs, gy);
         const int cz = cell_coord(o.z, P.origin[2], cs, gz);
@@ -403,14 +407,15 @@ DEV void neighbor_find_one(const double4 *p, const int *level, const DevInst &P,
                 int sx = cx - off, ex = cx + off;
                 if (sx < 0) sx = 0;
                 if (ex > gx - 1) ex = gx - 1;
-                const int rowb = coff + zz * gy * gx + yy * gx;
+                const int rowb = row_slot(P, L, yy, zz);
                 // the reference scans the sorted range from the first to the
                 // last non-empty cell of the row; cell by cell in x order is
                 // the same sequence, and works for either storage order
                 for (int x = sx; x <= ex; ++x) {
-                    const int s0 = cstart[rowb + x];
+                    const int h = rowb + ((x >> 2) << 6) + (x & 3);
+                    const int s0 = cstart[h];
                     if (s0 == DEM_EMPTY) continue;
-                    const int e0 = cend[rowb + x];
+                    const int e0 = cend[h];
                     for (int j = s0; j < e0; ++j) {
                         if (j == i) continue;
                         const double4 q = p[j];
@@ -472,8 +477,9 @@ DEV int nfg_row(const double4 *p, const DevInst &P, const int *cstart, const int
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             const bool in = x0 + u <= ex;
-            s4[u] = in ? cstart[rowb + x0 + u] : DEM_EMPTY;
-            e4[u] = in ? cend[rowb + x0 + u] : 0;
+            const int x = x0 + u, h = rowb + ((x >> 2) << 6) + (x & 3);
+            s4[u] = in ? cstart[h] : DEM_EMPTY;
+            e4[u] = in ? cend[h] : 0;
         }
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
@@ -538,7 +544,7 @@ DEV void neighbor_find_group16(const double4 *p, const int *level, const DevInst
                 ex = cx + off;
                 if (sx < 0) sx = 0;
                 if (ex > gx - 1) ex = gx - 1;
-                rowb = P.cellOff[L] + zz * gy * gx + yy * gx;
+                rowb = row_slot(P, L, yy, zz);
             }
         }
         int keep[NFG_KEEP];
@@ -666,7 +672,7 @@ __global__ void __launch_bounds__(NF_BS) k_neighbor_tiles(const double4 *p, cons
                       z = z0 + tid / (NF_EXT * NF_EXT);
             int g0 = 0;
             if (x >= 0 && x < gx && y >= 0 && y < gy && z >= 0 && z < gz) {
-                const int h = P.cellOff[0] + z * gy * gx + y * gx + x;
+                const int h = cell_slot(P, 0, x, y, z);
                 const int s0 = cstart[h], e0 = cend[h];
                 if (s0 != DEM_EMPTY) {
                     g0 = s0;
