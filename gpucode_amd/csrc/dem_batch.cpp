@@ -417,7 +417,7 @@ static int batch_step_device(dem_batch *b, int nsteps) {
     // hands the rebuilt instances their new control records, and they
     // resume with block j + 2.  Nothing returns to the host in between; it
     // looks at the control records only to see whether every instance is
-    // through (every 16 blocks once nsteps global steps have run).
+    // through (after each block once nsteps global steps have run).
     const int R = b->slot_every < 1 ? 1 : b->slot_every;
     // an instance that rebuilds after every step takes 2R global steps per step
     const long long gmax = ((long long)nsteps + 1) * 2 * R;
@@ -438,7 +438,9 @@ static int batch_step_device(dem_batch *b, int nsteps) {
         }
         HIPCHK(hipGetLastError());
         ++blocks;
-        if ((G >= nsteps && blocks - last_check >= 16) || blocks - last_check >= 256 || G >= gmax) {
+        // from nsteps global steps on, an instance is at most two blocks
+        // from its end (a rebuild slot and the block beside it)
+        if (G >= nsteps || blocks - last_check >= 256 || G >= gmax) {
             last_check = blocks;
             HIPCHK(hipMemcpyAsync(b->h_ctl, b->d_ctl, (size_t)n * sizeof(DBatchCtl),
                                   hipMemcpyDeviceToHost, b->stream));

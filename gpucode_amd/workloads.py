@@ -298,18 +298,23 @@ def small_bed(n=3000, R_d=8.0, mu=0.5, seed=7, dt=None, jitter=0.03):
 
 
 # --- parameter sweeps (batched instances) -------------------------------------
-SWEEP_PARAMS = ("mu", "mu_wall", "rfrc", "liquid", "theta_deg", "tension", "restitution",
-                "poly_width")
+# The eight inputs of the screened sweep: sliding friction, restitution,
+# liquid fraction, contact angle, lognormal sigma of the diameters, Young's
+# modulus, cylinder-to-particle radius ratio and feed rate.
+SWEEP_PARAMS = ("mu", "restitution", "liquid", "theta_deg", "sigma", "log10_E", "R_over_r",
+                "feed_frac")
 SWEEP_RANGES = {
-    "mu": (0.05, 0.8),           # particle-particle sliding friction
-    "mu_wall": (0.05, 0.8),      # particle-wall sliding friction
-    "rfrc": (0.0, 0.1),          # rolling friction coefficient
-    "liquid": (0.001, 0.03),     # liquid volume per particle volume
-    "theta_deg": (0.0, 40.0),    # contact angle
-    "tension": (0.02, 0.08),     # surface tension [N/m]
+    "mu": (0.1, 0.8),            # sliding friction (particle-particle and wall)
     "restitution": (0.3, 0.9),   # normal restitution at 0.5 m/s
-    "poly_width": (0.0, 0.3),    # diameters uniform in [(1-w) d, d]
+    "liquid": (0.0, 0.03),       # liquid volume per particle volume
+    "theta_deg": (0.0, 60.0),    # contact angle
+    "sigma": (0.0, 0.4),         # lognormal sigma of the diameters
+    "log10_E": (5.0, 6.0),       # Young's modulus 0.1-1 MPa (soft spheres)
+    "R_over_r": (10.0, 30.0),    # cylinder radius / median particle radius
+    "feed_frac": (0.1, 0.5),     # the feed spans this fraction of the run (rate = N / (f T))
 }
+SWEEP_D = 2.0e-3        # median diameter [m]
+SWEEP_BATCHES = 10      # feed batches per packing
 
 
 def sweep_point(rng):
@@ -317,29 +322,93 @@ def sweep_point(rng):
     return np.array([rng.uniform(*SWEEP_RANGES[k]) for k in SWEEP_PARAMS])
 
 
-def sweep_instance(x, n, seed, *, d=1.0e-3, E=1.0e6, bed_height_d=20.0, fill=3.0):
-    """A wet pour-and-settle packing for sweep point x (SWEEP_PARAMS order):
-    n spheres of largest diameter d, soft (E = 1 MPa) so that dt is about
-    1e-5 s, fed at random into a cylinder sized for a settled bed of about
-    bed_height_d diameters (radius clamped to 5-15 d), filling fill times
-    that height; capillary bridges on (contact angle theta, rupture
-    (1 + theta/2) V^(1/3))."""
-    mu, mu_w, rfrc, liquid, theta_deg, tension, e, w = [float(v) for v in x]
+def sweep_instance(x, seed, *, steps=30000, bed_d=8.0, batches=SWEEP_BATCHES, tension=0.072):
+    """The pour-and-settle packing of sweep point x (SWEEP_PARAMS order).
+
+    Spheres of median diameter 2 mm, lognormal with the point's sigma and
+    clipped to 1-4 mm (one to three size levels), in a cylinder of radius
+    R_over_r median radii; N fills a settled bed about bed_d largest
+    diameters tall (phi 0.6), and the cylinder is 1.8 times that tall.  The
+    spheres are fed as the reference feeds them (main.cpp:436-460,
+    ParticleFeed): placed at random, at rest, in the upper half of the
+    cylinder, in `batches` equal batches, one every feed_frac * steps /
+    batches steps, so the feed spans feed_frac of the run (feed rate N /
+    (feed_frac T) spheres per second).  A batch that does not fit in the
+    feed zone (the spheres fed before have not yet fallen out of it) is
+    completed at the next chunk boundaries: a fast feed is choked by the
+    zone, as a real hopper's would be.  Capillary bridges (liquid fraction,
+    contact angle, gamma 0.072 N/m, rupture (1 + theta/2) V^(1/3)) when the
+    liquid fraction is above 1e-4.  Returns the deck, diameters, the feed
+    schedule (step, count) and the feed seed; positions come from
+    feed_batch as the run reaches each batch."""
+    mu, e, liquid, theta_deg, sigma, lg_E, R_over_r, feed_frac = [float(v) for v in x]
     rng = np.random.default_rng(seed)
-    dia = None
-    nLevels = 1
-    if w > 1e-3:
-        dia = d * rng.uniform(1.0 - w, 1.0, n)
-        dia[0] = d
-        nLevels = 2 if w > 0.15 else 1
-    vol_per = (math.pi / 6) / 0.6 * (1.0 if dia is None else float(np.mean((dia / d) ** 3)))
-    R_d = min(15.0, max(5.0, math.sqrt(n * vol_per / (math.pi * bed_height_d))))
-    H = d * fill * (n * vol_per / (math.pi * R_d * R_d) + 4)
-    inp = make_input(n, d, R_d * d, H, mu, E=E, e=e, rfrc=rfrc, cutGap=0.2, nLevels=nLevels,
-                     forceCap=1, liquid=liquid, tension=tension, theta=math.radians(theta_deg),
-                     rupture_mode=1, mu_wall=mu_w)
-    pos = feed_positions(inp, dia, n, seed + 1)
-    return dict(name="sweep", inp=inp, dia=dia, pos=pos, x=np.asarray(x, float))
+    dm = SWEEP_D
+    R = 0.5 * R_over_r * dm
+    # the bed is bed_d largest diameters tall, so that the analysis's core
+    # region (2 d_max inside the bed at each end) holds 6 d_max of it
+    rel3, d_est = 1.0, dm
+    if sigma > 1e-3:
+        probe = lognormal_diameters(4000, int(rng.integers(1 << 31)), median=dm, sigma=sigma,
+                                    lo=0.5 * dm, hi=2.0 * dm)
+        rel3, d_est = float(np.mean((probe / dm) ** 3)), float(probe.max())
+    h = bed_d * d_est
+    n = int(round(0.6 * 6.0 * (R / dm) ** 2 * (h / dm) / rel3))
+    if sigma > 1e-3:
+        dia = lognormal_diameters(n, int(rng.integers(1 << 31)), median=dm, sigma=sigma,
+                                  lo=0.5 * dm, hi=2.0 * dm)
+        d = float(dia.max())
+        ratio = d / float(dia.min())
+    else:
+        dia, d, ratio = None, dm, 1.0
+    nLevels = 1 if ratio <= 1.5 else (2 if ratio <= 2.5 else 3)
+    H = 1.8 * h + 2.0 * d
+    wet = liquid > 1e-4
+    inp = make_input(n, d, R, H, mu, E=10.0 ** lg_E, e=e, cutGap=0.2, nLevels=nLevels,
+                     forceCap=1 if wet else 0, liquid=liquid if wet else 0.0, tension=tension,
+                     theta=math.radians(theta_deg), rupture_mode=1,
+                     feed=(0.5 * H, H - 0.5 * d))
+    if dia is not None:
+        dmin = float(dia.min())
+        inp.sizeLevelInput = 1
+        for l in range(nLevels):
+            inp.levelDia[l] = d if l == nLevels - 1 else dmin + (d - dmin) / nLevels * (l + 1)
+    inp.cutGap = max(inp.cutGap, min_cut_gap(inp, dia))
+    every = max(1, int(round(feed_frac * steps / batches)))
+    counts = [n * (q + 1) // batches - n * q // batches for q in range(batches)]
+    schedule = [(q * every, c) for q, c in enumerate(counts)]
+    return dict(name="sweep", inp=inp, dia=dia, pos=np.zeros((0, 3)), schedule=schedule,
+                feed_seed=int(rng.integers(1, 1 << 62)), x=np.asarray(x, float), n=n,
+                steps=steps)
+
+
+def feed_due(w, t):
+    """Spheres of sweep instance w due by step t (the batches scheduled at or
+    before t)."""
+    return sum(c for s, c in w["schedule"] if s <= t)
+
+
+def feed_batch(engine, w, want, state):
+    """Feed sweep instance w into engine (a single packing or a batch
+    instance) up to `want` spheres: the current positions are read back,
+    ParticleFeed places new spheres in the feed zone clear of them (as many
+    as fit), and they are appended at rest.  state carries the feed seed
+    from call to call; the placement depends only on the instance's own
+    state, so a batched run and a plain run feed identically.  Returns the
+    number placed."""
+    S, rad, _, _ = derive(w["inp"], w["dia"])
+    count = engine.total
+    if want <= count:
+        return 0
+    pos = np.zeros((want, 3))
+    if count:
+        pos[:count] = engine.particles()[0]
+    seed = state.get("seed", w["feed_seed"])
+    new, seed = particle_feed(S, rad[:want], pos, count, seed)
+    state["seed"] = seed
+    if new > count:
+        engine.feed(pos[count:new])
+    return new - count
 
 
 # --- batched uneven instances (the autotuned multi-instance launch) ----------
