@@ -1521,6 +1521,23 @@ DEV double fcap(const DevInst &P, double S, double V, double effR) {
     return P.twoPI * effR * P.stension * dm_exp(f1 - f2 * dm_exp(f3 * lS + f4 * (lS * lS)));
 }
 
+// The same force in FP32 for mixed precision: a force, stored nowhere and
+// deciding no branch (the bridge's existence and rupture are decided in
+// FP64 by the caller), from the same regression with hardware log and exp
+DEV double fcap_f32(const DevInst &P, double S, double V, double effR) {
+    S = fmax(P.gapmn, S);
+    const float sv = (float)(S / sqrt(V));
+    const float L = __logf((float)V), L2 = L * L, L3 = L2 * L;
+    const double *a = P.wc;
+    const float f1 = (float)a[0] + (float)a[1] * L + (float)a[2] * L2 + (float)a[3] * L3;
+    const float f2 = (float)a[4] + (float)a[5] * L + (float)a[6] * L2;
+    const float f3 = (float)a[7] + (float)a[8] * L + (float)a[9] * L2 + (float)a[10] * L3;
+    const float f4 = (float)a[11] + (float)a[12] * L + (float)a[13] * L2;
+    const float lS = __logf(sv);
+    const float e = __expf(f1 - f2 * __expf(f3 * lS + f4 * (lS * lS)));
+    return (double)((float)(P.twoPI * effR * P.stension) * e);
+}
+
 DEV double rupture(const DevInst &P, double V) {
     return P.ruptureMode == 1 ? (1.0 + 0.5 * P.theta) * dm_pow(V, 1.0 / 3.0)
                               : 0.99 * dm_pow(V, 0.33);
@@ -1581,16 +1598,20 @@ DEV void pp_particle(const DevInst &P, const StepArgs &A, int i, const double4 p
                 contact<true>(P.pp, P.rolling, P.dt, h, -gap, nv, R, ri, cpv, F, M, Mr);
         }
         if (WET && P.forceCap == 1) {
-            int jnl = s.lqd[j];
-            if (jnl < 12) jnl = 12;
-            double dj = rj * 2;
-            double jVol = P.vol * (dj * dj * dj) / jnl;
-            double V = iVol + jVol;
-            double rup = rupture(P, V);
             int fl = loaded ? h.flags : A.h[hi].flags;
             int brg = (fl >> 8) & 1;
+            double V = 0.0;
+            // the liquid volume and rupture distance only where they decide
+            // or act: a bridge that holds unless it ruptures, or a force
+            if (brg || gap < P.layer2 * (ri + rj)) {
+                int jnl = s.lqd[j];
+                if (jnl < 12) jnl = 12;
+                double dj = rj * 2;
+                double jVol = P.vol * (dj * dj * dj) / jnl;
+                V = iVol + jVol;
+            }
             if (gap < P.layer2 * (ri + rj)) brg = 1;
-            else if (gap > rup) brg = 0;
+            else if (brg && gap > rupture(P, V)) brg = 0;
             int nfl = (fl & 0xff) | (brg << 8);
             if (loaded)
                 h.flags = nfl;
@@ -1600,7 +1621,8 @@ DEV void pp_particle(const DevInst &P, const StepArgs &A, int i, const double4 p
             if (brg == 1) {
                 double effR = 2 * ri * rj / (ri + rj);
                 V = V * 8.0;
-                V3 fc = smulv(-1.0 * fcap(P, Sp, V, effR), nv);
+                const double fmag = MIXED ? fcap_f32(P, Sp, V, effR) : fcap(P, Sp, V, effR);
+                V3 fc = smulv(-1.0 * fmag, nv);
                 F = add(F, fc);
                 nbridge = nbridge + 1;
             }
