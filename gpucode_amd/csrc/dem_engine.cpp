@@ -18,6 +18,9 @@
 static const int kChunk = 64;
 static const int kPairMinInterval = 64;
 
+// shared pair records: dry packings whose slots fit the 64-bit slot masks
+static bool pair_fits(const dem_ctx *c) { return c->S.forceCap != 1 && c->K <= 64; }
+
 StepArgs dem_step_args(dem_ctx *c, bool fused) {
     StepArgs A;
     memset(&A, 0, sizeof(A));
@@ -632,7 +635,7 @@ extern "C" int dem_set_pair_records(dem_ctx *c, int on) {
 }
 
 extern "C" int dem_pair_records_used(dem_ctx *c) {
-    return c && c->pair_on && !c->pair_bad && dem_step_kernel(c) == DEM_KERNEL_WAVE &&
+    return c && c->pair_on && !c->pair_bad && dem_step_kernel(c) == DEM_KERNEL_WAVE && pair_fits(c) &&
            (c->rebuild_interval < 0 || c->rebuild_interval >= kPairMinInterval);
 }
 
@@ -748,7 +751,10 @@ int dem_do_rebuild(dem_ctx *c) {
         return rc;
     c->rebuild_pending = false;
     c->n_rebuild++;
-    c->rebuild_interval = c->iter_at_rebuild >= 0 ? c->iter - c->iter_at_rebuild : -1;
+    // two rebuilds at one step (a restart's lists, then its RESTART rebuild;
+    // a feed) say nothing about how far apart motion brings them
+    if (c->iter_at_rebuild >= 0 && c->iter > c->iter_at_rebuild)
+        c->rebuild_interval = c->iter - c->iter_at_rebuild;
     c->iter_at_rebuild = c->iter;
     c->forces_valid = false;
     c->layout_age = 0;
@@ -904,9 +910,15 @@ extern "C" int dem_step_unfused(dem_ctx *c) {
 // the fused path
 // ---------------------------------------------------------------------
 // the step kernel that dem_step launches: DEM_KERNEL_LOOP, _COMPACT or _WAVE
+// (the wave kernel takes any packing: wet ones with the capillary block, a
+// particle with more than 64 slots stepped whole by its owner lane; the
+// block-compaction kernel only dry packings with at most 64 slots)
 int dem_step_kernel(const dem_ctx *c) {
-    if (c->S.forceCap == 1 || c->K > 64) return DEM_KERNEL_LOOP;
-    return c->variant == DEM_KERNEL_AUTO ? DEM_KERNEL_WAVE : c->variant;
+    const bool wet = c->S.forceCap == 1;
+    if (c->variant == DEM_KERNEL_COMPACT)
+        return wet || c->K > 64 ? DEM_KERNEL_LOOP : DEM_KERNEL_COMPACT;
+    if (c->variant == DEM_KERNEL_AUTO) return wet || c->K > 64 ? DEM_KERNEL_LOOP : DEM_KERNEL_WAVE;
+    return c->variant;
 }
 
 extern "C" int dem_set_relayout(dem_ctx *c, int every) {
@@ -921,9 +933,8 @@ extern "C" int dem_set_kernel(dem_ctx *c, int variant) {
     if (!c) return dem_fail(DEM_ERR_ARG, "null context");
     if (variant < DEM_KERNEL_AUTO || variant > DEM_KERNEL_WAVE)
         return dem_fail(DEM_ERR_ARG, "unknown kernel variant %d", variant);
-    if ((variant == DEM_KERNEL_COMPACT || variant == DEM_KERNEL_WAVE) &&
-        (c->S.forceCap == 1 || c->K > 64))
-        return dem_fail(DEM_ERR_ARG, "the compacting kernels need a dry packing with <= 64 slots");
+    if (variant == DEM_KERNEL_COMPACT && (c->S.forceCap == 1 || c->K > 64))
+        return dem_fail(DEM_ERR_ARG, "the block-compaction kernel needs a dry packing with <= 64 slots");
     int rc = dem_pair_unpack(c);
     if (rc) return rc;
     c->variant = variant;
@@ -938,7 +949,7 @@ static int run_chunk(dem_ctx *c, int nsteps, int *executed) {
     // the pair records, so they are made only while rebuilds come at least
     // kPairMinInterval steps apart (or their spacing is not known yet).
     int rc;
-    if (kern == DEM_KERNEL_WAVE && c->pair_on &&
+    if (kern == DEM_KERNEL_WAVE && c->pair_on && pair_fits(c) &&
         (c->rebuild_interval < 0 || c->rebuild_interval >= kPairMinInterval))
         rc = dem_pair_pack(c);
     else
@@ -974,7 +985,7 @@ static int run_chunk(dem_ctx *c, int nsteps, int *executed) {
             A.hc_out = c->hc[(c->hqc + s + 1) & 1];
         }
         if (kern == DEM_KERNEL_WAVE)
-            launch_step_wave(c->P, A, mixed, pair, c->stream);
+            launch_step_wave(c->P, A, mixed, pair, wet, c->stream);
         else if (kern == DEM_KERNEL_COMPACT)
             launch_step_compact(c->P, A, c->K, mixed, c->stream);
         else
@@ -1068,7 +1079,10 @@ extern "C" int dem_get_timing(dem_ctx *c, double *fused_ms, long long *fused_ste
 extern "C" const char *dem_step_kernel_name(dem_ctx *c) {
     if (!c) return "";
     switch (dem_step_kernel(c)) {
-    case DEM_KERNEL_WAVE: return c->precision == DEM_PREC_MIXED ? "k_step_wave_mixed" : "k_step_wave";
+    case DEM_KERNEL_WAVE:
+        if (c->S.forceCap == 1)
+            return c->precision == DEM_PREC_MIXED ? "k_step_wave_wet_mixed" : "k_step_wave_wet";
+        return c->precision == DEM_PREC_MIXED ? "k_step_wave_mixed" : "k_step_wave";
     case DEM_KERNEL_COMPACT: return "k_step_compact";
     default: return "k_step_fused";
     }

@@ -1543,6 +1543,41 @@ DEV double rupture(const DevInst &P, double V) {
                               : 0.99 * dm_pow(V, 0.33);
 }
 
+// liquid volume of a particle of radius r with nl bridges in the last step
+// (kernel.cu:429-434, 511-518: the particle's share, at least 1/12)
+DEV double liquid_volume(const DevInst &P, double r, int nl) {
+    if (nl < 12) nl = 12;
+    const double d = r * 2;
+    return P.vol * (d * d * d) / nl;
+}
+
+// The capillary block of one p-p slot (kernel.cu:509-550): the bridge
+// forms below layer2 (ri + rj), ruptures past the rupture distance of the
+// pair's liquid, and pulls i (only) along -n.  fl is the slot's flags word
+// (load stage in bits 0-7, bridge in bit 8); the new one is returned.  The
+// partner's bridge count (jnl_of) and the liquid volume are evaluated only
+// where they decide or act: a bridge that holds unless it ruptures, or a
+// forming one.
+template <bool MIXED, class JNL>
+DEV int bridge_slot(const DevInst &P, int fl, double gap, V3 nv, double ri, double rj, double iVol,
+                    JNL jnl_of, V3 &F, int &nbridge) {
+    int brg = (fl >> 8) & 1;
+    double V = 0.0;
+    if (brg || gap < P.layer2 * (ri + rj)) V = iVol + liquid_volume(P, rj, jnl_of());
+    if (gap < P.layer2 * (ri + rj)) brg = 1;
+    else if (brg && gap > rupture(P, V)) brg = 0;
+    if (brg == 1) {
+        const double Sp = gap < 0 ? 0.0 : gap;
+        const double effR = 2 * ri * rj / (ri + rj);
+        V = V * 8.0;
+        const double fmag = MIXED ? fcap_f32(P, Sp, V, effR) : fcap(P, Sp, V, effR);
+        const V3 fc = smulv(-1.0 * fmag, nv);
+        F = add(F, fc);
+        nbridge = nbridge + 1;
+    }
+    return (fl & 0xff) | (brg << 8);
+}
+
 DEV V3 vel_of(const double4 *v, int i) {
     const double4 a = v[2 * i];
     return v3(a.x, a.y, a.z);
@@ -1559,13 +1594,7 @@ DEV void pp_particle(const DevInst &P, const StepArgs &A, int i, const double4 p
     const StateBuf &s = A.in;
     const V3 ip = v3(pi.x, pi.y, pi.z);
     const double ri = pi.w;
-    double iVol = 0.0;
-    if (WET) {
-        int inl = s.lqd[i];
-        if (inl < 12) inl = 12;
-        double di = ri * 2;
-        iVol = P.vol * (di * di * di) / inl;
-    }
+    const double iVol = WET ? liquid_volume(P, ri, s.lqd[i]) : 0.0;
     double di = 2.0 * ri;
     F.z = F.z - di * di * di;
     const int n = A.nnb[i];
@@ -1598,34 +1627,13 @@ DEV void pp_particle(const DevInst &P, const StepArgs &A, int i, const double4 p
                 contact<true>(P.pp, P.rolling, P.dt, h, -gap, nv, R, ri, cpv, F, M, Mr);
         }
         if (WET && P.forceCap == 1) {
-            int fl = loaded ? h.flags : A.h[hi].flags;
-            int brg = (fl >> 8) & 1;
-            double V = 0.0;
-            // the liquid volume and rupture distance only where they decide
-            // or act: a bridge that holds unless it ruptures, or a force
-            if (brg || gap < P.layer2 * (ri + rj)) {
-                int jnl = s.lqd[j];
-                if (jnl < 12) jnl = 12;
-                double dj = rj * 2;
-                double jVol = P.vol * (dj * dj * dj) / jnl;
-                V = iVol + jVol;
-            }
-            if (gap < P.layer2 * (ri + rj)) brg = 1;
-            else if (brg && gap > rupture(P, V)) brg = 0;
-            int nfl = (fl & 0xff) | (brg << 8);
+            const int fl = loaded ? h.flags : A.h[hi].flags;
+            const int nfl = bridge_slot<MIXED>(P, fl, gap, nv, ri, rj, iVol,
+                                               [&]() { return s.lqd[j]; }, F, nbridge);
             if (loaded)
                 h.flags = nfl;
             else if (nfl != fl)
                 A.h[hi].flags = nfl;
-            double Sp = gap < 0 ? 0.0 : gap;
-            if (brg == 1) {
-                double effR = 2 * ri * rj / (ri + rj);
-                V = V * 8.0;
-                const double fmag = MIXED ? fcap_f32(P, Sp, V, effR) : fcap(P, Sp, V, effR);
-                V3 fc = smulv(-1.0 * fmag, nv);
-                F = add(F, fc);
-                nbridge = nbridge + 1;
-            }
         }
         if (loaded) A.h[hi] = h;
     }
@@ -1979,6 +1987,13 @@ struct WvStash {
     double4 p[64], va[64], vb[64];
 };
 
+// wet packings: the owners' liquid volumes (read by the contact lanes) and
+// each entry's bridge bit of the current round (summed by its owner)
+struct WvWet {
+    double vol[64];
+    int nb[64];
+};
+
 DEV void wave_sync_lds() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
@@ -2021,6 +2036,36 @@ DEV void pair_compute(const DevInst &P, HistRec &h, const double4 qi, const doub
         contact_mixed<true>(P.pp, P.rolling, P.dt, h, -gap, nv, R, ri, cpv, dF, dM, dMr);
     else
         contact<true>(P.pp, P.rolling, P.dt, h, -gap, nv, R, ri, cpv, dF, dM, dMr);
+}
+
+// one wet particle-particle entry: contact law where the spheres overlap,
+// then the capillary block of the slot, both as pp_particle evaluates them
+// (kernel.cu:458-550).  dF, dM, dMr take the contact's terms, Fc the
+// bridge force; the owner adds them in this order.  Returns 2 when the
+// record must be stored (a contact), 1 when only its flags changed, else 0.
+template <bool MIXED>
+DEV int wet_pair_compute(const DevInst &P, HistRec &h, const double4 qi, const double4 ai,
+                         const double4 bi, const double4 qj, const double4 aj, const double4 bj,
+                         double iVol, int jnl, V3 &dF, V3 &dM, double &dMr, V3 &Fc, int &nb) {
+    const V3 dv = v3(qi.x - qj.x, qi.y - qj.y, qi.z - qj.z);
+    const double l = len(dv);
+    const V3 nv = divv(dv, l);
+    const double ri = qi.w, rj = qj.w;
+    const double gap = l - ri - rj;
+    const bool touch = gap < 0.0;
+    if (touch) {
+        const V3 iv = v3(ai.x, ai.y, ai.z), iw = v3(ai.w, bi.x, bi.y);
+        const V3 jv = v3(aj.x, aj.y, aj.z), jw = v3(aj.w, bj.x, bj.y);
+        const double R = ri * rj / (ri + rj);
+        const V3 cpv = sub(sub(iv, jv), cross(add(mulv(iw, ri), mulv(jw, rj)), nv));
+        if (MIXED)
+            contact_mixed<true>(P.pp, P.rolling, P.dt, h, -gap, nv, R, ri, cpv, dF, dM, dMr);
+        else
+            contact<true>(P.pp, P.rolling, P.dt, h, -gap, nv, R, ri, cpv, dF, dM, dMr);
+    }
+    const int fl = h.flags;
+    h.flags = bridge_slot<MIXED>(P, fl, gap, nv, ri, rj, iVol, [&]() { return jnl; }, Fc, nb);
+    return touch ? 2 : (h.flags != fl ? 1 : 0);
 }
 
 // one wall entry: slot kw of particle ii, wall w.  The wall velocity is the
@@ -2135,17 +2180,29 @@ DEV WpEntry wave_entry_pair(unsigned t, unsigned T, unsigned off, unsigned mlo, 
 
 // block blk of BS particles (one wave per 64; lanes outside [lane_lo,
 // lane_hi) own no particle, for half- and quarter-waves); res is this
-// wave's [7][64] LDS slice.  PAIR: the history comes from the shared pair records (see
-// PAIR_MIRROR) instead of one record per directed slot; the forces are the
-// same bit for bit.
-template <bool MIXED, bool PAIR, int BS = WV_BS>
+// wave's [7][64] LDS slice ([10][64] when WET).  PAIR: the history comes
+// from the shared pair records (see PAIR_MIRROR) instead of one record per
+// directed slot; the forces are the same bit for bit.
+// WET (capillary bridges, directed records only): an entry is a slot in
+// contact, bridged, or close enough for a bridge to form (its flags are
+// read in phase A); it evaluates the contact law where the spheres overlap
+// and then the slot's capillary block, and its owner adds the contact's
+// terms, then the bridge force, in slot order: pp_particle's order.
+// A particle with more than 64 listed slots (polydisperse lists) does not
+// fit the slot masks: its owner steps it alone in phase D with the loop
+// kernel's code, so the kernel serves any list length.
+template <bool MIXED, bool PAIR, int BS = WV_BS, bool WET = false>
 DEV void step_wave_block(const DevInst &P, const StepArgs &A, unsigned blk, double (*res)[64],
-                         WvStash *st, int lane_lo = 0, int lane_hi = 64, int stop = INT_MAX) {
+                         WvStash *st, WvWet *wt = nullptr, int lane_lo = 0, int lane_hi = 64,
+                         int stop = INT_MAX) {
+    static_assert(!(WET && PAIR), "wet packings keep directed records");
     const int tid = threadIdx.x, lane = tid & 63;
     const int i = (int)(blk * BS) + tid;
     const int wbase = i - lane;
     const bool own = i < A.total && lane >= lane_lo && lane < lane_hi;
     const long cap = A.cap;
+    const long Kcap = (long)A.K * cap;
+    bool heavy = false;  // more than 64 slots: stepped whole by its owner in phase D
 
     // phase A: contact masks of the pair slots and the wall slots.  Only
     // the slots of nmask are tested (the others cannot touch before the
@@ -2207,9 +2264,35 @@ DEV void step_wave_block(const DevInst &P, const StepArgs &A, unsigned blk, doub
                     }
                 }
             }
+        } else if constexpr (WET) {
+            // every listed slot: a bridge holds beyond the near gap
+            const int n = A.nnb[i];
+            heavy = n > 64;
+            const int nt = heavy ? 0 : n;
+            for (int r0 = 0; r0 < nt; r0 += WV_NB) {
+                int jj[WV_NB], fl[WV_NB];
+#pragma unroll
+                for (int u = 0; u < WV_NB; ++u) jj[u] = r0 + u < nt ? A.nbr[(long)(r0 + u) * cap + i] : i;
+#pragma unroll
+                for (int u = 0; u < WV_NB; ++u)
+                    fl[u] = r0 + u < nt ? A.h[hist_idx(am_own, aoff_own, r0 + u, i, cap, Kcap)].flags : 0;
+                double4 q[WV_NB];
+#pragma unroll
+                for (int u = 0; u < WV_NB; ++u) q[u] = A.in.p[jj[u]];
+#pragma unroll
+                for (int u = 0; u < WV_NB; ++u) {
+                    const V3 dv = v3(pi.x - q[u].x, pi.y - q[u].y, pi.z - q[u].z);
+                    const double gap = len(dv) - pi.w - q[u].w;
+                    if (r0 + u < nt &&
+                        (gap < 0.0 || ((fl[u] >> 8) & 1) || gap < P.layer2 * (pi.w + q[u].w)))
+                        mask |= 1ull << (r0 + u);
+                }
+            }
         } else {
             unsigned long long nm = A.nmask[i];
             const int n = A.nnb[i];
+            heavy = n > 64;
+            if (heavy) nm = 0ull;
             if (n < 64) nm &= (1ull << n) - 1ull;
             while (nm) {
                 int ks[WV_NB], jj[WV_NB];
@@ -2234,14 +2317,15 @@ DEV void step_wave_block(const DevInst &P, const StepArgs &A, unsigned blk, doub
         __builtin_amdgcn_s_setprio(0);
         if constexpr (PAIR) brk = (unsigned long long)__double_as_longlong(vbi.w) & ~mask;
         const V3 ip = v3(pi.x, pi.y, pi.z);
-        for (int k = 0; k < nw; ++k) {
+        for (int k = 0; k < (heavy ? 0 : nw); ++k) {
             V3 nv;
             double gap;
             wall_geometry(P, A.Tw, ip, pi.w, A.wl[(long)k * cap + i], nv, gap);
             if (gap < 0.0) wmask |= 1u << k;
         }
         const double di = 2.0 * pi.w;
-        grav = di * di * di;
+        grav = heavy ? 0.0 : di * di * di;
+        if constexpr (WET) wt->vol[lane] = liquid_volume(P, pi.w, A.in.lqd[i]);
     }
     wave_sync_lds();  // the stash is read by other lanes in phase C
     // phase B: number the wave's entries, pairs before walls (before
@@ -2257,12 +2341,13 @@ DEV void step_wave_block(const DevInst &P, const StepArgs &A, unsigned blk, doub
     const unsigned mlo = (unsigned)mask, mhi = (unsigned)(mask >> 32);
     const unsigned blo = (unsigned)brk, bhi = (unsigned)(brk >> 32);
 
+    int nbridge = 0;  // WET: bridges of the owner's slots
+
     // phase C, software-pipelined.  Directed records: a round first issues
     // the loads of its neighbours' state, then those of the next round's
     // neighbour indices (or wall ids) and history records, then computes;
     // the history of round r+1 streams in from HBM while round r computes.
     // Pair records: one round deeper (below).
-    const long Kcap = (long)A.K * cap;
     typedef typename std::conditional<PAIR, WpEntry, WvEntry>::type Entry;
     typedef typename std::conditional<PAIR, PairHot, HistRec>::type Rec;
     Entry e;
@@ -2336,6 +2421,8 @@ DEV void step_wave_block(const DevInst &P, const StepArgs &A, unsigned blk, doub
         if constexpr (PAIR) is_pair = ec.kind == 0;
         else is_pair = !ec.wall;
         double4 qj, aj, bj;
+        int jl = 0;
+        (void)jl;
         double2 cold = make_double2(0.0, 0.0);
         if constexpr (PAIR) {
             qj = qn; aj = an; bj = bn;
@@ -2352,6 +2439,7 @@ DEV void step_wave_block(const DevInst &P, const StepArgs &A, unsigned blk, doub
                 qj = A.in.p[jc];
                 aj = A.in.v[2 * jc];
                 bj = A.in.v[2 * jc + 1];
+                if constexpr (WET) jl = A.in.lqd[jc];
             }
             if (r0 + 64 < T) {
                 e = entry_at(r0 + 64 + lane);
@@ -2369,8 +2457,10 @@ DEV void step_wave_block(const DevInst &P, const StepArgs &A, unsigned blk, doub
                 if (hr.flags & PR_COLD) A.hc_out[rc] = A.hc_in[rc];
             }
         } else if (ec.valid) {
-            V3 dF = v3(0.0, 0.0, 0.0), dM = v3(0.0, 0.0, 0.0);
+            V3 dF = v3(0.0, 0.0, 0.0), dM = v3(0.0, 0.0, 0.0), Fc = v3(0.0, 0.0, 0.0);
             double dMr = 0.0;
+            int nb = 0;
+            (void)Fc; (void)nb;
             const int o = ec.ii - wbase;
             const double4 qi = st->p[o], ai = st->va[o], bi = st->vb[o];
             if (is_pair) {
@@ -2387,6 +2477,14 @@ DEV void step_wave_block(const DevInst &P, const StepArgs &A, unsigned blk, doub
                         A.hq_out[rc] = pair_compress(hc, R, cold, need);
                         if (need) A.hc_out[rc] = cold;
                     }
+                } else if constexpr (WET) {
+                    HistRec hc = hr;
+                    const int wr = wet_pair_compute<MIXED>(P, hc, qi, ai, bi, qj, aj, bj, wt->vol[o],
+                                                           jl, dF, dM, dMr, Fc, nb);
+                    if (wr == 2)
+                        A.h[hic] = hc;
+                    else if (wr == 1)
+                        A.h[hic].flags = hc.flags;
                 } else {
                     HistRec hc = hr;
                     pair_compute<MIXED>(P, hc, qi, ai, bi, qj, aj, bj, dF, dM, dMr);
@@ -2398,6 +2496,10 @@ DEV void step_wave_block(const DevInst &P, const StepArgs &A, unsigned blk, doub
             res[0][lane] = dF.x; res[1][lane] = dF.y; res[2][lane] = dF.z;
             res[3][lane] = dM.x; res[4][lane] = dM.y; res[5][lane] = dM.z;
             res[6][lane] = dMr;
+            if constexpr (WET) {
+                res[7][lane] = Fc.x; res[8][lane] = Fc.y; res[9][lane] = Fc.z;
+                wt->nb[lane] = nb;
+            }
         }
         wave_sync_lds();
         if (nforce) {
@@ -2408,6 +2510,10 @@ DEV void step_wave_block(const DevInst &P, const StepArgs &A, unsigned blk, doub
                 Fx = Fx + res[0][s]; Fy = Fy + res[1][s]; Fz = Fz + res[2][s];
                 Mx = Mx + res[3][s]; My = My + res[4][s]; Mz = Mz + res[5][s];
                 Mr = Mr + res[6][s];
+                if constexpr (WET) {  // the bridge force after the slot's contact terms
+                    Fx = Fx + res[7][s]; Fy = Fy + res[8][s]; Fz = Fz + res[9][s];
+                    nbridge = nbridge + wt->nb[s];
+                }
             }
         }
         wave_sync_lds();
@@ -2416,7 +2522,19 @@ DEV void step_wave_block(const DevInst &P, const StepArgs &A, unsigned blk, doub
     bool over = false;
     if (own) {
         const double4 pi = st->p[lane], va = st->va[lane], vb = st->vb[lane];
-        const V3 F = v3(Fx, Fy, Fz), M = v3(Mx, My, Mz);
+        V3 F = v3(Fx, Fy, Fz), M = v3(Mx, My, Mz);
+        if constexpr (!PAIR) {
+            if (heavy) {  // the loop kernel's walk over all slots, then the walls
+                const V3 iv = v3(va.x, va.y, va.z), iw = v3(va.w, vb.x, vb.y);
+                F = v3(0.0, 0.0, 0.0);
+                M = v3(0.0, 0.0, 0.0);
+                Mr = 0.0;
+                nbridge = 0;
+                pp_particle<WET, MIXED>(P, A, i, pi, iv, iw, F, M, Mr, nbridge);
+                pw_particle<MIXED>(P, A, i, pi, iv, iw, F, M, Mr);
+            }
+        }
+        if constexpr (WET) A.out.lqd[i] = nbridge;
         if (A.store_forces) {
             A.Fx[i] = F.x; A.Fy[i] = F.y; A.Fz[i] = F.z;
             A.Mx[i] = M.x; A.My[i] = M.y; A.Mz[i] = M.z;
@@ -2436,7 +2554,7 @@ DEV void step_wave_block(const DevInst &P, const StepArgs &A, unsigned blk, doub
 // The mixed-precision form is held to 3 waves per SIMD (it would take 171
 // registers and fit only 2); the FP64 form fits 3 by itself and is
 // scheduled better without the hint.
-template <bool MIXED, bool PAIR>
+template <bool MIXED, bool PAIR, bool WET = false>
 DEV void step_wave_one(const DevInst &P, const StepArgs &A) {
     // a rebuild is due before this step when stop <= A.step (tested in
     // step_wave_block once the first loads are issued)
@@ -2469,9 +2587,14 @@ DEV void step_wave_one(const DevInst &P, const StepArgs &A) {
         }
     }
     const unsigned c = (A.step & 1) ? A.xb[x + 1] - 1u - pos : A.xb[x] + pos;
-    __shared__ double res[7][64];
+    __shared__ double res[WET ? 10 : 7][64];
     __shared__ WvStash stash;
-    step_wave_block<MIXED, PAIR, 64>(P, A, c, res, &stash, lo, hi, stop);
+    if constexpr (WET) {
+        __shared__ WvWet wet;
+        step_wave_block<MIXED, PAIR, 64, true>(P, A, c, res, &stash, &wet, lo, hi, stop);
+    } else {
+        step_wave_block<MIXED, PAIR, 64>(P, A, c, res, &stash, nullptr, lo, hi, stop);
+    }
 }
 
 template <bool PAIR>
@@ -2485,7 +2608,18 @@ k_step_wave_mixed(DevInst P, StepArgs A) {
     step_wave_one<true, PAIR>(P, A);
 }
 
-void launch_step_wave(const DevInst &P, const StepArgs &A, bool mixed, bool pair, hipStream_t s) {
+// wet packings (capillary bridges; directed records)
+__global__ void __launch_bounds__(64) k_step_wave_wet(DevInst P, StepArgs A) {
+    step_wave_one<false, false, true>(P, A);
+}
+
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3)))
+k_step_wave_wet_mixed(DevInst P, StepArgs A) {
+    step_wave_one<true, false, true>(P, A);
+}
+
+void launch_step_wave(const DevInst &P, const StepArgs &A, bool mixed, bool pair, bool wet,
+                      hipStream_t s) {
     if (A.total <= 0) return;
     unsigned most = 0;
     for (int x = 0; x < 8; ++x) {
@@ -2494,7 +2628,9 @@ void launch_step_wave(const DevInst &P, const StepArgs &A, bool mixed, bool pair
         most = n + L + 2u * Qn > most ? n + L + 2u * Qn : most;
     }
     const dim3 g(8 * most), b(64);
-    if (mixed && pair) hipLaunchKernelGGL((k_step_wave_mixed<true>), g, b, 0, s, P, A);
+    if (wet && mixed) hipLaunchKernelGGL(k_step_wave_wet_mixed, g, b, 0, s, P, A);
+    else if (wet) hipLaunchKernelGGL(k_step_wave_wet, g, b, 0, s, P, A);
+    else if (mixed && pair) hipLaunchKernelGGL((k_step_wave_mixed<true>), g, b, 0, s, P, A);
     else if (mixed) hipLaunchKernelGGL((k_step_wave_mixed<false>), g, b, 0, s, P, A);
     else if (pair) hipLaunchKernelGGL((k_step_wave<true>), g, b, 0, s, P, A);
     else hipLaunchKernelGGL((k_step_wave<false>), g, b, 0, s, P, A);
@@ -2526,6 +2662,20 @@ DEV bool batch_step_args(const BatchInst &B, int step, StepArgs &A, int &stop) {
     return true;
 }
 
+// one block of WV_BS particles of a batched instance, dry or wet
+template <bool MIXED>
+DEV void step_wave_batched(const DevInst &P, const StepArgs &A, unsigned blk, int stop) {
+    __shared__ double res_all[WV_BS / 64][10][64];
+    __shared__ WvStash stash[WV_BS / 64];
+    __shared__ WvWet wet[WV_BS / 64];
+    const int w = threadIdx.x >> 6;
+    if (P.forceCap == 1)
+        step_wave_block<MIXED, false, WV_BS, true>(P, A, blk, res_all[w], &stash[w], &wet[w], 0, 64,
+                                                   stop);
+    else
+        step_wave_block<MIXED, false>(P, A, blk, res_all[w], &stash[w], nullptr, 0, 64, stop);
+}
+
 template <bool MIXED>
 __global__ void __launch_bounds__(WV_BS) k_step_batch_wave(const DevInst *insts, const BatchInst *tab,
                                                            const int2 *tasks, int step) {
@@ -2533,10 +2683,7 @@ __global__ void __launch_bounds__(WV_BS) k_step_batch_wave(const DevInst *insts,
     StepArgs A;
     int stop;
     if (!batch_step_args(tab[tk.x], step, A, stop)) return;
-    __shared__ double res_all[WV_BS / 64][7][64];
-    __shared__ WvStash stash[WV_BS / 64];
-    step_wave_block<MIXED, false>(insts[tk.x], A, (unsigned)tk.y, res_all[threadIdx.x >> 6],
-                                  &stash[threadIdx.x >> 6], 0, 64, stop);
+    step_wave_batched<MIXED>(insts[tk.x], A, (unsigned)tk.y, stop);
 }
 
 template <bool MIXED>
@@ -2614,10 +2761,7 @@ __global__ void __launch_bounds__(WV_BS) k_step_dbatch_wave(const DevInst *insts
     StepArgs A;
     int stop;
     if (!dbatch_step_args(dins[tk.x], ctl[tk.x], g, A, stop)) return;
-    __shared__ double res_all[WV_BS / 64][7][64];
-    __shared__ WvStash stash[WV_BS / 64];
-    step_wave_block<MIXED, false>(insts[tk.x], A, (unsigned)tk.y, res_all[threadIdx.x >> 6],
-                                  &stash[threadIdx.x >> 6], 0, 64, stop);
+    step_wave_batched<MIXED>(insts[tk.x], A, (unsigned)tk.y, stop);
 }
 
 template <bool MIXED>

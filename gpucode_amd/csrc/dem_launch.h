@@ -46,7 +46,8 @@ void launch_step_fused(const DevInst &P, const StepArgs &A, bool wet, bool mixed
 // dry step with contact compaction; needs K <= 64
 void launch_step_compact(const DevInst &P, const StepArgs &A, int K, bool mixed, hipStream_t s);
 // pair: shared pair records (A.noff, A.nrec, A.hq_in, A.hq_out; see PAIR_MIRROR)
-void launch_step_wave(const DevInst &P, const StepArgs &A, bool mixed, bool pair, hipStream_t s);
+void launch_step_wave(const DevInst &P, const StepArgs &A, bool mixed, bool pair, bool wet,
+                      hipStream_t s);
 // shared pair records: counts (c: 4 arrays of cap), fill from the directed
 // records (o: exclusive scans of the counts), write back
 // work of each chunk of 64 particles (contacts: per-particle contact counts)

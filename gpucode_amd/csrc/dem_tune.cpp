@@ -81,7 +81,7 @@ extern "C" int dem_autotune(const dem_input *in, const double *dia, int n, const
         int K = 0;
         for (int l = 0; l < S.nLevels; ++l) K = S.levelNNebors[l] > K ? S.levelNNebors[l] : K;
         for (int kq = 0; kq < 3; ++kq) {
-            if (kernels[kq] != DEM_KERNEL_LOOP && (S.forceCap == 1 || K > 64)) continue;
+            if (kernels[kq] == DEM_KERNEL_COMPACT && (S.forceCap == 1 || K > 64)) continue;
             for (int mo = 0; mo < 2; ++mo) {
                 double ms = 0.0;
                 rc = time_candidate(ci, dia, n, pos, vel, steps, precision, kernels[kq], mo, &ms);

@@ -229,10 +229,13 @@ int dem_step(dem_ctx *ctx, int nsteps);
  * stream.  Raises DEM_ERR_STATE if a rebuild would have been needed. */
 int dem_time_steps(dem_ctx *ctx, int nsteps, double *ms_per_step, int *rebuild_needed);
 int dem_sync(dem_ctx *ctx);
-/* DEM_KERNEL_* selection for dem_step; COMPACT (block-wide contact
- * compaction) and WAVE (per-wave compaction) are refused for wet packings
- * and lists longer than 64 slots.  AUTO takes WAVE where it applies, else
- * LOOP. */
+/* DEM_KERNEL_* selection for dem_step.  LOOP: one thread per particle
+ * walks its slots.  WAVE: per-wave compaction of the slots in contact (wet
+ * packings: in contact, bridged or close enough for a bridge to form), any
+ * list length.  COMPACT (block-wide compaction) is refused for wet
+ * packings and lists longer than 64 slots.  AUTO takes WAVE for dry
+ * packings of at most 64 slots, else LOOP.  Every choice gives the same
+ * results. */
 int dem_set_kernel(dem_ctx *ctx, int variant);
 /* Neighbour search form.  TILED stages the positions of each aligned tile
  * of 4 x 4 x 4 cells and its one-cell rim in LDS and scans them from there

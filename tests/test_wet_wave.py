@@ -1,0 +1,173 @@
+"""The slot-parallel (wave) step kernel on wet packings and on lists longer
+than 64 slots.
+
+Wet packings: an entry of the wave's list is a slot in contact, bridged, or
+close enough for a bridge to form; its lane evaluates the contact law and the
+capillary block (kernel.cu:458-550) and the owner adds the contact's terms,
+then the bridge force, in slot order.  A particle with more than 64 listed
+slots is stepped whole by its owner lane.  Either way the sums run in the
+order of the one-thread loop kernel, so the results must equal it bit for
+bit, and the oracle's within the FP64 parity bars.
+"""
+import numpy as np
+import pytest
+
+import gpucode_amd as G
+from gpucode_amd import workloads as W
+from oracle import pyoracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def bimodal_bed(n_small=3000, n_big=6, R=3.0, wet=False, seed=3, cut=1.0):
+    """Spheres of d = 4 mm (d_max) among a pocket bed of d = 1 mm spheres
+    (reduced diameter 0.25), three size levels.  With cutGap 1.0 the search
+    gap is one small diameter, so each large sphere lists 115-128 small
+    ones: more slots than the wave kernel's 64-bit slot masks hold."""
+    ds = 0.25
+    pos = W.pocket_bed(n_small, R, d=ds, a=0.275, seed=seed)
+    rng = np.random.default_rng(seed)
+    zmax = pos[:, 2].max()
+    big = []
+    tries = 0
+    while len(big) < n_big and tries < 1000:
+        tries += 1
+        c = np.array([rng.uniform(-R + 0.7, R - 0.7), rng.uniform(-R + 0.7, R - 0.7),
+                      rng.uniform(0.7, zmax - 0.7)])
+        if c[0] ** 2 + c[1] ** 2 > (R - 0.7) ** 2 or any(np.linalg.norm(c - b) < 1.6 for b in big):
+            continue
+        big.append(c)
+    big = np.array(big)
+    keep = np.ones(len(pos), bool)
+    for c in big:
+        keep &= np.linalg.norm(pos - c, axis=1) >= 0.5 + ds / 2 + 0.01
+    allp = np.concatenate([big, pos[keep]])
+    n = len(allp)
+    d_max = 4e-3
+    dia = np.full(n, ds * d_max)
+    dia[:len(big)] = d_max
+    inp = W.make_input(n, d_max, R * d_max, d_max * (zmax + 3), 0.4, cutGap=cut, nLevels=3,
+                       forceCap=1 if wet else 0, liquid=0.01 if wet else 0.0,
+                       theta=np.radians(20.0) if wet else 0.0, rupture_mode=1 if wet else 0)
+    return dict(name="bimodal_bed", inp=inp, dia=dia, pos=allp, nbig=len(big))
+
+
+CASES = {
+    # monodisperse touching bed, the reference's capillary law (theta = 0)
+    "wet_mono": lambda: W.wet_bed(n=3000),
+    # three size levels, theta = 20 deg law, 20 % liquid: long bridges
+    "wet_poly": lambda: W.wet_bed(n=3000, poly=True, reference_law=False, theta_deg=20.0,
+                                  liquid=0.2),
+    # lists of more than 64 slots, dry and wet
+    "bimodal_dry": lambda: bimodal_bed(),
+    "bimodal_wet": lambda: bimodal_bed(wet=True),
+}
+
+
+def speed(case):
+    """Velocity scale (reduced units) that brings several rebuilds within
+    300 steps: the long-list beds search 4x wider."""
+    return 300.0 if case.startswith("bimodal") else 30.0
+
+
+def engines(w, kernels, precision, vel):
+    out = []
+    for k in kernels:
+        e = G.Engine(w["inp"], w["dia"], precision)
+        e.set_kernel(k)
+        e.set_particles(w["pos"], vel)
+        e.store_forces(True)
+        out.append(e)
+    return out
+
+
+@pytest.mark.parametrize("precision", [G.PREC_FP64, G.PREC_MIXED])
+@pytest.mark.parametrize("case", list(CASES))
+def test_wave_kernel_equals_loop_kernel(case, precision):
+    w = CASES[case]()
+    n = len(w["pos"])
+    vel = np.random.default_rng(11).normal(size=(n, 3)) * speed(case)
+    print(f"case {case}, precision {precision}, {n} spheres", flush=True)
+    a, b = engines(w, [G.Engine.KERNEL_WAVE, G.Engine.KERNEL_LOOP], precision, vel)
+    assert a.step_kernel_name().startswith("k_step_wave")
+    assert b.step_kernel_name() == "k_step_fused"
+    a.step(300)
+    b.step(300)
+    assert a.n_rebuild == b.n_rebuild and a.n_rebuild > 1
+    for u, v in zip(a.particles(), b.particles()):
+        assert np.array_equal(u, v)
+    for u, v in zip(a.forces(), b.forces()):
+        assert np.array_equal(u, v)
+    ha, hb = a.hist_pp(), b.hist_pp()
+    for f in ha:
+        assert np.array_equal(ha[f], hb[f]), f
+    hwa, hwb = a.hist_pw(), b.hist_pw()
+    for f in hwa:
+        assert np.array_equal(hwa[f], hwb[f]), f
+    assert np.array_equal(a.lqdcnt(), b.lqdcnt())
+    num = a.nebor()[0]
+    if w["inp"].forceCap == 1:
+        assert hb["lqdBrg"].sum() > n // 4  # bridges act
+    if case.startswith("bimodal"):
+        assert num[:w["nbig"]].min() > 64  # the owner-stepped particles
+
+
+@pytest.mark.parametrize("case", ["wet_poly", "bimodal_wet"])
+def test_wet_wave_kernel_matches_oracle(case):
+    w = CASES[case]()
+    print(f"case {case}", flush=True)
+    e = G.Engine(w["inp"], w["dia"])
+    e.set_kernel(G.Engine.KERNEL_WAVE)
+    o = O.Oracle(w["inp"], w["dia"])
+    e.set_particles(w["pos"])
+    o.set_particles(w["pos"])
+    e.step(200)
+    o.step(200)
+    p1, v1, w1 = e.particles()
+    p2, v2, w2 = o.particles()
+    assert np.abs(p1 - p2).max() <= 1e-9
+    assert np.abs(v1 - v2).max() <= 1e-7 * max(np.abs(v2).max(), 1e-12)
+    assert np.abs(w1 - w2).max() <= 1e-6 * max(np.abs(w2).max(), 1e-12)
+    n1, l1 = e.nebor(o.K)
+    n2, l2 = o.nebor()
+    assert np.array_equal(n1, n2) and np.array_equal(l1, l2)
+    assert np.array_equal(e.lqdcnt(), o.lqdcnt())
+    h1, h2 = e.hist_pp(o.K), o.hist_pp()
+    assert np.array_equal(h1["lqdBrg"], h2["lqdBrg"])
+    assert h2["lqdBrg"].sum() > 0
+
+
+def test_batched_wet_wave_instances_equal_single_runs():
+    # wet and long-list instances on the batched wave kernel (device-driven
+    # schedule, rebuild slots every 2 steps) follow their single runs
+    rng = np.random.default_rng(12)
+    cases = []
+    for name in ("wet_mono", "wet_poly", "bimodal_wet", "bimodal_dry"):
+        w = CASES[name]()
+        cases.append((w, rng.normal(size=(len(w["pos"]), 3)) * speed(name)))
+    w = W.small_bed(n=1500, seed=4)
+    cases.append((w, rng.normal(size=(1500, 3)) * 30.0))
+    b = G.Batch([w["inp"] for w, _ in cases], [w["dia"] for w, _ in cases], G.PREC_MIXED)
+    b.set_schedule(device=True, slot_every=2)
+    singles = []
+    for k, (w, vel) in enumerate(cases):
+        b[k].set_kernel(G.Engine.KERNEL_WAVE)
+        b[k].set_particles(w["pos"], vel)
+        e = G.Engine(w["inp"], w["dia"], G.PREC_MIXED)
+        e.set_kernel(G.Engine.KERNEL_WAVE)
+        e.set_particles(w["pos"], vel)
+        singles.append(e)
+    steps = 200
+    b.step(steps)
+    for e in singles:
+        e.step(steps)
+    for k, e in enumerate(singles):
+        print(f"instance {k}: {e.step_kernel_name()}, rebuilds {e.n_rebuild}", flush=True)
+        assert b[k].n_rebuild == e.n_rebuild
+        for u, v in zip(b[k].particles(), e.particles()):
+            assert np.array_equal(u, v), k
+        hb, he = b[k].hist_pp(), e.hist_pp()
+        for f in hb:
+            assert np.array_equal(hb[f], he[f]), (k, f)
+        assert np.array_equal(b[k].lqdcnt(), e.lqdcnt())
+    b.close()
