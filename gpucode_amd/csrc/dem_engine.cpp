@@ -914,11 +914,9 @@ extern "C" int dem_step_unfused(dem_ctx *c) {
 // particle with more than 64 slots stepped whole by its owner lane; the
 // block-compaction kernel only dry packings with at most 64 slots)
 int dem_step_kernel(const dem_ctx *c) {
-    const bool wet = c->S.forceCap == 1;
     if (c->variant == DEM_KERNEL_COMPACT)
-        return wet || c->K > 64 ? DEM_KERNEL_LOOP : DEM_KERNEL_COMPACT;
-    if (c->variant == DEM_KERNEL_AUTO) return wet || c->K > 64 ? DEM_KERNEL_LOOP : DEM_KERNEL_WAVE;
-    return c->variant;
+        return c->S.forceCap == 1 || c->K > 64 ? DEM_KERNEL_LOOP : DEM_KERNEL_COMPACT;
+    return c->variant == DEM_KERNEL_AUTO ? DEM_KERNEL_WAVE : c->variant;
 }
 
 extern "C" int dem_set_relayout(dem_ctx *c, int every) {
@@ -1144,10 +1142,16 @@ extern "C" int dem_contact_stats(dem_ctx *c, dem_stats *out) {
     if (!c || !out) return dem_fail(DEM_ERR_ARG, "bad argument");
     int rc;
     if (c->rebuild_pending && (rc = dem_rebuild(c))) return rc;
-    if ((rc = dem_pair_unpack(c))) return rc;
+    // the counts come from positions and lists; only the bridge count reads
+    // the directed records, and pair records are kept for dry packings
+    // only, which have no bridges: they stay packed (a write-back here
+    // would cost the next step a new pack)
+    const bool packed_dry = c->pair_packed && c->S.forceCap != 1;
+    if (!packed_dry && (rc = dem_pair_unpack(c))) return rc;
     unsigned long long h[8] = {0};
     HIPCHK(hipMemcpyAsync(c->stats, h, sizeof h, hipMemcpyHostToDevice, c->stream));
     StepArgs A = dem_step_args(c, true);
+    if (packed_dry) A.h = nullptr;  // no bridge flags to read
     launch_stats(c->P, A, c->stats, c->stream);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(h, c->stats, sizeof h, hipMemcpyDeviceToHost, c->stream));

@@ -3446,7 +3446,8 @@ __global__ void __launch_bounds__(256) k_stats(DevInst P, StepArgs A, unsigned l
             double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
             double gap = sqrt(dx * dx + dy * dy + dz * dz) - pi.w - pj.w;
             if (gap < 0.0) nc++;
-            if ((A.h[hist_idx(A.amask[i], A.actoff[i], k, i, A.cap, (long)A.K * A.cap)].flags >> 8) & 1)
+            if (A.h &&
+                (A.h[hist_idx(A.amask[i], A.actoff[i], k, i, A.cap, (long)A.K * A.cap)].flags >> 8) & 1)
                 nbr++;
         }
         nw = A.nwl[i];

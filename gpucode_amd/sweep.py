@@ -103,20 +103,6 @@ def _class_of(w):
     return (int(w["inp"].forceCap), int(w["inp"].nLevels), 0 if n < 2500 else (1 if n < 5000 else 2))
 
 
-def _wave_fits(w, cut_gap):
-    """Whether the wave kernel can step packing w at this skin: dry, at most
-    64 list slots."""
-    from ._capi import DemInput
-    from .engine import derive
-
-    if w["inp"].forceCap == 1:
-        return False
-    inp = DemInput.from_buffer_copy(w["inp"])
-    inp.cutGap = cut_gap
-    S = derive(inp, w["dia"])[0]
-    return max(S.levelNNebors[l] for l in range(S.nLevels)) <= 64
-
-
 def tune_classes(cfg, ws, log=None):
     """The autotuner on one packing per class (wet or dry, size levels,
     size), started from its first feed batch at rest; every packing of the
@@ -140,8 +126,6 @@ def tune_classes(cfg, ws, log=None):
     for w in ws:
         ch = dict(choice[_class_of(w)])
         ch["cutGap"] = max(ch["cutGap"], W.min_cut_gap(w["inp"], w["dia"]))
-        if ch["kernel"] == Engine.KERNEL_WAVE and not _wave_fits(w, ch["cutGap"]):
-            ch["kernel"] = Engine.KERNEL_LOOP  # this packing's lists are longer than 64
         out.append(ch)
     if log:
         log(f"tuned {len(choice)} classes")

@@ -233,9 +233,8 @@ int dem_sync(dem_ctx *ctx);
  * walks its slots.  WAVE: per-wave compaction of the slots in contact (wet
  * packings: in contact, bridged or close enough for a bridge to form), any
  * list length.  COMPACT (block-wide compaction) is refused for wet
- * packings and lists longer than 64 slots.  AUTO takes WAVE for dry
- * packings of at most 64 slots, else LOOP.  Every choice gives the same
- * results. */
+ * packings and lists longer than 64 slots.  AUTO takes WAVE.  Every
+ * choice gives the same results. */
 int dem_set_kernel(dem_ctx *ctx, int variant);
 /* Neighbour search form.  TILED stages the positions of each aligned tile
  * of 4 x 4 x 4 cells and its one-cell rim in LDS and scans them from there

@@ -284,7 +284,7 @@ def test_mixed_wet_lognormal_one_step_forces(settled_wet_bed):
     e.store_forces(True)
     e.step(1)
     o.step(1)
-    assert e.step_kernel_name() == "k_step_fused"
+    assert e.step_kernel_name() == "k_step_wave_wet_mixed"
     r = one_step_errors(e, o, b["rad"])
     h = o.hist_pp()
     r["bridged_slots"] = int((h["lqdBrg"] == 1).sum())
