@@ -131,6 +131,7 @@ _SIGS = {
     "dem_set_order": (C.c_int, [P, C.c_int]),
     "dem_set_relayout": (C.c_int, [P, C.c_int]),
     "dem_n_relayout": (C.c_int, [P]),
+    "dem_set_wave_split": (C.c_int, [P, C.c_int]),
     "dem_set_pair_records": (C.c_int, [P, C.c_int]),
     "dem_pair_records_used": (C.c_int, [P]),
     "dem_iter": (C.c_int, [P]),

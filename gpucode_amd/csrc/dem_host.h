@@ -82,6 +82,7 @@ struct dem_ctx {
     double *F[7] = {};           // Fx Fy Fz Mx My Mz Mr
     int store_forces = 0;
     int variant = DEM_KERNEL_AUTO;  // fused step kernel choice
+    int wave_split = 0;             // wave kernel sub-waves per chunk (0: one, or 1, 2, 4)
     bool want_morton = true;        // Morton storage order when the grid allows
 
     // device flags and counters
@@ -144,6 +145,7 @@ void dem_pair_free(dem_ctx *c);
 StepArgs dem_step_args(dem_ctx *c, bool fused);
 // DEM_KERNEL_LOOP, _COMPACT or _WAVE: the kernel dem_step launches
 int dem_step_kernel(const dem_ctx *c);
+int dem_wave_split(const dem_ctx *c);
 // host bookkeeping after a chunk of nsteps launches whose device stop
 // index read back as stop; returns the steps executed
 int dem_finish_chunk(dem_ctx *c, int nsteps, int stop);

@@ -167,6 +167,7 @@ struct StepArgs {
     int cap, total, step, store_forces;
     int K;                     // neighbour slots per particle (list stride)
     unsigned xb[9];            // wave kernel: chunks [xb[x], xb[x+1]) go to XCD x
+    int wsplit;                // wave kernel: sub-waves per chunk of 64 (1, 2 or 4)
     double Tw;                 // lid height for this step
 };
 

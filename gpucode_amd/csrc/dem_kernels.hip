@@ -2566,13 +2566,21 @@ DEV void step_wave_one(const DevInst &P, const StepArgs &A) {
     // nothing else: each particle's step reads only the previous step).
     // The last L chunks an XCD takes are split into two half-waves of 32
     // particles each, and the last Qn of them into four quarter-waves of
-    // 16, so the launch ends on short waves.
+    // 16, so the launch ends on short waves.  With A.wsplit > 1 (a packing
+    // too small to give every wave slot of the chip several chunks) every
+    // chunk is split into wsplit sub-waves of 64 / wsplit particles instead.
     const unsigned x = blockIdx.x % 8u, k = blockIdx.x / 8u;
     const unsigned n = A.xb[x + 1] - A.xb[x], L = n / 4u < WV_TAIL ? n / 4u : WV_TAIL;
     const unsigned Qn = L / 6u < WV_QTAIL ? L / 6u : WV_QTAIL;
     unsigned pos = k;
     int lo = 0, hi = 64;
-    if (k >= n - L) {
+    if (A.wsplit > 1) {
+        const unsigned parts = (unsigned)A.wsplit, q = k % parts;
+        pos = k / parts;
+        if (pos >= n) return;
+        lo = (int)(q * (64u / parts));
+        hi = lo + (int)(64u / parts);
+    } else if (k >= n - L) {
         const unsigned j = k - (n - L);
         if (j < 2u * (L - Qn)) {
             pos = n - L + j / 2u;
@@ -2625,7 +2633,8 @@ void launch_step_wave(const DevInst &P, const StepArgs &A, bool mixed, bool pair
     for (int x = 0; x < 8; ++x) {
         const unsigned n = A.xb[x + 1] - A.xb[x], L = n / 4u < WV_TAIL ? n / 4u : WV_TAIL;
         const unsigned Qn = L / 6u < WV_QTAIL ? L / 6u : WV_QTAIL;
-        most = n + L + 2u * Qn > most ? n + L + 2u * Qn : most;
+        const unsigned want = A.wsplit > 1 ? n * (unsigned)A.wsplit : n + L + 2u * Qn;
+        most = want > most ? want : most;
     }
     const dim3 g(8 * most), b(64);
     if (wet && mixed) hipLaunchKernelGGL(k_step_wave_wet_mixed, g, b, 0, s, P, A);

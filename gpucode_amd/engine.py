@@ -300,6 +300,11 @@ class Engine:
     def n_relayout(self):
         return A.lib().dem_n_relayout(self._ctx)
 
+    def set_wave_split(self, parts):
+        """Wave kernel block shape: waves per chunk of 64 particles (1, 2, 4;
+        0: one)."""
+        check(A.lib().dem_set_wave_split(self._ctx, int(parts)))
+
     def set_pair_records(self, on):
         """Shared pair records for the wave kernel (default on): one history
         record per pair of equal radii instead of two mirrored ones."""

@@ -256,6 +256,12 @@ int dem_set_order(dem_ctx *ctx, int morton);
  * Records only move, so results are identical with any setting.  No
  * reference counterpart (the reference keeps one slot-major layout). */
 int dem_set_relayout(dem_ctx *ctx, int every);
+/* Block shape of the wave kernel: every chunk of 64 particles is stepped by
+ * `parts` waves (1, 2 or 4) of 64 / parts owners each; 0 (default) is one
+ * wave per chunk.  Results are the same for every choice.  No reference
+ * counterpart (the reference's launch shape is fixed,
+ * ParticleSystem.cu:231-250). */
+int dem_set_wave_split(dem_ctx *ctx, int parts);
 /* Layout refreshes done so far. */
 int dem_n_relayout(dem_ctx *ctx);
 /* Shared pair records for the WAVE kernel (on by default): a pair of

@@ -171,3 +171,34 @@ def test_batched_wet_wave_instances_equal_single_runs():
             assert np.array_equal(hb[f], he[f]), (k, f)
         assert np.array_equal(b[k].lqdcnt(), e.lqdcnt())
     b.close()
+
+
+@pytest.mark.parametrize("case", ["wet_poly", "dry_pairs"])
+def test_wave_split_changes_nothing(case):
+    # one, two or four waves per chunk of 64 particles: each wave compacts
+    # the contacts of its own owners, and every owner still sums its entries
+    # in slot order, so the runs are identical
+    if case == "dry_pairs":
+        w = W.small_bed(n=3000)
+    else:
+        w = CASES[case]()
+    n = len(w["pos"])
+    vel = np.random.default_rng(13).normal(size=(n, 3)) * 30.0
+    runs = []
+    for parts in (1, 2, 4):
+        e = G.Engine(w["inp"], w["dia"], G.PREC_MIXED)
+        e.set_wave_split(parts)
+        e.set_particles(w["pos"], vel)
+        e.store_forces(True)
+        e.step(300)
+        runs.append((e.particles(), e.forces(), e.hist_pp(), e.lqdcnt(), e.n_rebuild,
+                     e.pair_records_used))
+        e.close()
+    for r in runs[1:]:
+        for u, v in zip(r[0] + r[1], runs[0][0] + runs[0][1]):
+            assert np.array_equal(u, v)
+        for f in r[2]:
+            assert np.array_equal(r[2][f], runs[0][2][f]), f
+        assert np.array_equal(r[3], runs[0][3]) and r[4] == runs[0][4]
+    if case == "dry_pairs":
+        assert runs[0][5]  # the shared pair records were in use
