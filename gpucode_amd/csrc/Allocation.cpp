@@ -107,6 +107,28 @@ void dem_fill_inst(const dem_setup &S, DevInst &P) {
     P.ncellTab = (int)tab;
 }
 
+// Interleaved-level Morton keys (DevInst::morton = 2): the level-0 cell
+// holding the centre of each cell of every level, by the device's own
+// expression, must fit 10 bits per axis, and with the level bits the key 31.
+static bool interleaved_layout(const dem_setup &S, int *level_bits, int *key_bits) {
+    const double c0 = S.cellSize[0];
+    int m = 1;
+    for (int l = 0; l < S.nLevels; ++l)
+        for (int a = 0; a < 3; ++a) {
+            const double top = floor(((double)(S.gridSize[l][a] - 1) + 0.5) * S.cellSize[l] / c0);
+            if (!(top < 1023.0)) return false;
+            m = (int)top + 1 > m ? (int)top + 1 : m;
+        }
+    int b = 0;
+    while ((1 << b) < m) ++b;
+    int lb = 0;
+    while ((1 << lb) < S.nLevels) ++lb;
+    if (3 * b + lb > 31) return false;
+    *level_bits = lb;
+    *key_bits = 3 * b + lb < 1 ? 1 : 3 * b + lb;
+    return true;
+}
+
 // Sort-key layout: Morton codes of the cells when every level grid fits 10
 // bits per axis and the key space 31 bits, else the row-major hash.
 static bool morton_layout(const dem_setup &S, unsigned off[DEM_MAXL], int *key_bits) {
@@ -134,7 +156,13 @@ void dem_refresh_inst(dem_ctx *c) {
     while (bits < 31 && ((long long)1 << bits) <= (long long)c->S.sysNCells) ++bits;
     c->key_bits = bits;
     c->P.morton = 0;
-    if (c->want_morton) {
+    c->P.levelBits = 0;
+    int lb = 0, ib = 0;
+    if (c->want_morton && c->S.nLevels > 1 && interleaved_layout(c->S, &lb, &ib)) {
+        c->P.morton = 2;
+        c->P.levelBits = lb;
+        c->key_bits = ib;
+    } else if (c->want_morton) {
         unsigned off[DEM_MAXL] = {0};
         int mb = 0;
         if (morton_layout(c->S, off, &mb)) {

@@ -61,10 +61,17 @@ struct DevInst {
     int nLevels;
     // storage order of the particles: 0 = row-major cell hash (the
     // reference's sort key), 1 = Morton (Z-order) code of the cell, which
-    // keeps 3-D neighbours close in memory.  Either way particles of one
-    // cell are stored by original index.
+    // keeps 3-D neighbours close in memory, each size level in a range of
+    // its own; 2 (several levels) = the levels interleaved in one Morton
+    // order: the key of a cell of level L is the Morton code of the
+    // level-0 cell holding its centre, then L, so the particles of every
+    // level at one place are stored together (a sphere's neighbours of
+    // other levels then sit next to it, not in another level's range).
+    // Either way the particles of one cell are contiguous and stored by
+    // original index.
     int morton;
-    unsigned mortonOff[DEM_MAXL];  // key offset of each level's Morton range
+    unsigned mortonOff[DEM_MAXL];  // key offset of each level's Morton range (morton 1)
+    int levelBits;             // bits of the level in the key (morton 2)
     // The CellStart/CellEnd tables are stored in blocks of 4 x 4 x 4 cells:
     // block (x/4, y/4, z/4) of level L in row-major block order from
     // blkOff[L], the 64 cells of a block row-major inside it.  The 27 cells

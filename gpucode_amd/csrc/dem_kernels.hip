@@ -283,8 +283,20 @@ DEV void cell_of(const DevInst &P, const double4 q, int L, int &hash, unsigned &
     const int cy = cell_coord(q.y, P.origin[1], cs, P.gy[L]);
     const int cz = cell_coord(q.z, P.origin[2], cs, P.gz[L]);
     hash = (cz * P.gy[L] * P.gx[L]) + (cy * P.gx[L]) + cx + P.cellOff[L];
-    key = P.morton ? P.mortonOff[L] + (spread3(cx) | (spread3(cy) << 1) | (spread3(cz) << 2))
-                   : (unsigned)hash;
+    if (P.morton == 2) {
+        // the level-0 cell holding the centre of this cell: every level's
+        // cells are at least as large as level 0's, so two cells of one
+        // level never share it
+        const double c0 = P.cellSize[0];
+        const unsigned xc = (unsigned)floor(((double)cx + 0.5) * cs / c0);
+        const unsigned yc = (unsigned)floor(((double)cy + 0.5) * cs / c0);
+        const unsigned zc = (unsigned)floor(((double)cz + 0.5) * cs / c0);
+        key = ((spread3(xc) | (spread3(yc) << 1) | (spread3(zc) << 2)) << P.levelBits) | (unsigned)L;
+    } else if (P.morton == 1) {
+        key = P.mortonOff[L] + (spread3(cx) | (spread3(cy) << 1) | (spread3(cz) << 2));
+    } else {
+        key = (unsigned)hash;
+    }
 }
 
 DEV void calc_hash_one(const DevInst &P, const double4 *p, const int *level, const int *id,
@@ -759,7 +771,7 @@ __global__ void __launch_bounds__(NF_BS) k_neighbor_tiles(const double4 *p, cons
 }
 
 bool neighbor_tiles_apply(const DevInst &P) {
-    return P.morton && P.nLevels == 1 && (P.mortonOff[0] & 63u) == 0 && P.gx[0] <= 1024 &&
+    return P.morton == 1 && P.nLevels == 1 && (P.mortonOff[0] & 63u) == 0 && P.gx[0] <= 1024 &&
            P.gy[0] <= 1024 && P.gz[0] <= 1024;
 }
 
