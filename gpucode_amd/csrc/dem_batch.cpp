@@ -417,11 +417,13 @@ static int batch_step_device(dem_batch *b, int nsteps) {
     // hands the rebuilt instances their new control records, and they
     // resume with block j + 2.  Nothing returns to the host in between; it
     // looks at the control records only to see whether every instance is
-    // through (after each block once nsteps global steps have run).
+    // through: first once nsteps global steps have run, then no earlier
+    // than the instance furthest from its end could finish (it takes at
+    // most one step per global step), and every 256 blocks.
     const int R = b->slot_every < 1 ? 1 : b->slot_every;
     // an instance that rebuilds after every step takes 2R global steps per step
     const long long gmax = ((long long)nsteps + 1) * 2 * R;
-    long long G = 0, slots = 0, blocks = 0, last_check = 0;
+    long long G = 0, slots = 0, blocks = 0, last_check = 0, next_check = nsteps;
     int pend = -1;
     bool done = nsteps == 0;
     while (!done) {
@@ -438,9 +440,7 @@ static int batch_step_device(dem_batch *b, int nsteps) {
         }
         HIPCHK(hipGetLastError());
         ++blocks;
-        // from nsteps global steps on, an instance is at most two blocks
-        // from its end (a rebuild slot and the block beside it)
-        if (G >= nsteps || blocks - last_check >= 256 || G >= gmax) {
+        if (G >= next_check || blocks - last_check >= 256 || G >= gmax) {
             last_check = blocks;
             HIPCHK(hipMemcpyAsync(b->h_ctl, b->d_ctl, (size_t)n * sizeof(DBatchCtl),
                                   hipMemcpyDeviceToHost, b->stream));
@@ -449,15 +449,20 @@ static int batch_step_device(dem_batch *b, int nsteps) {
             HIPCHK(hipStreamSynchronize(b->stream));
             ++b->host_syncs;
             done = true;
-            for (int k = 0; k < n && done; ++k) {
+            int most = 0;  // steps left of the instance furthest from its end
+            for (int k = 0; k < n; ++k) {
                 const DBatchCtl &C = b->h_ctl[k];
                 const int seen = (int)G - C.base, st = b->h_stop[k];
                 int seg = st < seen ? st : seen;
                 seg = seg < C.left ? seg : C.left;
                 // still running, or rebuilding at the next slot
-                if (!C.failed && seg < C.left) done = false;
+                if (!C.failed && seg < C.left) {
+                    done = false;
+                    most = C.left - seg > most ? C.left - seg : most;
+                }
             }
             if (done) break;
+            next_check = G + (most > R ? most : R);
             if (G >= gmax) return dem_fail(DEM_ERR_STATE, "batched run made no progress");
         }
         const int q = (int)(slots & 3);
