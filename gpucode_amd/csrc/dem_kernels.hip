@@ -396,68 +396,6 @@ void launch_cell_range(const unsigned *keys, const StateBuf &s, const int *level
 // The file is compiled with -ffp-contract=off, so the distance test rounds
 // like the host restatement.
 // ---------------------------------------------------------------------
-DEV void neighbor_find_one(const double4 *p, const int *level, const DevInst &P, const int *cstart,
-                           const int *cend, int *nbr, int *nnb, int *overflow, int cap, int i) {
-    const double4 o = p[i];
-    const double ri = o.w;
-    const int iL = level[i];
-    const int K = P.levelK[iL];
-    int num = 0;
-    for (int L = 0; L < P.nLevels; ++L) {
-        const double cs = P.cellSize[L];
-        const int gx = P.gx[L], gy = P.gy[L], gz = P.gz[L];
-        const int cx = cell_coord(o.x, P.origin[0], cs, gx);
-        const int cy = cell_coord(o.y, P.origin[1], cs, gy);
-        const int cz = cell_coord(o.z, P.origin[2], cs, gz);
-        const int off = (int)ceil(P.levelSize[iL] / P.levelSize[L]);
-        for (int dz = -off; dz <= off; ++dz) {
-            const int zz = cz + dz;
-            if (zz < 0 || zz > gz - 1) continue;
-            for (int dy = -off; dy <= off; ++dy) {
-                const int yy = cy + dy;
-                if (yy < 0 || yy > gy - 1) continue;
-                int sx = cx - off, ex = cx + off;
-                if (sx < 0) sx = 0;
-                if (ex > gx - 1) ex = gx - 1;
-                const int rowb = row_slot(P, L, yy, zz);
-                // the reference scans the sorted range from the first to the
-                // last non-empty cell of the row; cell by cell in x order is
-                // the same sequence, and works for either storage order
-                for (int x = sx; x <= ex; ++x) {
-                    const int h = rowb + ((x >> 2) << 6) + (x & 3);
-                    const int s0 = cstart[h];
-                    if (s0 == DEM_EMPTY) continue;
-                    const int e0 = cend[h];
-                    for (int j = s0; j < e0; ++j) {
-                        if (j == i) continue;
-                        const double4 q = p[j];
-                        double dx = o.x - q.x, dy2 = o.y - q.y, dz2 = o.z - q.z;
-                        double len = sqrt(dx * dx + dy2 * dy2 + dz2 * dz2);
-                        if (len < q.w + ri + P.searchGap) {
-                            if (num < K) nbr[(long)num * cap + i] = j;
-                            ++num;
-                        }
-                    }
-                }
-            }
-        }
-    }
-    if (num > K) {
-        atomicMax(overflow, num);
-        num = K;
-    }
-    nnb[i] = num;
-}
-
-// The same search by a group of 16 lanes per particle, for the batched
-// rebuild of small instances, where one thread per particle leaves the chip
-// nearly empty and the search time is the latency of one particle's walk
-// over its rows.  The rows of the scan (level, z row, y row in the order
-// above) are dealt out 16 at a time, one per lane; a lane walks its row's
-// cells in x order and candidates in storage order, so the list is the
-// concatenation of the rows' hits in row order: a 16-lane prefix sum of the
-// hit counts places each lane's hits.  Up to NFG_KEEP hits of a row stay
-// in registers from the counting pass; a row with more is walked again.
 #define NFG_KEEP 8
 
 DEV void nfg_hit(int j, int i, const double4 q, const double4 o, double ri, double gap, int *keep,
@@ -509,6 +447,56 @@ DEV int nfg_row(const double4 *p, const DevInst &P, const int *cstart, const int
     return c;
 }
 
+DEV void neighbor_find_one(const double4 *p, const int *level, const DevInst &P, const int *cstart,
+                           const int *cend, int *nbr, int *nnb, int *overflow, int cap, int i) {
+    const double4 o = p[i];
+    const double ri = o.w;
+    const int iL = level[i];
+    const int K = P.levelK[iL];
+    int num = 0;
+    int keep[NFG_KEEP];  // unused: the rows write their hits straight into the list
+    for (int L = 0; L < P.nLevels; ++L) {
+        const double cs = P.cellSize[L];
+        const int gx = P.gx[L], gy = P.gy[L], gz = P.gz[L];
+        const int cx = cell_coord(o.x, P.origin[0], cs, gx);
+        const int cy = cell_coord(o.y, P.origin[1], cs, gy);
+        const int cz = cell_coord(o.z, P.origin[2], cs, gz);
+        const int off = (int)ceil(P.levelSize[iL] / P.levelSize[L]);
+        for (int dz = -off; dz <= off; ++dz) {
+            const int zz = cz + dz;
+            if (zz < 0 || zz > gz - 1) continue;
+            for (int dy = -off; dy <= off; ++dy) {
+                const int yy = cy + dy;
+                if (yy < 0 || yy > gy - 1) continue;
+                int sx = cx - off, ex = cx + off;
+                if (sx < 0) sx = 0;
+                if (ex > gx - 1) ex = gx - 1;
+                // the reference scans the sorted range from the first to the
+                // last non-empty cell of the row; cell by cell in x order is
+                // the same sequence, and works for either storage order.  The
+                // cell ranges of four cells load together, candidates two at
+                // a time, so a thread keeps several loads in flight.
+                num += nfg_row(p, P, cstart, cend, i, o, ri, row_slot(P, L, yy, zz), sx, ex, keep,
+                               true, nbr, cap, num, K);
+            }
+        }
+    }
+    if (num > K) {
+        atomicMax(overflow, num);
+        num = K;
+    }
+    nnb[i] = num;
+}
+
+// The same search by a group of 16 lanes per particle, for the batched
+// rebuild of small instances, where one thread per particle leaves the chip
+// nearly empty and the search time is the latency of one particle's walk
+// over its rows.  The rows of the scan (level, z row, y row in the order
+// above) are dealt out 16 at a time, one per lane; a lane walks its row's
+// cells in x order and candidates in storage order, so the list is the
+// concatenation of the rows' hits in row order: a 16-lane prefix sum of the
+// hit counts places each lane's hits.  Up to NFG_KEEP hits of a row stay
+// in registers from the counting pass; a row with more is walked again.
 // g = lane within the particle's group of 16; every lane of the group calls
 // this with the same i (valid = false for a group past the end)
 DEV void neighbor_find_group16(const double4 *p, const int *level, const DevInst &P,
