@@ -77,6 +77,7 @@ void dem_fill_inst(const dem_setup &S, DevInst &P) {
     P.twoPI = 2 * 3.1415926;
     P.stension = S.stension;
     willett_coeffs(S.theta, P.wc);
+    for (int k = 0; k < 14; ++k) P.wcf[k] = (float)P.wc[k];
     P.theta = S.theta;
     P.forceCap = S.forceCap;
     P.ruptureMode = S.ruptureMode;

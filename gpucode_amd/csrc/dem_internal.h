@@ -47,6 +47,7 @@ struct DevInst {
     // liquid (kernel.cu:65-81, 429-550)
     double vol, gapmn, layer2, twoPI, stension;
     double wc[14];             // Willett coefficients at theta
+    float wcf[14];             // the same rounded to FP32 (the mixed-precision bridge force)
     double theta;
     int forceCap, ruptureMode;
     // cylinder (heydem.h:39-46)

@@ -1388,10 +1388,12 @@ DEV double normal_law(const DevLaw &c, HistRec &h, double d, double R) {
 // keeping its length, add v_t dt, cap it at xi_max.  Returns xi_max; t is
 // the new spring (also stored in h) and u = 1 - |t|/xi_max where the spring
 // sticks below the cap with xi_max above the tolerance (the Mindlin ratio is
-// then 1 - u^1.5), else -1 (ratio 1).
+// then 1 - u^1.5), else -1 (ratio 1).  tl: |t| where the spring stays
+// below the cap (t is then the vector whose length was just taken, so the
+// value is the one len(t) would give again), else -1.
 template <bool PP>
 DEV double spring_update(const DevLaw &c, HistRec &h, double fN, double d, double R, V3 n, V3 vt,
-                         double dt, V3 &t, double &u) {
+                         double dt, V3 &t, double &u, double &tl) {
     const double tol = PP ? 1.0e-7 : 1.0e-10;
     V3 old = v3(h.dx, h.dy, h.dz);
     t = cross(smulv(-1.0, n), cross(n, old));
@@ -1403,8 +1405,10 @@ DEV double spring_update(const DevLaw &c, HistRec &h, double fN, double d, doubl
     double dsmx = c.dsmxA * fN / (c.dsmxB * sqrt(R * d));
     double l2 = len(t);
     u = -1.0;
+    tl = -1.0;
     if (l2 < dsmx) {
         if (dsmx > tol) u = 1.0 - l2 / dsmx;
+        tl = l2;
     } else {
         if (l2 > tol)
             t = divv(smulv(dsmx, t), l2);
@@ -1421,11 +1425,11 @@ template <bool PP>
 DEV V3 tangential_law(const DevLaw &c, HistRec &h, double fN, double d, double R, V3 n, V3 vt,
                       double dt) {
     V3 t;
-    double u;
-    const double dsmx = spring_update<PP>(c, h, fN, d, R, n, vt, dt, t, u);
+    double u, tl;
+    const double dsmx = spring_update<PP>(c, h, fN, d, R, n, vt, dt, t, u, tl);
     const double ratio = u >= 0.0 ? 1.0 - pow15(u) : 1.0;
     V3 ft = v3(0.0, 0.0, 0.0);
-    double l3 = len(t);
+    const double l3 = tl >= 0.0 ? tl : len(t);
     if (l3 < dsmx) {
         if (l3 > 1.0e-10) ft = divv(smulv(-1.0 * c.sfrc * fN * ratio, t), l3);
     } else {
@@ -1470,8 +1474,8 @@ DEV void contact_mixed(const DevLaw &c, double rolling, double dt, HistRec &h, d
     const double cpvn = dot(cpv, n);
     const V3 vt = sub(cpv, smulv(cpvn, n));
     V3 t;
-    double u;
-    const double dsmx = spring_update<PP>(c, h, fN, d, R, n, vt, dt, t, u);
+    double u, tl;
+    const double dsmx = spring_update<PP>(c, h, fN, d, R, n, vt, dt, t, u, tl);
     float ratio = 1.0f;
     if (u >= 0.0) {
         const float uf = (float)u;
@@ -1480,7 +1484,7 @@ DEV void contact_mixed(const DevLaw &c, double rolling, double dt, HistRec &h, d
     const float fNf = (float)fN;
     const float sf = -(float)c.sfrc * fNf * ratio;
     float fx = 0.f, fy = 0.f, fz = 0.f;
-    const double l3 = len(t);
+    const double l3 = tl >= 0.0 ? tl : len(t);
     if (l3 < dsmx) {
         if (l3 > 1.0e-10) {
             const float k = sf / (float)l3;
@@ -1528,11 +1532,11 @@ DEV double fcap_f32(const DevInst &P, double S, double V, double effR) {
     S = fmax(P.gapmn, S);
     const float sv = (float)(S / sqrt(V));
     const float L = __logf((float)V), L2 = L * L, L3 = L2 * L;
-    const double *a = P.wc;
-    const float f1 = (float)a[0] + (float)a[1] * L + (float)a[2] * L2 + (float)a[3] * L3;
-    const float f2 = (float)a[4] + (float)a[5] * L + (float)a[6] * L2;
-    const float f3 = (float)a[7] + (float)a[8] * L + (float)a[9] * L2 + (float)a[10] * L3;
-    const float f4 = (float)a[11] + (float)a[12] * L + (float)a[13] * L2;
+    const float *a = P.wcf;  // the coefficients rounded to FP32 once, on the host
+    const float f1 = a[0] + a[1] * L + a[2] * L2 + a[3] * L3;
+    const float f2 = a[4] + a[5] * L + a[6] * L2;
+    const float f3 = a[7] + a[8] * L + a[9] * L2 + a[10] * L3;
+    const float f4 = a[11] + a[12] * L + a[13] * L2;
     const float lS = __logf(sv);
     const float e = __expf(f1 - f2 * __expf(f3 * lS + f4 * (lS * lS)));
     return (double)((float)(P.twoPI * effR * P.stension) * e);
