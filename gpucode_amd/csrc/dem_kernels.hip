@@ -1527,10 +1527,14 @@ DEV double fcap(const DevInst &P, double S, double V, double effR) {
 
 // The same force in FP32 for mixed precision: a force, stored nowhere and
 // deciding no branch (the bridge's existence and rupture are decided in
-// FP64 by the caller), from the same regression with hardware log and exp
-DEV double fcap_f32(const DevInst &P, double S, double V, double effR) {
+// FP64 by the caller), from the same regression with hardware log and exp;
+// the scaled gap S / sqrt(V) and the pair's effective radius 2 ri rj /
+// (ri + rj), which feed only this force, are formed in FP32 as well
+DEV double fcap_f32(const DevInst &P, double S, double V, double ri, double rj) {
     S = fmax(P.gapmn, S);
-    const float sv = (float)(S / sqrt(V));
+    const float sv = (float)S / sqrtf((float)V);
+    const float rif = (float)ri, rjf = (float)rj;
+    const float effR = 2.0f * rif * rjf / (rif + rjf);
     const float L = __logf((float)V), L2 = L * L, L3 = L2 * L;
     const float *a = P.wcf;  // the coefficients rounded to FP32 once, on the host
     const float f1 = a[0] + a[1] * L + a[2] * L2 + a[3] * L3;
@@ -1539,7 +1543,7 @@ DEV double fcap_f32(const DevInst &P, double S, double V, double effR) {
     const float f4 = a[11] + a[12] * L + a[13] * L2;
     const float lS = __logf(sv);
     const float e = __expf(f1 - f2 * __expf(f3 * lS + f4 * (lS * lS)));
-    return (double)((float)(P.twoPI * effR * P.stension) * e);
+    return (double)((float)P.twoPI * effR * (float)P.stension * e);
 }
 
 DEV double rupture(const DevInst &P, double V) {
@@ -1572,9 +1576,9 @@ DEV int bridge_slot(const DevInst &P, int fl, double gap, V3 nv, double ri, doub
     else if (brg && gap > rupture(P, V)) brg = 0;
     if (brg == 1) {
         const double Sp = gap < 0 ? 0.0 : gap;
-        const double effR = 2 * ri * rj / (ri + rj);
         V = V * 8.0;
-        const double fmag = MIXED ? fcap_f32(P, Sp, V, effR) : fcap(P, Sp, V, effR);
+        const double fmag =
+            MIXED ? fcap_f32(P, Sp, V, ri, rj) : fcap(P, Sp, V, 2 * ri * rj / (ri + rj));
         const V3 fc = smulv(-1.0 * fmag, nv);
         F = add(F, fc);
         nbridge = nbridge + 1;
