@@ -195,7 +195,8 @@ def c4_main(args):
     t0 = time.time()
     choices = None
     if args.c4_tune_steps > 0:
-        choices = G.tune_instances(ws, prec, steps=args.c4_tune_steps, log=log)
+        choices = G.tune_instances(ws, prec, steps=args.c4_tune_steps, log=log,
+                                   kernels=args.c4_kernels.split(","))
     t_tune = time.time() - t0
     b = G.Batch.from_instances(ws, choices, prec)
     b.set_schedule(device=True, slot_every=args.slot_every)
@@ -279,7 +280,7 @@ def c4_main(args):
                              "rebuild slots included"},
         "cpu_baseline": cpu,
         "tuning": {"seconds": t_tune, "steps_per_candidate": args.c4_tune_steps,
-                   "choices": kinds,
+                   "block_shapes_allowed": args.c4_kernels, "choices": kinds,
                    "ms_per_step_default_sum": sum(c["ms_default"] for c in choices) if choices else None,
                    "ms_per_step_tuned_sum": sum(c["ms_per_step"] for c in choices) if choices else None},
         "schedule": {"global_launches": tm["launches"] - tm0["launches"], "slots": sch["slots"],
@@ -486,6 +487,10 @@ def main():
                     help="instead of the 2M-sphere step: the batched uneven instances, "
                          "autotuned, stepped together (prints its own JSON line)")
     ap.add_argument("--c4-instances", type=int, default=256)
+    ap.add_argument("--c4-kernels", default="wave",
+                    help="block shapes the C4 autotuner may choose (comma-separated: loop, "
+                         "wave); one shape steps the batch in one launch per step (measured "
+                         "3 %% faster over the run than letting instances mix the two)")
     ap.add_argument("--c4-tune-steps", type=int, default=200,
                     help="steps per autotuner candidate (0: no tuning)")
     ap.add_argument("--c4-segment", type=int, default=5000,

@@ -71,15 +71,19 @@ def autotune(inp, pos, dia=None, vel=None, steps=200, cut_gaps=None, precision=A
 
 
 def tune_instances(instances, precision=A.PREC_MIXED, steps=200, cut_gaps=(0.1, 0.2, 0.3, 0.5),
-                   log=None):
+                   log=None, kernels=("loop", "wave")):
     """dem_autotune on each instance's own packing: every skin of cut_gaps
     (which sets the cell size and the list length) x block shape (the LOOP
     kernel, one thread per particle, or the WAVE kernel, a wave per chunk of
     64 with contact compaction) x storage order, timed for `steps` steps
     from the instance's start.  Returns one dict per instance: cutGap,
     kernel, morton, ms_per_step (the choice), ms_default (the instance's
-    own configuration)."""
+    own configuration).  kernels: the block shapes that may be chosen (a
+    batch whose instances all take one shape steps in one launch per step,
+    not two)."""
     from .workloads import min_cut_gap
+
+    kq_ok = tuple(q for q, name in ((0, "loop"), (2, "wave")) if name in kernels)
 
     out = []
     for k, w in enumerate(instances):
@@ -91,7 +95,7 @@ def tune_instances(instances, precision=A.PREC_MIXED, steps=200, cut_gaps=(0.1, 
                                cut_gaps=gaps, precision=precision)
         # a batch runs the LOOP and WAVE kernels; the COMPACT kernel (block
         # compaction) is a single-instance form, so it is not a candidate
-        cand = [(table[s, kq, mo], s, kq, mo) for s in range(len(gaps)) for kq in (0, 2)
+        cand = [(table[s, kq, mo], s, kq, mo) for s in range(len(gaps)) for kq in kq_ok
                 for mo in (0, 1) if table[s, kq, mo] >= 0]
         ms, s, kq, mo = min(cand)
         out.append(dict(cutGap=gaps[s], kernel=(Engine.KERNEL_LOOP, None, Engine.KERNEL_WAVE)[kq],
