@@ -196,7 +196,8 @@ def c4_main(args):
     choices = None
     if args.c4_tune_steps > 0:
         choices = G.tune_instances(ws, prec, steps=args.c4_tune_steps, log=log,
-                                   kernels=args.c4_kernels.split(","))
+                                   kernels=args.c4_kernels.split(","),
+                                   cut_gaps=tuple(float(g) for g in args.c4_skins.split(",")))
     t_tune = time.time() - t0
     b = G.Batch.from_instances(ws, choices, prec)
     b.set_schedule(device=True, slot_every=args.slot_every)
@@ -487,6 +488,8 @@ def main():
                     help="instead of the 2M-sphere step: the batched uneven instances, "
                          "autotuned, stepped together (prints its own JSON line)")
     ap.add_argument("--c4-instances", type=int, default=256)
+    ap.add_argument("--c4-skins", default="0.1,0.2,0.3,0.5",
+                    help="candidate skins (cutGap) of the C4 autotuner, comma-separated")
     ap.add_argument("--c4-kernels", default="wave",
                     help="block shapes the C4 autotuner may choose (comma-separated: loop, "
                          "wave); one shape steps the batch in one launch per step (measured "
