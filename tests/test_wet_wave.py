@@ -202,3 +202,15 @@ def test_wave_split_changes_nothing(case):
         assert np.array_equal(r[3], runs[0][3]) and r[4] == runs[0][4]
     if case == "dry_pairs":
         assert runs[0][5]  # the shared pair records were in use
+
+
+def test_wave_split_refuses_other_shapes():
+    w = W.small_bed(n=200, R_d=4.0)
+    e = G.Engine(w["inp"])
+    for bad in (-1, 3, 8, 64):
+        with pytest.raises(G.DemError) as err:
+            e.set_wave_split(bad)
+        assert err.value.code == -1  # DEM_ERR_ARG
+    for ok in (0, 1, 2, 4):
+        e.set_wave_split(ok)
+    e.close()
