@@ -200,7 +200,8 @@ static int build_tasks(dem_batch *b) {
         for (int k : order) {
             const bool wave = kern[k] == DEM_KERNEL_WAVE;
             if (wave != (pass == 0)) continue;
-            const int nblk = (tot[k] + 255) / 256;
+            const int bs = wave ? WVB_BS : LPB_BS;
+            const int nblk = (tot[k] + bs - 1) / bs;
             for (int q = 0; q < nblk; ++q) t.push_back(make_int2(k, q));
         }
         if (pass == 0) nw = (int)t.size();

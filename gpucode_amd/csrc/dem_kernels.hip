@@ -2725,22 +2725,24 @@ DEV bool batch_step_args(const BatchInst &B, int step, StepArgs &A, int &stop) {
     return true;
 }
 
-// one block of WV_BS particles of a batched instance, dry or wet
+// one block of WVB_BS particles of a batched instance, dry or wet: one wave
+// per block, so a wave slot that frees up takes the next chunk at once (as
+// in the single-instance launch)
 template <bool MIXED>
 DEV void step_wave_batched(const DevInst &P, const StepArgs &A, unsigned blk, int stop) {
-    __shared__ double res_all[WV_BS / 64][10][64];
-    __shared__ WvStash stash[WV_BS / 64];
-    __shared__ WvWet wet[WV_BS / 64];
+    __shared__ double res_all[WVB_BS / 64][10][64];
+    __shared__ WvStash stash[WVB_BS / 64];
+    __shared__ WvWet wet[WVB_BS / 64];
     const int w = threadIdx.x >> 6;
     if (P.forceCap == 1)
-        step_wave_block<MIXED, false, WV_BS, true>(P, A, blk, res_all[w], &stash[w], &wet[w], 0, 64,
-                                                   stop);
+        step_wave_block<MIXED, false, WVB_BS, true>(P, A, blk, res_all[w], &stash[w], &wet[w], 0, 64,
+                                                    stop);
     else
-        step_wave_block<MIXED, false>(P, A, blk, res_all[w], &stash[w], nullptr, 0, 64, stop);
+        step_wave_block<MIXED, false, WVB_BS>(P, A, blk, res_all[w], &stash[w], nullptr, 0, 64, stop);
 }
 
 template <bool MIXED>
-__global__ void __launch_bounds__(WV_BS) k_step_batch_wave(const DevInst *insts, const BatchInst *tab,
+__global__ void __launch_bounds__(WVB_BS) k_step_batch_wave(const DevInst *insts, const BatchInst *tab,
                                                            const int2 *tasks, int step) {
     const int2 tk = tasks[xcd_block(gridDim.x)];
     StepArgs A;
@@ -2768,10 +2770,10 @@ void launch_step_batch(const DevInst *insts, const BatchInst *tab, const int2 *t
     if (ntasks <= 0) return;
     if (wave) {
         if (mixed)
-            hipLaunchKernelGGL((k_step_batch_wave<true>), dim3(ntasks), dim3(WV_BS), 0, s, insts, tab,
+            hipLaunchKernelGGL((k_step_batch_wave<true>), dim3(ntasks), dim3(WVB_BS), 0, s, insts, tab,
                                tasks, step);
         else
-            hipLaunchKernelGGL((k_step_batch_wave<false>), dim3(ntasks), dim3(WV_BS), 0, s, insts, tab,
+            hipLaunchKernelGGL((k_step_batch_wave<false>), dim3(ntasks), dim3(WVB_BS), 0, s, insts, tab,
                                tasks, step);
     } else {
         if (mixed)
@@ -2817,7 +2819,7 @@ DEV bool dbatch_step_args(const DBatchInst &D, const DBatchCtl &C, int g, StepAr
 }
 
 template <bool MIXED>
-__global__ void __launch_bounds__(WV_BS) k_step_dbatch_wave(const DevInst *insts, const DBatchInst *dins,
+__global__ void __launch_bounds__(WVB_BS) k_step_dbatch_wave(const DevInst *insts, const DBatchInst *dins,
                                                             const DBatchCtl *ctl, const int2 *tasks,
                                                             int g) {
     const int2 tk = tasks[xcd_block(gridDim.x)];
@@ -2847,10 +2849,10 @@ void launch_step_dbatch(const DevInst *insts, const DBatchInst *dins, const DBat
     if (ntasks <= 0) return;
     if (wave) {
         if (mixed)
-            hipLaunchKernelGGL((k_step_dbatch_wave<true>), dim3(ntasks), dim3(WV_BS), 0, s, insts, dins,
+            hipLaunchKernelGGL((k_step_dbatch_wave<true>), dim3(ntasks), dim3(WVB_BS), 0, s, insts, dins,
                                ctl, tasks, g);
         else
-            hipLaunchKernelGGL((k_step_dbatch_wave<false>), dim3(ntasks), dim3(WV_BS), 0, s, insts,
+            hipLaunchKernelGGL((k_step_dbatch_wave<false>), dim3(ntasks), dim3(WVB_BS), 0, s, insts,
                                dins, ctl, tasks, g);
     } else {
         if (mixed)

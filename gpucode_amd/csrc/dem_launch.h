@@ -67,6 +67,10 @@ void launch_pair_unpack(const double4 *p, const int *nnb, const unsigned long lo
 void launch_analyze(const DevInst *insts, const AnaArgs *args, double *out, int n, hipStream_t s);
 void launch_rebuild_batch(const DevInst *insts, const RbInst *tab, int ninst, const int2 *ptasks,
                           int nptasks, const int2 *ctasks, int nctasks, hipStream_t s);
+// particles per task of the batched step kernels: a wave per task for the
+// wave kernel, 256 threads for the loop kernel
+#define WVB_BS 64
+#define LPB_BS 256
 void launch_step_batch(const DevInst *insts, const BatchInst *tab, const int2 *tasks, int ntasks,
                        bool wave, bool mixed, int step, hipStream_t s);
 void launch_pp_force(const DevInst &P, const StepArgs &A, bool wet, bool mixed, hipStream_t s);
