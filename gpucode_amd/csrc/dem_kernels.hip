@@ -35,6 +35,17 @@ DEV unsigned xcd_block(unsigned nb) {
     return start + k;
 }
 
+// the same map, each XCD's range walked backwards when rev (odd steps of
+// the batched wave kernel: a launch starts on the tasks the one before
+// finished last, whose data are still in L2)
+DEV unsigned xcd_block_dir(unsigned nb, bool rev) {
+    const unsigned b = blockIdx.x;
+    if (nb < 64) return rev ? nb - 1u - b : b;
+    const unsigned per = nb / 8, rem = nb % 8, x = b % 8, k = b / 8;
+    const unsigned start = x * per + (x < rem ? x : rem), cnt = per + (x < rem ? 1u : 0u);
+    return rev ? start + cnt - 1u - k : start + k;
+}
+
 __global__ void k_fill_i32(int *p, int v, long n) {
     long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
@@ -2744,7 +2755,7 @@ DEV void step_wave_batched(const DevInst &P, const StepArgs &A, unsigned blk, in
 template <bool MIXED>
 __global__ void __launch_bounds__(WVB_BS) k_step_batch_wave(const DevInst *insts, const BatchInst *tab,
                                                            const int2 *tasks, int step) {
-    const int2 tk = tasks[xcd_block(gridDim.x)];
+    const int2 tk = tasks[xcd_block_dir(gridDim.x, step & 1)];
     StepArgs A;
     int stop;
     if (!batch_step_args(tab[tk.x], step, A, stop)) return;
@@ -2822,7 +2833,7 @@ template <bool MIXED>
 __global__ void __launch_bounds__(WVB_BS) k_step_dbatch_wave(const DevInst *insts, const DBatchInst *dins,
                                                             const DBatchCtl *ctl, const int2 *tasks,
                                                             int g) {
-    const int2 tk = tasks[xcd_block(gridDim.x)];
+    const int2 tk = tasks[xcd_block_dir(gridDim.x, g & 1)];
     StepArgs A;
     int stop;
     if (!dbatch_step_args(dins[tk.x], ctl[tk.x], g, A, stop)) return;
