@@ -3211,13 +3211,18 @@ __global__ void __launch_bounds__(RB_BS) kb_cell_range(const DevInst *insts, con
     if (k < R.total) cell_range_one(R.keys, R.pb, R.lvlb, insts[R.inst], R.cstart, R.cend, R.total, k);
 }
 
-// four blocks of 1024 lanes per task of 256 particles, 16 lanes per particle
-#define KBL_BS 1024
+// one-wave blocks, 16 lanes per particle: 64 blocks per task of 256
+// particles.  The slots run beside the step kernel, whose waves hold most of
+// every SIMD's registers; a block of one wave starts wherever one wave fits,
+// where a block of 1024 lanes would wait for four free wave slots on every
+// SIMD of one CU.
+#define KBL_BS 64
+#define KBL_PER (RB_BS * 16 / KBL_BS)
 __global__ void __launch_bounds__(KBL_BS) kb_lists(const DevInst *insts, const RbInst *tab,
                                                    const int2 *tasks) {
-    const int2 tk = tasks[blockIdx.x >> 2];
+    const int2 tk = tasks[blockIdx.x / KBL_PER];
     const RbInst &R = tab[tk.x];
-    const int first = tk.y * RB_BS + (blockIdx.x & 3) * (KBL_BS / 16);
+    const int first = tk.y * RB_BS + (int)(blockIdx.x % KBL_PER) * (KBL_BS / 16);
     if (first >= R.total) return;  // the whole block: no lane is needed for a shuffle
     const int i = first + (int)(threadIdx.x >> 4), g = threadIdx.x & 15;
     const bool valid = i < R.total;
@@ -3282,7 +3287,7 @@ void launch_rebuild_batch(const DevInst *insts, const RbInst *tab, int ninst, co
     if (nctasks > 0) hipLaunchKernelGGL(kb_fill_cells, dim3(nctasks), dim3(RB_BS), 0, s, tab, ctasks);
     if (nptasks > 0) {
         hipLaunchKernelGGL(kb_cell_range, dim3(nptasks), dim3(RB_BS), 0, s, insts, tab, ptasks);
-        hipLaunchKernelGGL(kb_lists, dim3(4 * nptasks), dim3(KBL_BS), 0, s, insts, tab, ptasks);
+        hipLaunchKernelGGL(kb_lists, dim3(KBL_PER * nptasks), dim3(KBL_BS), 0, s, insts, tab, ptasks);
         hipLaunchKernelGGL(kb_active, dim3(nptasks), dim3(RB_BS), 0, s, insts, tab, ptasks);
         hipLaunchKernelGGL(kb_actoff, dim3(ninst), dim3(SCAN_BS), 0, s, tab);
         hipLaunchKernelGGL(kb_hist_remap, dim3(nptasks), dim3(RB_BS), 0, s, tab, ptasks);
