@@ -3103,12 +3103,21 @@ void launch_analyze(const DevInst *insts, const AnaArgs *args, double *out, int 
 // ---------------------------------------------------------------------
 #define RB_BS 256
 #define BSORT_BS 1024
+// The particle and cell stages run in one-wave blocks, KB_PER per task of
+// RB_BS: beside the step kernel a block of one wave starts wherever one
+// wave fits.
+#define KB_WV 64
+#define KB_PER (RB_BS / KB_WV)
 
-__global__ void __launch_bounds__(RB_BS) kb_calc_hash(const DevInst *insts, const RbInst *tab,
+DEV int kb_item(const int2 tk) {
+    return tk.y * RB_BS + (int)(blockIdx.x % KB_PER) * KB_WV + (int)threadIdx.x;
+}
+
+__global__ void __launch_bounds__(KB_WV) kb_calc_hash(const DevInst *insts, const RbInst *tab,
                                                       const int2 *tasks) {
-    const int2 tk = tasks[blockIdx.x];
+    const int2 tk = tasks[blockIdx.x / KB_PER];
     const RbInst &R = tab[tk.x];
-    const int i = tk.y * RB_BS + threadIdx.x;
+    const int i = kb_item(tk);
     if (i < R.total) calc_hash_one(insts[R.inst], R.pa, R.lvla, R.ida, R.keys, R.vals, i);
 }
 
@@ -3180,10 +3189,10 @@ __global__ void __launch_bounds__(BSORT_BS) kb_sort(const RbInst *tab) {
     }
 }
 
-__global__ void __launch_bounds__(RB_BS) kb_permute(const RbInst *tab, const int2 *tasks) {
-    const int2 tk = tasks[blockIdx.x];
+__global__ void __launch_bounds__(KB_WV) kb_permute(const RbInst *tab, const int2 *tasks) {
+    const int2 tk = tasks[blockIdx.x / KB_PER];
     const RbInst &R = tab[tk.x];
-    const int i = tk.y * RB_BS + threadIdx.x;
+    const int i = kb_item(tk);
     if (i >= R.total) return;
     StateBuf a, b;
     a.p = (double4 *)R.pa; a.v = (double4 *)R.va; a.lqd = (int *)R.lqa;
@@ -3192,22 +3201,22 @@ __global__ void __launch_bounds__(RB_BS) kb_permute(const RbInst *tab, const int
 }
 
 // empty cells and a cleared overflow flag; tasks over blocks of cells
-__global__ void __launch_bounds__(RB_BS) kb_fill_cells(const RbInst *tab, const int2 *tasks) {
-    const int2 tk = tasks[blockIdx.x];
+__global__ void __launch_bounds__(KB_WV) kb_fill_cells(const RbInst *tab, const int2 *tasks) {
+    const int2 tk = tasks[blockIdx.x / KB_PER];
     const RbInst &R = tab[tk.x];
-    const int c = tk.y * RB_BS + threadIdx.x;
+    const int c = kb_item(tk);
     if (c < R.ncells) {
         R.cstart[c] = DEM_EMPTY;
         R.cend[c] = DEM_EMPTY;
     }
-    if (tk.y == 0 && threadIdx.x == 0 && R.total > 0) *R.overflow = 0;
+    if (c == 0 && R.total > 0) *R.overflow = 0;
 }
 
-__global__ void __launch_bounds__(RB_BS) kb_cell_range(const DevInst *insts, const RbInst *tab,
+__global__ void __launch_bounds__(KB_WV) kb_cell_range(const DevInst *insts, const RbInst *tab,
                                                        const int2 *tasks) {
-    const int2 tk = tasks[blockIdx.x];
+    const int2 tk = tasks[blockIdx.x / KB_PER];
     const RbInst &R = tab[tk.x];
-    const int k = tk.y * RB_BS + threadIdx.x;
+    const int k = kb_item(tk);
     if (k < R.total) cell_range_one(R.keys, R.pb, R.lvlb, insts[R.inst], R.cstart, R.cend, R.total, k);
 }
 
@@ -3232,11 +3241,11 @@ __global__ void __launch_bounds__(KBL_BS) kb_lists(const DevInst *insts, const R
     if (valid && g == 1) wall_list_one(R.pb, P, R.Tw, R.wl, R.nwl, R.cap, i);
 }
 
-__global__ void __launch_bounds__(RB_BS) kb_active(const DevInst *insts, const RbInst *tab,
+__global__ void __launch_bounds__(KB_WV) kb_active(const DevInst *insts, const RbInst *tab,
                                                   const int2 *tasks) {
-    const int2 tk = tasks[blockIdx.x];
+    const int2 tk = tasks[blockIdx.x / KB_PER];
     const RbInst &R = tab[tk.x];
-    const int i = tk.y * RB_BS + threadIdx.x;
+    const int i = kb_item(tk);
     if (i < R.total)
         active_slots_one(R.pb, R.nbr, R.nnb, R.amask, R.keys2, R.nmask,
                          near_gap_of(insts[R.inst]), R.cap, i);
@@ -3267,10 +3276,10 @@ __global__ void __launch_bounds__(SCAN_BS) kb_actoff(const RbInst *tab) {
     }
 }
 
-__global__ void __launch_bounds__(RB_BS) kb_hist_remap(const RbInst *tab, const int2 *tasks) {
-    const int2 tk = tasks[blockIdx.x];
+__global__ void __launch_bounds__(KB_WV) kb_hist_remap(const RbInst *tab, const int2 *tasks) {
+    const int2 tk = tasks[blockIdx.x / KB_PER];
     const RbInst &R = tab[tk.x];
-    const int i = tk.y * RB_BS + threadIdx.x;
+    const int i = kb_item(tk);
     if (i >= R.total) return;
     hist_remap_one(R.vals, R.idb, R.ida, R.nbr, R.nnb, R.amask, R.actoff, R.nbr_old, R.nnb_old,
                    R.amask_old, R.actoff_old, R.hp, R.hp_old, R.cap, (long)R.K * R.cap, R.old_total, i);
@@ -3280,17 +3289,17 @@ __global__ void __launch_bounds__(RB_BS) kb_hist_remap(const RbInst *tab, const 
 void launch_rebuild_batch(const DevInst *insts, const RbInst *tab, int ninst, const int2 *ptasks,
                           int nptasks, const int2 *ctasks, int nctasks, hipStream_t s) {
     if (ninst <= 0) return;
-    if (nptasks > 0)
-        hipLaunchKernelGGL(kb_calc_hash, dim3(nptasks), dim3(RB_BS), 0, s, insts, tab, ptasks);
+    const dim3 pg(KB_PER * nptasks), cg(KB_PER * nctasks), wv(KB_WV);
+    if (nptasks > 0) hipLaunchKernelGGL(kb_calc_hash, pg, wv, 0, s, insts, tab, ptasks);
     hipLaunchKernelGGL(kb_sort, dim3(ninst), dim3(BSORT_BS), 0, s, tab);
-    if (nptasks > 0) hipLaunchKernelGGL(kb_permute, dim3(nptasks), dim3(RB_BS), 0, s, tab, ptasks);
-    if (nctasks > 0) hipLaunchKernelGGL(kb_fill_cells, dim3(nctasks), dim3(RB_BS), 0, s, tab, ctasks);
+    if (nptasks > 0) hipLaunchKernelGGL(kb_permute, pg, wv, 0, s, tab, ptasks);
+    if (nctasks > 0) hipLaunchKernelGGL(kb_fill_cells, cg, wv, 0, s, tab, ctasks);
     if (nptasks > 0) {
-        hipLaunchKernelGGL(kb_cell_range, dim3(nptasks), dim3(RB_BS), 0, s, insts, tab, ptasks);
+        hipLaunchKernelGGL(kb_cell_range, pg, wv, 0, s, insts, tab, ptasks);
         hipLaunchKernelGGL(kb_lists, dim3(KBL_PER * nptasks), dim3(KBL_BS), 0, s, insts, tab, ptasks);
-        hipLaunchKernelGGL(kb_active, dim3(nptasks), dim3(RB_BS), 0, s, insts, tab, ptasks);
+        hipLaunchKernelGGL(kb_active, pg, wv, 0, s, insts, tab, ptasks);
         hipLaunchKernelGGL(kb_actoff, dim3(ninst), dim3(SCAN_BS), 0, s, tab);
-        hipLaunchKernelGGL(kb_hist_remap, dim3(nptasks), dim3(RB_BS), 0, s, tab, ptasks);
+        hipLaunchKernelGGL(kb_hist_remap, pg, wv, 0, s, tab, ptasks);
     }
 }
 
